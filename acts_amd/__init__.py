@@ -1,0 +1,18 @@
+"""acts_amd -- MI355X-native batched Kalman track fitter.
+
+Drop-in for the reference path Acts::KalmanFitter::fit with
+Propagator<EigenStepper<>, DirectNavigator>, GainMatrixUpdater and
+GainMatrixSmoother, run as hand-written HIP kernels (one lane per track)
+behind the C ABI of include/acts_kf.h.
+"""
+from . import capi  # noqa: F401
+from .detector import (ConstantField, Detector, barrel_detector,  # noqa: F401
+                       barrel_endcap_detector)
+from .events import Events, generate, gen_config, workload  # noqa: F401
+from .fitter import FitResult, KalmanFitter, KalmanFitterOptions  # noqa: F401
+
+__all__ = [
+    "capi", "ConstantField", "Detector", "barrel_detector", "barrel_endcap_detector",
+    "Events", "generate", "gen_config", "workload", "FitResult", "KalmanFitter",
+    "KalmanFitterOptions",
+]

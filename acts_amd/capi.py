@@ -1,0 +1,230 @@
+"""ctypes mirror of include/acts_kf.h and include/acts_kf_gen.h.
+
+The native library is loaded from the package directory; it must have been
+built beforehand (``python -m acts_amd.build``).  Nothing here falls back to
+a CPU implementation: a missing library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libacts_kf.so")
+
+# ---- constants (acts_kf.h) -------------------------------------------------
+OK = 0
+ERR_INVALID_ARGUMENT = -1
+ERR_DEVICE = -2
+ERR_OUT_OF_MEMORY = -3
+ERR_UNSUPPORTED = -4
+
+CAT_KALMAN, CAT_PROPAGATOR, CAT_STEPPER, CAT_SURFACE, CAT_FIELD = (
+    0x100, 0x200, 0x300, 0x400, 0x500)
+STATUS_OK = 0
+STATUS_UPDATE_FAILED = CAT_KALMAN | 1
+STATUS_SMOOTH_FAILED = CAT_KALMAN | 2
+STATUS_NO_MEASUREMENT_FOUND = CAT_KALMAN | 4
+STATUS_PROPAGATION_FAILURE = CAT_PROPAGATOR | 1
+STATUS_STEP_COUNT_LIMIT = CAT_PROPAGATOR | 2
+STATUS_NEXT_TARGET_LIMIT = CAT_PROPAGATOR | 3
+STATUS_STEP_SIZE_STALLED = CAT_STEPPER | 1
+STATUS_STEP_INVALID = CAT_STEPPER | 2
+STATUS_STEP_ADJUSTMENT_FAILED = CAT_STEPPER | 3
+STATUS_NOT_ON_SURFACE = CAT_SURFACE | 1
+STATUS_FIELD_OUT_OF_BOUNDS = CAT_FIELD | 1
+
+STATE_HAS_MEASUREMENT = 1 << 0
+STATE_HAS_PARAMETERS = 1 << 1
+STATE_IS_OUTLIER = 1 << 2
+STATE_IS_HOLE = 1 << 3
+STATE_HAS_MATERIAL = 1 << 4
+
+SURFACE_PLANE, SURFACE_CYLINDER, SURFACE_DISC = 0, 1, 2
+SURFACE_SENSITIVE = 1
+
+FIELD_CONSTANT, FIELD_GRID_XYZ = 0, 1
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+
+class Surface(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("material", C.c_int32),
+        ("flags", C.c_uint32),
+        ("reserved", C.c_int32),
+        ("rotation", C.c_double * 9),
+        ("translation", C.c_double * 3),
+        ("bounds", C.c_double * 2),
+    ]
+
+
+class MaterialSlab(C.Structure):
+    _fields_ = [
+        ("x0", C.c_float),
+        ("l0", C.c_float),
+        ("ar", C.c_float),
+        ("z", C.c_float),
+        ("molar_rho", C.c_float),
+        ("thickness", C.c_float),
+    ]
+
+
+class Field(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("reserved", C.c_int32),
+        ("constant", C.c_double * 3),
+        ("n", C.c_int32 * 3),
+        ("reserved2", C.c_int32),
+        ("point_min", C.c_double * 3),
+        ("point_max", C.c_double * 3),
+        ("values", C.POINTER(C.c_double)),
+    ]
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("multiple_scattering", C.c_int32),
+        ("energy_loss", C.c_int32),
+        ("use_joseph", C.c_int32),
+        ("precision", C.c_int32),
+        ("outlier_chi2_cut", C.c_double),
+        ("step_tolerance", C.c_double),
+        ("step_size_cutoff", C.c_double),
+        ("initial_step_size", C.c_double),
+        ("max_step_size", C.c_double),
+        ("max_runge_kutta_step_trials", C.c_uint32),
+        ("max_steps", C.c_uint32),
+        ("max_target_skipping", C.c_uint32),
+        ("loop_protection", C.c_int32),
+        ("path_limit", C.c_double),
+        ("loop_fraction", C.c_double),
+        ("surface_tolerance", C.c_double),
+        ("near_limit", C.c_double),
+        ("abs_pdg", C.c_int32),
+        ("mass", C.c_float),
+        ("abs_charge", C.c_float),
+        ("reserved", C.c_int32),
+    ]
+
+
+_P = C.c_void_p
+
+
+class BatchIn(C.Structure):
+    _fields_ = [
+        ("n_tracks", C.c_int64),
+        ("start_params", _P),
+        ("start_cov", _P),
+        ("start_plane", _P),
+        ("seq_offsets", _P),
+        ("seq_surfaces", _P),
+        ("meas_offsets", _P),
+        ("meas_surfaces", _P),
+        ("meas_values", _P),
+        ("meas_cov", _P),
+        ("max_sequence_length", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+BATCH_OUT_FIELDS = [
+    "status", "n_states", "chi2", "ndf", "n_measurements", "n_holes",
+    "n_outliers", "n_steps", "n_step_trials", "state_surface", "state_flags",
+    "state_path", "state_chi2", "predicted", "predicted_cov", "filtered",
+    "filtered_cov", "smoothed", "smoothed_cov", "jacobian",
+]
+
+
+class BatchOut(C.Structure):
+    _fields_ = [(name, _P) for name in BATCH_OUT_FIELDS]
+
+
+class GenConfig(C.Structure):
+    _fields_ = [
+        ("n_tracks", C.c_int64),
+        ("seed", C.c_uint64),
+        ("pt_min", C.c_double),
+        ("pt_max", C.c_double),
+        ("pt_log_uniform", C.c_int32),
+        ("n_threads", C.c_int32),
+        ("eta_min", C.c_double),
+        ("eta_max", C.c_double),
+        ("phi_min", C.c_double),
+        ("phi_max", C.c_double),
+        ("vertex_sigma_z", C.c_double),
+        ("sigma_loc0", C.c_double),
+        ("sigma_loc1", C.c_double),
+        ("start_sigma", C.c_double * 6),
+        ("hole_probability", C.c_double),
+        ("outlier_fraction", C.c_double),
+        ("outlier_min", C.c_int32),
+        ("outlier_max", C.c_int32),
+        ("outlier_distance", C.c_double),
+        ("min_measurements", C.c_int32),
+        ("max_attempts", C.c_int32),
+    ]
+
+
+# symbol -> (restype, argtypes); every declaration in include/*.h
+SIGNATURES = {
+    "acts_kf_default_options": (None, [C.POINTER(Options)]),
+    "acts_kf_create": (C.c_int, [C.POINTER(Surface), C.c_int32, C.POINTER(MaterialSlab),
+                                 C.c_int32, C.POINTER(Field), C.POINTER(Options),
+                                 C.c_int32, C.POINTER(C.c_void_p)]),
+    "acts_kf_fit_batch": (C.c_int, [C.c_void_p, C.POINTER(BatchIn), C.POINTER(BatchOut),
+                                    C.c_void_p]),
+    "acts_kf_fit_batch_device": (C.c_int, [C.c_void_p, C.POINTER(BatchIn),
+                                           C.POINTER(BatchOut), C.c_void_p]),
+    "acts_kf_sync": (C.c_int, [C.c_void_p]),
+    "acts_kf_destroy": (None, [C.c_void_p]),
+    "acts_kf_status_string": (C.c_char_p, [C.c_int32]),
+    "acts_kf_last_error": (C.c_char_p, []),
+    "acts_kf_gen_run": (C.c_int, [C.POINTER(GenConfig), C.POINTER(Surface), C.c_int32,
+                                  C.POINTER(MaterialSlab), C.c_int32, C.POINTER(Field),
+                                  C.POINTER(Options), C.POINTER(C.c_void_p)]),
+    "acts_kf_gen_sizes": (None, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                 C.POINTER(C.c_int64)]),
+    "acts_kf_gen_fetch": (None, [C.c_void_p] + [C.c_void_p] * 12),
+    "acts_kf_gen_free": (None, [C.c_void_p]),
+}
+
+_lib = None
+
+
+def load_library(path: str = LIB_PATH) -> C.CDLL:
+    """Load libacts_kf.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} not found: build it with `python -m acts_amd.build` "
+            "(there is no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def default_options() -> Options:
+    o = Options()
+    load_library().acts_kf_default_options(C.byref(o))
+    return o
+
+
+def last_error() -> str:
+    return load_library().acts_kf_last_error().decode()
+
+
+def status_string(code: int) -> str:
+    return load_library().acts_kf_status_string(int(code)).decode()
+
+
+def check(code: int, what: str) -> None:
+    if code != OK:
+        raise RuntimeError(f"{what} failed ({code}): {last_error()}")

@@ -1,0 +1,479 @@
+// kf_capi.hip -- the C ABI of include/acts_kf.h: context set-up, device
+// memory, kernel launches.  Replaces the construction and the fit() call of
+// Acts::KalmanFitter<Propagator<EigenStepper<>, DirectNavigator>, traj>
+// (KalmanFitter.hpp:265-270, 721-731) for a whole batch of tracks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kf_kernel.hpp"
+
+using namespace kf;
+
+namespace {
+
+thread_local std::string g_lastError;
+
+int fail(int code, const std::string& msg) {
+  g_lastError = msg;
+  return code;
+}
+
+#define KF_HIP(call)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return fail(ACTS_KF_ERR_DEVICE,                                         \
+                  std::string(#call) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+constexpr int kBlock = 64;
+
+__global__ __launch_bounds__(kBlock) void fitKernel(
+    DevBatch in, DevOut out, DevScratch sc, const DevSurface* __restrict__ geo,
+    const DevSlab* __restrict__ slabs, DevField F, DevOpts o, PhysConst pc,
+    int32_t nSurfaces) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= in.n) return;
+  fitOneTrack(t, in, out, sc, geo, slabs, F, o, pc, nSurfaces);
+}
+
+__global__ void maxLengthKernel(const int64_t* off, int64_t n, int32_t* result) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int32_t len = static_cast<int32_t>(off[t + 1] - off[t]);
+  atomicMax(result, len);
+}
+
+// Device buffer that only grows.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t need) {
+    if (need <= bytes) return 0;
+    if (p) {
+      if (hipFree(p) != hipSuccess) return -1;
+      p = nullptr;
+      bytes = 0;
+    }
+    if (need == 0) return 0;
+    if (hipMalloc(&p, need) != hipSuccess) return -1;
+    bytes = need;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+};
+
+PhysConst makePhysConst() {
+  PhysConst c;
+  const long double MeV = 1e-3L;
+  c.me = static_cast<float>(static_cast<double>(1e-3 * 0.5109989461L));
+  c.k = static_cast<float>(static_cast<double>(1e-3 * 30.7075L) * 1.0);
+  c.plasmaScale = static_cast<float>(static_cast<double>(1e-9 * 28.816L));
+  c.pad = 0.f;
+  c.highland = static_cast<double>(1e-3 * 13.6L);
+  c.rossiGreisen = static_cast<double>(1e-3 * 17.5L);
+  c.minP = 10 * 1e-3;
+  (void)MeV;
+  return c;
+}
+
+}  // namespace
+
+struct acts_kf_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int32_t nSurfaces = 0;
+  DevSurface* geo = nullptr;
+  DevSlab* slabs = nullptr;
+  DevField field{};
+  DevOpts opts{};
+  PhysConst pc{};
+  DevBuf scratchD, scratchI, lenBuf;
+  // staging for the host-buffer entry point
+  DevBuf inBuf, outBuf;
+};
+
+extern "C" {
+
+void acts_kf_default_options(acts_kf_options* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->multiple_scattering = 1;
+  o->energy_loss = 1;
+  o->use_joseph = 0;
+  o->precision = ACTS_KF_PRECISION_FP64;
+  o->outlier_chi2_cut = 0.0;
+  o->step_tolerance = 1e-4;
+  o->step_size_cutoff = 0.0;
+  o->initial_step_size = 10000.0;
+  o->max_step_size = DBL_MAX;
+  o->max_runge_kutta_step_trials = 10000;
+  o->max_steps = 1000;
+  o->max_target_skipping = 100;
+  o->loop_protection = 1;
+  o->path_limit = DBL_MAX;
+  o->loop_fraction = 0.5;
+  o->surface_tolerance = 1e-4;
+  o->near_limit = -100 * 1e-3;
+  o->abs_pdg = 13;
+  o->mass = 0.1056583755f;
+  o->abs_charge = 1.0f;
+}
+
+const char* acts_kf_last_error(void) { return g_lastError.c_str(); }
+
+const char* acts_kf_status_string(int32_t s) {
+  switch (s) {
+    case ACTS_KF_STATUS_OK: return "success";
+    case ACTS_KF_STATUS_UPDATE_FAILED: return "Kalman update failed";
+    case ACTS_KF_STATUS_SMOOTH_FAILED: return "Kalman smooth failed";
+    case ACTS_KF_CAT_KALMAN | 3: return "Kalman output conversion failed";
+    case ACTS_KF_STATUS_NO_MEASUREMENT_FOUND:
+      return "No measurement detected during the propagation";
+    case ACTS_KF_CAT_KALMAN | 5: return "Reverse propagation failed";
+    case ACTS_KF_CAT_KALMAN | 6:
+      return "Inconsistent reference surfaces between forward and reversed filtered tracks";
+    case ACTS_KF_STATUS_PROPAGATION_FAILURE: return "Propagation failed";
+    case ACTS_KF_STATUS_STEP_COUNT_LIMIT:
+      return "Propagation reached the configured maximum number of steps";
+    case ACTS_KF_STATUS_NEXT_TARGET_LIMIT:
+      return "Propagation reached the configured maximum number of next target calls";
+    case ACTS_KF_STATUS_STEP_SIZE_STALLED: return "Step size fell below minimum threshold";
+    case ACTS_KF_STATUS_STEP_INVALID: return "Step was invalid";
+    case ACTS_KF_STATUS_STEP_ADJUSTMENT_FAILED:
+      return "Step size adjustment exceeds maximum trials";
+    case ACTS_KF_STATUS_NOT_ON_SURFACE:
+      return "Global to local transformation failed: position not on surface.";
+    case ACTS_KF_STATUS_FIELD_OUT_OF_BOUNDS:
+      return "The lookup position was outside of the validity domain of the field";
+    default: return "unknown";
+  }
+}
+
+int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
+                   const acts_kf_material_slab* materials, int32_t nMaterials,
+                   const acts_kf_field* field, const acts_kf_options* options,
+                   int32_t device, acts_kf_ctx** outCtx) {
+  if (!outCtx || !surfaces || nSurfaces <= 0 || !field || !options)
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: null or empty argument");
+  if (nMaterials > 0 && !materials)
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: materials is null");
+  if (field->kind != ACTS_KF_FIELD_CONSTANT)
+    return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: field kind not supported");
+  if (options->precision != ACTS_KF_PRECISION_FP64)
+    return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: precision mode not supported");
+  for (int32_t i = 0; i < nSurfaces; ++i) {
+    const acts_kf_surface& s = surfaces[i];
+    if (s.type < 0 || s.type > 2)
+      return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: bad surface type");
+    if (s.material >= nMaterials)
+      return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: bad material index");
+  }
+  int nDev = 0;
+  KF_HIP(hipGetDeviceCount(&nDev));
+  if (device < 0 || device >= nDev)
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: bad device index");
+  KF_HIP(hipSetDevice(device));
+
+  std::vector<DevSurface> hs(nSurfaces);
+  for (int32_t i = 0; i < nSurfaces; ++i) {
+    const acts_kf_surface& s = surfaces[i];
+    DevSurface& d = hs[i];
+    std::memcpy(d.R, s.rotation, sizeof(d.R));
+    std::memcpy(d.t, s.translation, sizeof(d.t));
+    invert3(d.R, d.Ri);
+    for (int r = 0; r < 3; ++r)
+      d.ti[r] = -((d.Ri[3 * r] * d.t[0] + d.Ri[3 * r + 1] * d.t[1]) + d.Ri[3 * r + 2] * d.t[2]);
+    d.b0 = s.bounds[0];
+    d.b1 = s.bounds[1];
+    d.type = s.type;
+    d.flags = static_cast<int32_t>(s.flags);
+    d.mat = s.material;
+    d.pad = 0;
+  }
+  std::vector<DevSlab> hm(std::max(nMaterials, 1));
+  for (int32_t i = 0; i < nMaterials; ++i) {
+    const acts_kf_material_slab& m = materials[i];
+    DevSlab& d = hm[i];
+    // Material::fromMolarDensity + MaterialSlab (Material.cpp, MaterialSlab.hpp)
+    const float ne = m.z * m.molar_rho;
+    const float meanI =
+        static_cast<float>(static_cast<double>(1e-9 * 16.0L) * std::pow(m.z, 0.9f));
+    const float eps = 2 * FLT_EPSILON;
+    d.thickness = m.thickness;
+    d.tInX0 = (eps < m.x0) ? (m.thickness / m.x0) : 0;
+    d.tInL0 = (eps < m.l0) ? (m.thickness / m.l0) : 0;
+    d.molarElectronRho = ne;
+    d.meanI = meanI;
+    d.vacuum = (m.ar <= 0.f) ? 1 : 0;
+  }
+
+  acts_kf_ctx* c = new acts_kf_ctx();
+  c->device = device;
+  c->nSurfaces = nSurfaces;
+  auto cleanup = [&](int code, const std::string& msg) {
+    acts_kf_destroy(c);
+    return fail(code, msg);
+  };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(ACTS_KF_ERR_DEVICE, "hipStreamCreate failed");
+  if (hipMalloc(&c->geo, sizeof(DevSurface) * nSurfaces) != hipSuccess ||
+      hipMalloc(&c->slabs, sizeof(DevSlab) * hm.size()) != hipSuccess)
+    return cleanup(ACTS_KF_ERR_OUT_OF_MEMORY, "hipMalloc geometry failed");
+  if (hipMemcpy(c->geo, hs.data(), sizeof(DevSurface) * nSurfaces, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(c->slabs, hm.data(), sizeof(DevSlab) * hm.size(), hipMemcpyHostToDevice) !=
+          hipSuccess)
+    return cleanup(ACTS_KF_ERR_DEVICE, "hipMemcpy geometry failed");
+  c->field.kind = field->kind;
+  for (int i = 0; i < 3; ++i) c->field.B[i] = field->constant[i];
+  DevOpts& o = c->opts;
+  o.stepTolerance = options->step_tolerance;
+  o.stepSizeCutOff = options->step_size_cutoff;
+  o.initialStepSize = options->initial_step_size;
+  o.maxStepSize = options->max_step_size;
+  o.pathLimit = options->path_limit;
+  o.loopFraction = options->loop_fraction;
+  o.surfaceTolerance = options->surface_tolerance;
+  o.nearLimit = options->near_limit;
+  o.chi2Cut = options->outlier_chi2_cut;
+  o.maxTrials = options->max_runge_kutta_step_trials;
+  o.maxSteps = options->max_steps;
+  o.maxTargetSkipping = options->max_target_skipping;
+  o.loopProtection = options->loop_protection;
+  o.multipleScattering = options->multiple_scattering;
+  o.energyLoss = options->energy_loss;
+  o.useJoseph = options->use_joseph;
+  o.absPdg = options->abs_pdg;
+  o.mass = options->mass;
+  o.absQ = options->abs_charge;
+  c->pc = makePhysConst();
+  *outCtx = c;
+  return ACTS_KF_OK;
+}
+
+static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
+                     const acts_kf_batch_out* out, hipStream_t stream,
+                     int32_t maxLen) {
+  const int64_t n = in->n_tracks;
+  if (n == 0) return ACTS_KF_OK;
+  const int64_t stride = (n + kBlock - 1) / kBlock * kBlock;
+  const int32_t states = std::max(maxLen, 1);
+  const size_t needD = sizeof(double) * static_cast<size_t>(states) * kScratchComps * stride;
+  const size_t needI = sizeof(int32_t) * static_cast<size_t>(states) * kIScratchComps * stride;
+  if (c->scratchD.ensure(needD) || c->scratchI.ensure(needI))
+    return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "scratch allocation failed");
+  DevBatch b;
+  b.n = n;
+  b.startParams = in->start_params;
+  b.startCov = in->start_cov;
+  b.startPlane = in->start_plane;
+  b.seqOff = in->seq_offsets;
+  b.seqSurf = in->seq_surfaces;
+  b.measOff = in->meas_offsets;
+  b.measSurf = in->meas_surfaces;
+  b.measVal = in->meas_values;
+  b.measCov = in->meas_cov;
+  DevOut d;
+  d.status = out->status;
+  d.nStates = out->n_states;
+  d.chi2 = out->chi2;
+  d.ndf = out->ndf;
+  d.nMeas = out->n_measurements;
+  d.nHoles = out->n_holes;
+  d.nOutliers = out->n_outliers;
+  d.nSteps = out->n_steps;
+  d.nTrials = out->n_step_trials;
+  d.stateSurface = out->state_surface;
+  d.stateFlags = out->state_flags;
+  d.statePath = out->state_path;
+  d.stateChi2 = out->state_chi2;
+  d.predicted = out->predicted;
+  d.predictedCov = out->predicted_cov;
+  d.filtered = out->filtered;
+  d.filteredCov = out->filtered_cov;
+  d.smoothed = out->smoothed;
+  d.smoothedCov = out->smoothed_cov;
+  d.jacobian = out->jacobian;
+  DevScratch sc;
+  sc.d = static_cast<double*>(c->scratchD.p);
+  sc.i = static_cast<int32_t*>(c->scratchI.p);
+  sc.stride = stride;
+  sc.maxStates = states;
+  const int64_t blocks = stride / kBlock;
+  hipLaunchKernelGGL(fitKernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream,
+                     b, d, sc, c->geo, c->slabs, c->field, c->opts, c->pc, c->nSurfaces);
+  KF_HIP(hipGetLastError());
+  return ACTS_KF_OK;
+}
+
+int acts_kf_fit_batch_device(acts_kf_ctx* c, const acts_kf_batch_in* in,
+                             const acts_kf_batch_out* out, void* stream) {
+  if (!c || !in || !out) return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "null argument");
+  if (in->n_tracks < 0) return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "negative n_tracks");
+  if (in->n_tracks > 0 &&
+      (!in->start_params || !in->start_cov || !in->start_plane || !in->seq_offsets ||
+       !in->seq_surfaces || !in->meas_offsets || !in->meas_surfaces || !in->meas_values ||
+       !in->meas_cov))
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "null input array");
+  KF_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  int32_t maxLen = in->max_sequence_length;
+  if (maxLen <= 0 && in->n_tracks > 0) {
+    if (c->lenBuf.ensure(sizeof(int32_t))) return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "alloc");
+    KF_HIP(hipMemsetAsync(c->lenBuf.p, 0, sizeof(int32_t), s));
+    const int64_t nb = (in->n_tracks + 255) / 256;
+    hipLaunchKernelGGL(maxLengthKernel, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s,
+                       in->seq_offsets, in->n_tracks, static_cast<int32_t*>(c->lenBuf.p));
+    KF_HIP(hipGetLastError());
+    KF_HIP(hipMemcpyAsync(&maxLen, c->lenBuf.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    KF_HIP(hipStreamSynchronize(s));
+  }
+  return launchFit(c, in, out, s, maxLen);
+}
+
+int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
+                      const acts_kf_batch_out* out, void* stream) {
+  if (!c || !in || !out) return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "null argument");
+  const int64_t n = in->n_tracks;
+  if (n < 0) return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "negative n_tracks");
+  if (n == 0) return ACTS_KF_OK;
+  if (!in->start_params || !in->start_cov || !in->start_plane || !in->seq_offsets ||
+      !in->seq_surfaces || !in->meas_offsets || !in->meas_surfaces || !in->meas_values ||
+      !in->meas_cov)
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "null input array");
+  KF_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  const int64_t nSeq = in->seq_offsets[n];
+  const int64_t nMeas = in->meas_offsets[n];
+  int32_t maxLen = 0;
+  for (int64_t t = 0; t < n; ++t)
+    maxLen = std::max<int32_t>(maxLen, static_cast<int32_t>(in->seq_offsets[t + 1] -
+                                                            in->seq_offsets[t]));
+  // input staging layout (8-byte aligned sections)
+  struct Sec {
+    const void* src;
+    size_t bytes;
+  };
+  const Sec ins[] = {
+      {in->start_params, sizeof(double) * 6 * n},  {in->start_cov, sizeof(double) * 36 * n},
+      {in->start_plane, sizeof(double) * 12 * n},  {in->seq_offsets, sizeof(int64_t) * (n + 1)},
+      {in->seq_surfaces, sizeof(int32_t) * nSeq},  {in->meas_offsets, sizeof(int64_t) * (n + 1)},
+      {in->meas_surfaces, sizeof(int32_t) * nMeas}, {in->meas_values, sizeof(double) * 2 * nMeas},
+      {in->meas_cov, sizeof(double) * 4 * nMeas}};
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  size_t total = 0;
+  for (const Sec& q : ins) total += al(q.bytes);
+  if (c->inBuf.ensure(total)) return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "input staging alloc");
+  char* base = static_cast<char*>(c->inBuf.p);
+  void* dptr[9];
+  size_t off = 0;
+  for (int i = 0; i < 9; ++i) {
+    dptr[i] = base + off;
+    if (ins[i].bytes)
+      KF_HIP(hipMemcpyAsync(dptr[i], ins[i].src, ins[i].bytes, hipMemcpyHostToDevice, s));
+    off += al(ins[i].bytes);
+  }
+  acts_kf_batch_in din = *in;
+  din.start_params = static_cast<const double*>(dptr[0]);
+  din.start_cov = static_cast<const double*>(dptr[1]);
+  din.start_plane = static_cast<const double*>(dptr[2]);
+  din.seq_offsets = static_cast<const int64_t*>(dptr[3]);
+  din.seq_surfaces = static_cast<const int32_t*>(dptr[4]);
+  din.meas_offsets = static_cast<const int64_t*>(dptr[5]);
+  din.meas_surfaces = static_cast<const int32_t*>(dptr[6]);
+  din.meas_values = static_cast<const double*>(dptr[7]);
+  din.meas_cov = static_cast<const double*>(dptr[8]);
+  din.max_sequence_length = maxLen;
+
+  // output staging: the same arrays on the device, copied back afterwards
+  struct OSec {
+    void* host;
+    size_t bytes;
+  };
+  const OSec outs[] = {
+      {out->status, sizeof(int32_t) * n},          {out->n_states, sizeof(int32_t) * n},
+      {out->chi2, sizeof(float) * n},              {out->ndf, sizeof(int32_t) * n},
+      {out->n_measurements, sizeof(int32_t) * n},  {out->n_holes, sizeof(int32_t) * n},
+      {out->n_outliers, sizeof(int32_t) * n},      {out->n_steps, sizeof(int32_t) * n},
+      {out->n_step_trials, sizeof(int32_t) * n},   {out->state_surface, sizeof(int32_t) * nSeq},
+      {out->state_flags, sizeof(uint32_t) * nSeq}, {out->state_path, sizeof(double) * nSeq},
+      {out->state_chi2, sizeof(float) * nSeq},     {out->predicted, sizeof(double) * 6 * nSeq},
+      {out->predicted_cov, sizeof(double) * 36 * nSeq},
+      {out->filtered, sizeof(double) * 6 * nSeq},  {out->filtered_cov, sizeof(double) * 36 * nSeq},
+      {out->smoothed, sizeof(double) * 6 * nSeq},  {out->smoothed_cov, sizeof(double) * 36 * nSeq},
+      {out->jacobian, sizeof(double) * 36 * nSeq}};
+  constexpr int nOut = sizeof(outs) / sizeof(outs[0]);
+  size_t totalO = 0;
+  for (const OSec& q : outs)
+    if (q.host) totalO += al(q.bytes);
+  if (c->outBuf.ensure(totalO)) return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "output staging alloc");
+  char* ob = static_cast<char*>(c->outBuf.p);
+  void* optr[nOut];
+  off = 0;
+  for (int i = 0; i < nOut; ++i) {
+    optr[i] = outs[i].host ? ob + off : nullptr;
+    if (outs[i].host) off += al(outs[i].bytes);
+  }
+  acts_kf_batch_out dout;
+  dout.status = static_cast<int32_t*>(optr[0]);
+  dout.n_states = static_cast<int32_t*>(optr[1]);
+  dout.chi2 = static_cast<float*>(optr[2]);
+  dout.ndf = static_cast<int32_t*>(optr[3]);
+  dout.n_measurements = static_cast<int32_t*>(optr[4]);
+  dout.n_holes = static_cast<int32_t*>(optr[5]);
+  dout.n_outliers = static_cast<int32_t*>(optr[6]);
+  dout.n_steps = static_cast<int32_t*>(optr[7]);
+  dout.n_step_trials = static_cast<int32_t*>(optr[8]);
+  dout.state_surface = static_cast<int32_t*>(optr[9]);
+  dout.state_flags = static_cast<uint32_t*>(optr[10]);
+  dout.state_path = static_cast<double*>(optr[11]);
+  dout.state_chi2 = static_cast<float*>(optr[12]);
+  dout.predicted = static_cast<double*>(optr[13]);
+  dout.predicted_cov = static_cast<double*>(optr[14]);
+  dout.filtered = static_cast<double*>(optr[15]);
+  dout.filtered_cov = static_cast<double*>(optr[16]);
+  dout.smoothed = static_cast<double*>(optr[17]);
+  dout.smoothed_cov = static_cast<double*>(optr[18]);
+  dout.jacobian = static_cast<double*>(optr[19]);
+  int r = launchFit(c, &din, &dout, s, maxLen);
+  if (r) return r;
+  for (int i = 0; i < nOut; ++i)
+    if (outs[i].host && outs[i].bytes)
+      KF_HIP(hipMemcpyAsync(outs[i].host, optr[i], outs[i].bytes, hipMemcpyDeviceToHost, s));
+  return ACTS_KF_OK;
+}
+
+int acts_kf_sync(acts_kf_ctx* c) {
+  if (!c) return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "null context");
+  KF_HIP(hipSetDevice(c->device));
+  KF_HIP(hipDeviceSynchronize());
+  return ACTS_KF_OK;
+}
+
+void acts_kf_destroy(acts_kf_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->scratchD.release();
+  c->scratchI.release();
+  c->lenBuf.release();
+  c->inBuf.release();
+  c->outBuf.release();
+  if (c->geo) (void)hipFree(c->geo);
+  if (c->slabs) (void)hipFree(c->slabs);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+}  // extern "C"

@@ -1,0 +1,501 @@
+// kf_fit.hpp -- per-track forward filter + RTS smoother on the device.
+#pragma once
+
+#include "kf_device.hpp"
+
+namespace kf {
+
+struct DevOut {
+  int32_t* status;
+  int32_t* nStates;
+  float* chi2;
+  int32_t* ndf;
+  int32_t* nMeas;
+  int32_t* nHoles;
+  int32_t* nOutliers;
+  int32_t* nSteps;
+  int32_t* nTrials;
+  int32_t* stateSurface;
+  uint32_t* stateFlags;
+  double* statePath;
+  float* stateChi2;
+  double* predicted;
+  double* predictedCov;
+  double* filtered;
+  double* filteredCov;
+  double* smoothed;
+  double* smoothedCov;
+  double* jacobian;
+};
+
+// ---------------------------------------------------------------------------
+// detail::transportCovarianceToBound (CovarianceEngine.cpp:102-165) with
+// boundToBoundTransportJacobian (JacobianEngine.cpp:19-45) and
+// reinitializeJacobians (JacobianEngine.cpp:122-141).  The dense chain
+// F2B * (I + d s) * Jt * B2F is evaluated in the reference's association
+// order with its structural zeros skipped.  `onSurface` is the result of
+// globalToLocal at the current position (reinitialization keeps the old
+// jacToGlobal when it fails, as the reference ignores that Result).
+// ---------------------------------------------------------------------------
+__device__ inline void transportToBound(Track& T, const DevSurface& s,
+                                        bool onSurface, double J[36]) {
+  const V3 dir = T.dir;
+  double b2f[6], f2b[6];
+  positionJacobians(s, T.pos, b2f, f2b);
+  V3 fz;
+  if (s.type == ACTS_KF_SURFACE_CYLINDER)
+    fz = cylinderNormal(s, T.pos);
+  else
+    fz = col(s.R, 2);
+  // freeToPathDerivative (Surface.cpp:292-307)
+  const double dz = dot(fz, dir);
+  const double sv[3] = {(-1.0 * fz.x) / dz, (-1.0 * fz.y) / dz, (-1.0 * fz.z) / dz};
+  // direction rows of freeToBoundJacobian (JacobianHelpers.hpp:52-64)
+  const double cosTheta = dir.z;
+  const double sinTheta = cathetus(1, dir.z);
+  const double inv = 1. / sinTheta;
+  const double cosPhi = dir.x * inv, sinPhi = dir.y * inv;
+  const double invSinTheta = 1. / sinTheta;
+  const double F24 = -sinPhi * invSinTheta, F25 = cosPhi * invSinTheta;
+  const double F34 = cosPhi * cosTheta, F35 = sinPhi * cosTheta, F36 = -sinTheta;
+
+  // A = I + derivative * freeToPath, columns 0..2 (A(7, j) is zero)
+  double A[7][3];
+#pragma unroll
+  for (int k = 0; k < 7; ++k)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double ds = T.der[k] * sv[j];
+      A[k][j] = (k == j) ? 1.0 + ds : ds;
+    }
+  // A1 = F2B * A, columns 0..2 (columns 3..7 equal F2B)
+  double A1[6][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    A1[0][j] = (f2b[0] * A[0][j] + f2b[1] * A[1][j]) + f2b[2] * A[2][j];
+    A1[1][j] = (f2b[3] * A[0][j] + f2b[4] * A[1][j]) + f2b[5] * A[2][j];
+    A1[2][j] = F24 * A[4][j] + F25 * A[5][j];
+    A1[3][j] = (F34 * A[4][j] + F35 * A[5][j]) + F36 * A[6][j];
+    A1[4][j] = 0.0;
+    A1[5][j] = A[3][j];
+  }
+  // A2 = A1 * Jt, columns 4..7 (columns 0..3 equal A1's)
+  double A2[6][4];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const double t0 = T.tr[c], t1 = T.tr[4 + c], t2 = T.tr[8 + c];
+    const double r0 = T.br[c], r1 = T.br[4 + c], r2 = T.br[8 + c];
+    A2[0][c] = (A1[0][0] * t0 + A1[0][1] * t1) + A1[0][2] * t2;
+    A2[1][c] = (A1[1][0] * t0 + A1[1][1] * t1) + A1[1][2] * t2;
+    A2[2][c] = ((((A1[2][0] * t0 + A1[2][1] * t1) + A1[2][2] * t2) + F24 * r0) + F25 * r1);
+    A2[3][c] = (((((A1[3][0] * t0 + A1[3][1] * t1) + A1[3][2] * t2) + F34 * r0) + F35 * r1) +
+                F36 * r2);
+    A2[4][c] = (c == 3) ? 1.0 : 0.0;
+    double a5 = (A1[5][0] * t0 + A1[5][1] * t1) + A1[5][2] * t2;
+    if (c == 3) a5 = a5 + T.tr37;
+    A2[5][c] = a5;
+  }
+  // J = A2 * B2F
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    const double a0 = (r == 4) ? 0.0 : A1[r][0];
+    const double a1 = (r == 4) ? 0.0 : A1[r][1];
+    const double a2 = (r == 4) ? 0.0 : A1[r][2];
+    J[6 * r + 0] = (a0 * T.jp[0] + a1 * T.jp[2]) + a2 * T.jp[4];
+    J[6 * r + 1] = (a0 * T.jp[1] + a1 * T.jp[3]) + a2 * T.jp[5];
+    J[6 * r + 2] = A2[r][0] * T.jd[0] + A2[r][1] * T.jd[2];
+    J[6 * r + 3] = (A2[r][0] * T.jd[1] + A2[r][1] * T.jd[3]) + A2[r][2] * T.jd[4];
+    J[6 * r + 4] = A2[r][3];
+    J[6 * r + 5] = (r == 5) ? 1.0 : 0.0;
+  }
+  // cov = (J * cov) * J^T
+  double JC[36];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double acc = J[6 * i] * T.cov[j];
+#pragma unroll
+      for (int k = 1; k < 6; ++k) acc = acc + J[6 * i + k] * T.cov[6 * k + j];
+      JC[6 * i + j] = acc;
+    }
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double acc = JC[6 * i] * J[6 * j];
+#pragma unroll
+      for (int k = 1; k < 6; ++k) acc = acc + JC[6 * i + k] * J[6 * j + k];
+      T.cov[6 * i + j] = acc;
+    }
+  // reinitializeJacobians
+  resetTransport(T);
+  if (onSurface) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) T.jp[i] = b2f[i];
+    directionB2F(dir, T.jd);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// detail::performMaterialInteraction for a slab scaled by 1
+// (PointwiseMaterialInteraction.hpp:165-226, .cpp:31-87), forward direction
+// ---------------------------------------------------------------------------
+__device__ inline int materialInteraction(Track& T, const DevSurface& s,
+                                          const DevSlab& slab0,
+                                          const PhysConst& pc, const DevOpts& o) {
+  const double pathCorr = pathCorrection(s, T.pos, T.dir);
+  double l0, l1;
+  if (!globalToLocal(s, T.pos, l0, l1)) return ACTS_KF_STATUS_NOT_ON_SURFACE;
+  DevSlab slab = slab0;
+  const float sc = static_cast<float>(pathCorr);
+  slab.thickness *= sc;
+  slab.tInX0 *= sc;
+  slab.tInL0 *= sc;
+  if (slab.vacuum || slab.thickness <= 0.f) return 0;
+  const float qopF = static_cast<float>(T.qop);
+  const double mass = o.mass;
+  const double absQ = o.absQ;
+  double eLoss = 0, varPhi = 0, varTheta = 0, varQop = 0;
+  if (o.energyLoss) eLoss = energyLossBethe(pc, slab, o.mass, qopF, o.absQ);
+  if (o.multipleScattering) {
+    const double t0 = theta0(pc, slab, o.absPdg, o.mass, qopF, o.absQ);
+    const double n = sqrt(dot(T.dir, T.dir));
+    const double sigmaPhi = t0 * (n / perp(T.dir));
+    varPhi = sigmaPhi * sigmaPhi;
+    varTheta = t0 * t0;
+  }
+  if (o.energyLoss) {
+    const double sq = energyLossLandauSigmaQOverP(pc, slab, o.mass, qopF, o.absQ);
+    varQop = sq * sq;
+  }
+  const double momentum = copysign(absQ, T.qop) / T.qop;
+  const double nextE = sqrt(mass * mass + momentum * momentum) - eLoss * 1.0;
+  double nextP = (mass < nextE) ? cathetus(nextE, mass) : 0;
+  nextP = fmax(pc.minP, nextP);
+  const float signedQ = static_cast<float>(copysign(absQ, static_cast<double>(qopF)));
+  T.qop = signedQ / nextP;
+  T.cov[14] = fmax(0., T.cov[14] + varPhi);
+  T.cov[21] = fmax(0., T.cov[21] + varTheta);
+  T.cov[28] = fmax(0., T.cov[28] + varQop);
+  return 0;
+}
+
+// Eigen compute_inverse_size2
+__device__ inline void inverse2(double a, double b, double c, double d,
+                                double out[4]) {
+  // matrix [[a, b], [c, d]]
+  const double det = a * d - c * b;
+  const double invdet = 1.0 / det;
+  out[0] = d * invdet;
+  out[2] = -c * invdet;
+  out[1] = -b * invdet;
+  out[3] = a * invdet;
+}
+
+// GainMatrixUpdaterImpl.hpp:21-91 for a 2D (loc0, loc1) measurement.
+__device__ inline int gainUpdate(const double x[6], const double P[36],
+                                 const double m[2], const double V[4],
+                                 bool joseph, double f[6], double C[36],
+                                 double& chi2) {
+  double Si[4];
+  inverse2(P[0] + V[0], P[1] + V[1], P[6] + V[2], P[7] + V[3], Si);
+  double K[12];
+  bool nan = false;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    K[2 * i] = P[6 * i] * Si[0] + P[6 * i + 1] * Si[2];
+    K[2 * i + 1] = P[6 * i] * Si[1] + P[6 * i + 1] * Si[3];
+    nan = nan || isnan(K[2 * i]) || isnan(K[2 * i + 1]);
+  }
+  if (nan) return ACTS_KF_STATUS_UPDATE_FAILED;
+  const double r0 = m[0] - x[0], r1 = m[1] - x[1];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) f[i] = x[i] + (K[2 * i] * r0 + K[2 * i + 1] * r1);
+  normalizePhiTheta(f[2], f[3]);
+  if (!joseph) {
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const double t0 = (i == 0 ? 1.0 : 0.0) - K[2 * i];
+      const double t1 = (i == 1 ? 1.0 : 0.0) - K[2 * i + 1];
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        double acc = t0 * P[j] + t1 * P[6 + j];
+        if (i >= 2) acc = acc + P[6 * i + j];
+        C[6 * i + j] = acc;
+      }
+    }
+  } else {
+    double tmp[36], TP[36];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+        tmp[6 * i + j] = (i == j ? 1.0 : 0.0) - (j < 2 ? K[2 * i + j] : 0.0);
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        double acc = tmp[6 * i] * P[j];
+#pragma unroll
+        for (int k = 1; k < 6; ++k) acc = acc + tmp[6 * i + k] * P[6 * k + j];
+        TP[6 * i + j] = acc;
+      }
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        double acc = TP[6 * i] * tmp[6 * j];
+#pragma unroll
+        for (int k = 1; k < 6; ++k) acc = acc + TP[6 * i + k] * tmp[6 * j + k];
+        const double kv0 = K[2 * i] * V[0] + K[2 * i + 1] * V[2];
+        const double kv1 = K[2 * i] * V[1] + K[2 * i + 1] * V[3];
+        const double b = kv0 * K[2 * j] + kv1 * K[2 * j + 1];
+        C[6 * i + j] = acc + b;
+      }
+  }
+  // chi2 on the filtered residual with ((I - H K) V)^-1
+  const double e0 = m[0] - f[0], e1 = m[1] - f[1];
+  const double i00 = 1.0 - K[0], i01 = 0.0 - K[1], i10 = 0.0 - K[2], i11 = 1.0 - K[3];
+  double mi[4];
+  inverse2(i00 * V[0] + i01 * V[2], i00 * V[1] + i01 * V[3],
+           i10 * V[0] + i11 * V[2], i10 * V[1] + i11 * V[3], mi);
+  const double row0 = e0 * mi[0] + e1 * mi[2];
+  const double row1 = e0 * mi[1] + e1 * mi[3];
+  chi2 = row0 * e0 + row1 * e1;
+  return 0;
+}
+
+// calculatePredictedChi2 (TrackHelpers.hpp:676-694), 2D measurement
+__device__ inline double predictedChi2(const double x[6], const double P[36],
+                                       const double m[2], const double V[4]) {
+  const double r0 = m[0] - x[0], r1 = m[1] - x[1];
+  double Si[4];
+  inverse2(P[0] + V[0], P[1] + V[1], P[6] + V[2], P[7] + V[3], Si);
+  const double row0 = r0 * Si[0] + r1 * Si[2];
+  const double row1 = r0 * Si[1] + r1 * Si[3];
+  return row0 * r0 + row1 * r1;
+}
+
+// 6x6 inverse by partial-pivot LU with all indices static (row swaps done
+// with selects so that the matrix stays in registers).
+__device__ inline bool inverse6(const double* A, double* X) {
+  double lu[36];
+  int perm[6];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) lu[i] = A[i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) perm[i] = i;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    int piv = k;
+    double big = fabs(lu[6 * k + k]);
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i) {
+      const double v = fabs(lu[6 * i + k]);
+      if (v > big) {
+        big = v;
+        piv = i;
+      }
+    }
+    if (big != 0.0) {
+#pragma unroll
+      for (int i = k + 1; i < 6; ++i) {
+        if (piv == i) {
+#pragma unroll
+          for (int j = 0; j < 6; ++j) {
+            const double tmp = lu[6 * k + j];
+            lu[6 * k + j] = lu[6 * i + j];
+            lu[6 * i + j] = tmp;
+          }
+          const int tp = perm[k];
+          perm[k] = perm[i];
+          perm[i] = tp;
+        }
+      }
+#pragma unroll
+      for (int i = k + 1; i < 6; ++i) lu[6 * i + k] = lu[6 * i + k] / lu[6 * k + k];
+    }
+#pragma unroll
+    for (int i = k + 1; i < 6; ++i)
+#pragma unroll
+      for (int j = k + 1; j < 6; ++j)
+        lu[6 * i + j] = lu[6 * i + j] - lu[6 * i + k] * lu[6 * k + j];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    double b[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) b[i] = (perm[i] == c) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+      for (int i = k + 1; i < 6; ++i) b[i] = b[i] - lu[6 * i + k] * b[k];
+#pragma unroll
+    for (int k = 5; k >= 0; --k) {
+      b[k] = b[k] / lu[6 * k + k];
+#pragma unroll
+      for (int i = 0; i < k; ++i) b[i] = b[i] - lu[6 * i + k] * b[k];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) X[6 * i + c] = b[i];
+  }
+  return true;
+}
+
+// GainMatrixSmoother::calculate (GainMatrixSmoother.cpp:18-97)
+__device__ inline int smoothStep(const double xf[6], const double Cf[36],
+                                 const double xpN[6], const double CpN[36],
+                                 const double xsN[6], const double CsN[36],
+                                 const double JN[36], double xs[6],
+                                 double Cs[36]) {
+  double CfJt[36], Cpi[36], G[36];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double acc = Cf[6 * i] * JN[6 * j];
+#pragma unroll
+      for (int k = 1; k < 6; ++k) acc = acc + Cf[6 * i + k] * JN[6 * j + k];
+      CfJt[6 * i + j] = acc;
+    }
+  inverse6(CpN, Cpi);
+  bool nan = false;
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double acc = CfJt[6 * i] * Cpi[j];
+#pragma unroll
+      for (int k = 1; k < 6; ++k) acc = acc + CfJt[6 * i + k] * Cpi[6 * k + j];
+      G[6 * i + j] = acc;
+      nan = nan || isnan(acc);
+    }
+  if (nan) return ACTS_KF_STATUS_SMOOTH_FAILED;
+  double d[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) d[i] = xsN[i] - xpN[i];
+  d[2] = radianSym(d[2]);
+  d[3] = radianSym(d[3]);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double acc = G[6 * i] * d[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k) acc = acc + G[6 * i + k] * d[k];
+    xs[i] = xf[i] + acc;
+  }
+  normalizePhiTheta(xs[2], xs[3]);
+  double D[36], GD[36];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) D[i] = CsN[i] - CpN[i];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double acc = G[6 * i] * D[j];
+#pragma unroll
+      for (int k = 1; k < 6; ++k) acc = acc + G[6 * i + k] * D[6 * k + j];
+      GD[6 * i + j] = acc;
+    }
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double acc = GD[6 * i] * G[6 * j];
+#pragma unroll
+      for (int k = 1; k < 6; ++k) acc = acc + GD[6 * i + k] * G[6 * j + k];
+      Cs[6 * i + j] = Cf[6 * i + j] + acc;
+    }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// navigation: DirectNavigator.hpp:280-390, SteppingHelper.hpp:40-78
+// ---------------------------------------------------------------------------
+struct Nav {
+  int seqPos;   // DirectNavigator::State::surfaceIndex
+  int seqLen;
+  bool navBreak;
+  int tSurf;    // current target surface (-1 none)
+  int tIdx;     // intersection index of the target
+};
+
+__device__ inline int updateSurfaceStatus(Track& T, const DevSurface& s, int idx,
+                                          double tol) {
+  const Isect x = intersect(s, T.pos, T.dir, tol);
+  const double path = idx ? x.path[1] : x.path[0];
+  const int st = idx ? x.status[1] : x.status[0];
+  if (st == kOnSurface) {
+    T.cNav = kNotSet;
+    updateStep(T.cNav, path);
+    return kOnSurface;
+  }
+  if (st != kUnreachable && path > -DBL_MAX && path < DBL_MAX + kOnSurfaceTolerance) {
+    T.cNav = kNotSet;
+    updateStep(T.cNav, path);
+    return kReachable;
+  }
+  return kUnreachable;
+}
+
+template <typename SeqFn>
+__device__ inline void nextTarget(Nav& nv, const Track& T, const DevSurface* geo,
+                                  const DevOpts& o, SeqFn seqAt) {
+  nv.tSurf = -1;
+  if (nv.navBreak) return;
+  ++nv.seqPos;
+  while (nv.seqPos < nv.seqLen) {
+    const int sid = seqAt(nv.seqPos);
+    const Isect x = intersect(geo[sid], T.pos, T.dir, o.surfaceTolerance);
+    for (int i = 0; i < x.size; ++i) {
+      if (x.status[i] != kUnreachable && x.path[i] > o.nearLimit &&
+          x.path[i] < DBL_MAX + kOnSurfaceTolerance) {
+        nv.tSurf = sid;
+        nv.tIdx = i;
+        return;
+      }
+    }
+    ++nv.seqPos;
+  }
+  nv.navBreak = true;
+}
+
+// Propagator.ipp:49-79
+template <typename SeqFn>
+__device__ inline int getNextTarget(Nav& nv, Track& T, const DevSurface* geo,
+                                    const DevOpts& o, SeqFn seqAt) {
+  for (uint32_t i = 0; i < o.maxTargetSkipping; ++i) {
+    nextTarget(nv, T, geo, o, seqAt);
+    if (nv.tSurf < 0) return 0;
+    const int st = updateSurfaceStatus(T, geo[nv.tSurf], nv.tIdx, o.surfaceTolerance);
+    if (st == kOnSurface) continue;
+    if (st == kReachable) return 0;
+  }
+  nv.tSurf = -1;
+  return ACTS_KF_STATUS_NEXT_TARGET_LIMIT;
+}
+
+// LoopProtection.hpp:19-67 (forward)
+__device__ inline void loopProtection(const Track& T, const DevField& F,
+                                      const DevOpts& o, double& limit,
+                                      bool release) {
+  if (!o.loopProtection) return;
+  if (release) limit = DBL_MAX;
+  const V3 B = fieldAt(F, T.pos);
+  const double Bn = sqrt(dot(B, B));
+  if (Bn == 0) return;
+  const double p = copysign(static_cast<double>(o.absQ), T.qop) / T.qop;
+  const double helixPath = 2 * kPi * p / Bn;
+  const double loopLimit = o.loopFraction * helixPath;
+  if (fabs(loopLimit) < fabs(limit)) limit = loopLimit;
+}
+
+// PathLimitReached::checkAbort (StandardAborters.hpp:41-61)
+__device__ inline bool pathAbort(Track& T, double limit, double tol) {
+  const double distance = fabs(limit) - fabs(T.pathAcc);
+  if (fabs(distance) < fabs(tol)) return true;
+  updateStep(T.cActor, distance);
+  return false;
+}
+
+}  // namespace kf

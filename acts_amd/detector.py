@@ -1,0 +1,150 @@
+"""Detector geometry, material and field descriptions for the fitter.
+
+The geometry is a flat table of surfaces (acts_kf_surface): the direct
+navigator only needs each track's ordered surface sequence, so there is no
+tracking-geometry hierarchy here.  Builders cover the benchmark detectors:
+a silicon barrel of cylinders and, optionally, endcap discs.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+import math
+
+import numpy as np
+
+from . import capi
+
+# Units (Core/include/Acts/Definitions/Units.hpp)
+MM = 1.0
+CM = 10.0
+M = 1000.0
+GEV = 1.0
+MEV = 1e-3
+TESLA = 0.000299792458
+NS = 299792458000.0 * 1e-9  # 1 ns in mm (c = 1)
+UM = 1e-3
+
+# PDG muon mass (float, ParticleHypothesis.hpp:113)
+MUON_MASS = 0.1056583755
+
+
+@dataclass
+class Material:
+    """Material::fromMolarDensity arguments (floats in the reference)."""
+    x0: float
+    l0: float
+    ar: float
+    z: float
+    molar_rho: float
+
+
+def silicon() -> Material:
+    """Silicon as Tests/CommonHelpers/.../PredefinedMaterials.hpp:23-27."""
+    return Material(x0=9.370 * CM, l0=46.52 * CM, ar=28.0855, z=14.0,
+                    molar_rho=(2.329 / 28.0855) * 1.0 / (CM ** 3))
+
+
+@dataclass
+class SurfaceDesc:
+    type: int
+    rotation: np.ndarray  # 3x3, columns = local axes
+    translation: np.ndarray
+    bounds: tuple[float, float]
+    material: int = -1
+    sensitive: bool = True
+
+
+@dataclass
+class Detector:
+    surfaces: list[SurfaceDesc] = field(default_factory=list)
+    materials: list[tuple[Material, float]] = field(default_factory=list)
+    name: str = ""
+
+    def add_material(self, mat: Material, thickness: float) -> int:
+        self.materials.append((mat, thickness))
+        return len(self.materials) - 1
+
+    def surface_table(self):
+        arr = (capi.Surface * len(self.surfaces))()
+        for i, s in enumerate(self.surfaces):
+            arr[i].type = s.type
+            arr[i].material = s.material
+            arr[i].flags = capi.SURFACE_SENSITIVE if s.sensitive else 0
+            rot = np.asarray(s.rotation, dtype=np.float64).reshape(9)
+            for k in range(9):
+                arr[i].rotation[k] = rot[k]
+            for k in range(3):
+                arr[i].translation[k] = float(s.translation[k])
+            arr[i].bounds[0] = float(s.bounds[0])
+            arr[i].bounds[1] = float(s.bounds[1])
+        return arr
+
+    def material_table(self):
+        n = max(1, len(self.materials))
+        arr = (capi.MaterialSlab * n)()
+        for i, (m, t) in enumerate(self.materials):
+            arr[i].x0 = m.x0
+            arr[i].l0 = m.l0
+            arr[i].ar = m.ar
+            arr[i].z = m.z
+            arr[i].molar_rho = m.molar_rho
+            arr[i].thickness = t
+        return arr
+
+
+BARREL_RADII = (32.0, 72.0, 116.0, 172.0, 260.0, 360.0, 500.0, 660.0, 820.0, 1000.0)
+
+
+def barrel_detector(radii=BARREL_RADII, half_length=3000.0, thickness=0.3,
+                    material: Material | None = None) -> Detector:
+    """Cylindrical silicon barrel: one cylinder per radius, centred on z."""
+    det = Detector(name="barrel")
+    mi = det.add_material(material or silicon(), thickness)
+    for r in radii:
+        det.surfaces.append(SurfaceDesc(capi.SURFACE_CYLINDER, np.eye(3),
+                                        np.zeros(3), (r, half_length), mi, True))
+    return det
+
+
+def endcap_z_positions(z_min=600.0, z_max=2900.0, n=7) -> list[float]:
+    return [z_min + (z_max - z_min) * i / (n - 1) for i in range(n)]
+
+
+def barrel_endcap_detector(radii=BARREL_RADII, half_length=3000.0,
+                           disc_z=None, disc_rmin=30.0, disc_rmax=1000.0,
+                           thickness=0.3) -> Detector:
+    """Barrel plus 2 x 7 endcap discs at z = +-(600 .. 2900) mm."""
+    det = barrel_detector(radii, half_length, thickness)
+    det.name = "barrel+endcaps"
+    mi = 0
+    zs = endcap_z_positions() if disc_z is None else list(disc_z)
+    for side in (-1.0, 1.0):
+        for z in zs:
+            det.surfaces.append(SurfaceDesc(capi.SURFACE_DISC, np.eye(3),
+                                            np.array([0.0, 0.0, side * z]),
+                                            (disc_rmin, disc_rmax), mi, True))
+    return det
+
+
+@dataclass
+class ConstantField:
+    bx: float = 0.0
+    by: float = 0.0
+    bz: float = 2.0 * TESLA
+
+    def struct(self) -> capi.Field:
+        f = capi.Field()
+        f.kind = capi.FIELD_CONSTANT
+        f.constant[0], f.constant[1], f.constant[2] = self.bx, self.by, self.bz
+        return f
+
+
+def curvilinear_rotation(direction) -> np.ndarray:
+    """CurvilinearSurface::referenceFrame (CurvilinearSurface.cpp:25-39)."""
+    t = np.asarray(direction, dtype=np.float64)
+    t = t / np.linalg.norm(t)
+    ref = np.array([0.0, 0.0, 1.0]) if abs(t[2]) < 0.999995 else np.array([1.0, 0.0, 0.0])
+    u = np.cross(ref, t)
+    u /= np.linalg.norm(u)
+    v = np.cross(t, u)
+    return np.stack([u, v, t], axis=1)

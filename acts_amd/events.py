@@ -1,0 +1,176 @@
+"""Synthetic muon events (inputs of the batched fit) from the native generator.
+
+Workloads follow the benchmark configurations:
+  * ``barrel``  : 10 barrel layers, pT 1-10 GeV, |eta| < 1, Bz = 2 T
+  * ``endcap``  : barrel + 2x7 discs, |eta| < 2.5 (6-17 measurements)
+Start parameters sit on a curvilinear plane through the true vertex and are
+the truth smeared by ten times the measurement scale:
+sigma(loc0) = 0.15 mm, sigma(loc1) = 0.5 mm, sigma(phi) = sigma(theta) =
+1 mrad, sigma(q/p)/|q/p| = 10 %, sigma(t) = 1 ns; the start covariance is
+diag(sigma^2).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import capi
+from .detector import (ConstantField, Detector, MUON_MASS, NS, UM,
+                       barrel_detector, barrel_endcap_detector)
+
+
+@dataclass
+class Events:
+    n_tracks: int
+    start_params: np.ndarray   # [n,6]
+    start_cov: np.ndarray      # [n,36]
+    start_plane: np.ndarray    # [n,12]
+    seq_offsets: np.ndarray    # [n+1] int64
+    seq_surfaces: np.ndarray   # int32
+    meas_offsets: np.ndarray   # [n+1] int64
+    meas_surfaces: np.ndarray  # int32
+    meas_values: np.ndarray    # [m,2]
+    meas_cov: np.ndarray       # [m,4]
+    truth_start: np.ndarray    # [n,6]
+    truth_meas: np.ndarray     # [m,6]
+    meas_is_outlier: np.ndarray  # [m] int32
+
+    @property
+    def max_sequence_length(self) -> int:
+        if self.n_tracks == 0:
+            return 0
+        return int(np.max(np.diff(self.seq_offsets)))
+
+    def batch_in(self) -> capi.BatchIn:
+        """acts_kf_batch_in over these (host) arrays; keep `self` alive."""
+        b = capi.BatchIn()
+        b.n_tracks = self.n_tracks
+        for name in ("start_params", "start_cov", "start_plane", "seq_offsets",
+                     "seq_surfaces", "meas_offsets", "meas_surfaces", "meas_values",
+                     "meas_cov"):
+            arr = getattr(self, name)
+            assert arr.flags["C_CONTIGUOUS"]
+            setattr(b, name, arr.ctypes.data)
+        b.max_sequence_length = self.max_sequence_length
+        return b
+
+    def subset(self, idx) -> "Events":
+        """Events for the track indices `idx` (in that order)."""
+        idx = np.asarray(idx, dtype=np.int64)
+        so, mo = self.seq_offsets, self.meas_offsets
+        seq = [self.seq_surfaces[so[i]:so[i + 1]] for i in idx]
+        msl = [np.arange(mo[i], mo[i + 1]) for i in idx]
+        mid = np.concatenate(msl) if len(msl) else np.zeros(0, np.int64)
+        new_so = np.zeros(len(idx) + 1, np.int64)
+        new_so[1:] = np.cumsum([len(s) for s in seq])
+        new_mo = np.zeros(len(idx) + 1, np.int64)
+        new_mo[1:] = np.cumsum([len(m) for m in msl])
+        cat = lambda xs, dt: (np.ascontiguousarray(np.concatenate(xs)).astype(dt)
+                              if len(xs) else np.zeros(0, dt))
+        return Events(
+            n_tracks=len(idx),
+            start_params=np.ascontiguousarray(self.start_params[idx]),
+            start_cov=np.ascontiguousarray(self.start_cov[idx]),
+            start_plane=np.ascontiguousarray(self.start_plane[idx]),
+            seq_offsets=new_so, seq_surfaces=cat(seq, np.int32),
+            meas_offsets=new_mo,
+            meas_surfaces=np.ascontiguousarray(self.meas_surfaces[mid]),
+            meas_values=np.ascontiguousarray(self.meas_values[mid]),
+            meas_cov=np.ascontiguousarray(self.meas_cov[mid]),
+            truth_start=np.ascontiguousarray(self.truth_start[idx]),
+            truth_meas=np.ascontiguousarray(self.truth_meas[mid]),
+            meas_is_outlier=np.ascontiguousarray(self.meas_is_outlier[mid]))
+
+
+def particle_options(mass: float = MUON_MASS) -> capi.Options:
+    o = capi.default_options()
+    o.abs_pdg = 13
+    o.mass = mass
+    o.abs_charge = 1.0
+    return o
+
+
+def gen_config(n_tracks: int, seed: int, *, pt=(1.0, 10.0), pt_log=False,
+               eta=(-1.0, 1.0), sigma_loc=(15 * UM, 50 * UM),
+               hole_probability=0.0, outlier_fraction=0.0,
+               outlier_count=(1, 3), outlier_distance=20.0,
+               min_measurements=1, n_threads=0) -> capi.GenConfig:
+    c = capi.GenConfig()
+    c.n_tracks = n_tracks
+    c.seed = seed
+    c.pt_min, c.pt_max = pt
+    c.pt_log_uniform = 1 if pt_log else 0
+    c.n_threads = n_threads
+    c.eta_min, c.eta_max = eta
+    c.phi_min, c.phi_max = -np.pi, np.pi
+    c.vertex_sigma_z = 50.0
+    c.sigma_loc0, c.sigma_loc1 = sigma_loc
+    start = (10 * sigma_loc[0], 10 * sigma_loc[1], 1e-3, 1e-3, 0.1, 1.0 * NS)
+    for i, v in enumerate(start):
+        c.start_sigma[i] = v
+    c.hole_probability = hole_probability
+    c.outlier_fraction = outlier_fraction
+    c.outlier_min, c.outlier_max = outlier_count
+    c.outlier_distance = outlier_distance
+    c.min_measurements = min_measurements
+    c.max_attempts = 16
+    return c
+
+
+def generate(detector: Detector, field: ConstantField, cfg: capi.GenConfig,
+             mass: float = MUON_MASS) -> Events:
+    lib = capi.load_library()
+    surf = detector.surface_table()
+    mats = detector.material_table()
+    fld = field.struct()
+    opt = particle_options(mass)
+    handle = C.c_void_p()
+    capi.check(lib.acts_kf_gen_run(C.byref(cfg), surf, len(detector.surfaces), mats,
+                                   len(detector.materials), C.byref(fld), C.byref(opt),
+                                   C.byref(handle)), "acts_kf_gen_run")
+    try:
+        n, ns, nm = C.c_int64(), C.c_int64(), C.c_int64()
+        lib.acts_kf_gen_sizes(handle, C.byref(n), C.byref(ns), C.byref(nm))
+        n, ns, nm = n.value, ns.value, nm.value
+        ev = Events(
+            n_tracks=n,
+            start_params=np.zeros((n, 6)), start_cov=np.zeros((n, 36)),
+            start_plane=np.zeros((n, 12)),
+            seq_offsets=np.zeros(n + 1, np.int64), seq_surfaces=np.zeros(ns, np.int32),
+            meas_offsets=np.zeros(n + 1, np.int64), meas_surfaces=np.zeros(nm, np.int32),
+            meas_values=np.zeros((nm, 2)), meas_cov=np.zeros((nm, 4)),
+            truth_start=np.zeros((n, 6)), truth_meas=np.zeros((nm, 6)),
+            meas_is_outlier=np.zeros(nm, np.int32))
+        ptr = lambda a: a.ctypes.data
+        lib.acts_kf_gen_fetch(handle, ptr(ev.start_params), ptr(ev.start_cov),
+                              ptr(ev.start_plane), ptr(ev.seq_offsets), ptr(ev.seq_surfaces),
+                              ptr(ev.meas_offsets), ptr(ev.meas_surfaces),
+                              ptr(ev.meas_values), ptr(ev.meas_cov), ptr(ev.truth_start),
+                              ptr(ev.truth_meas), ptr(ev.meas_is_outlier))
+    finally:
+        lib.acts_kf_gen_free(handle)
+    return ev
+
+
+# ---- benchmark workloads ---------------------------------------------------
+WORKLOADS = {
+    # 1,000 muons, barrel only, |eta| < 1, uniform Bz = 2 T
+    "barrel": dict(detector=barrel_detector, n_tracks=1000, eta=(-1.0, 1.0),
+                   pt=(1.0, 10.0)),
+    # 100,000 muons, barrel + 2x7 endcap discs, |eta| < 2.5
+    "endcap": dict(detector=barrel_endcap_detector, n_tracks=100000, eta=(-2.5, 2.5),
+                   pt=(1.0, 10.0)),
+}
+
+
+def workload(name: str, n_tracks: int | None = None, seed: int = 42,
+             n_threads: int = 0):
+    """(detector, field, events) for a named benchmark workload."""
+    w = WORKLOADS[name]
+    det = w["detector"]()
+    field = ConstantField()
+    n = w["n_tracks"] if n_tracks is None else n_tracks
+    cfg = gen_config(n, seed, pt=w["pt"], eta=w["eta"], n_threads=n_threads)
+    return det, field, generate(det, field, cfg)

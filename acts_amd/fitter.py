@@ -1,0 +1,148 @@
+"""Host-side mirror of Acts::KalmanFitter for the batched MI355X fit.
+
+``KalmanFitter(detector, field, options).fit(events)`` plays the role of
+calling ``Acts::KalmanFitter<Propagator<EigenStepper<>, DirectNavigator>>
+::fit(begin, end, startParameters, kfOptions, surfaceSequence, tracks)``
+(Core/include/Acts/TrackFitting/KalmanFitter.hpp:721-731) once per track,
+for a whole batch: same options, same per-track error codes
+(``status`` holds the reference enum values), same track-state content.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, fields
+
+import numpy as np
+
+from . import capi
+from .detector import ConstantField, Detector, MUON_MASS
+from .events import Events
+
+
+@dataclass
+class KalmanFitterOptions:
+    """KalmanFitterOptions + PropagatorPlainOptions + StepperPlainOptions."""
+    multiple_scattering: bool = True        # KalmanFitter.hpp:166
+    energy_loss: bool = True                # KalmanFitter.hpp:169
+    use_joseph: bool = False                # GainMatrixUpdater
+    outlier_chi2_cut: float = 0.0           # SimpleOutlierFinder; <= 0 disables
+    precision: int = capi.PRECISION_FP64
+    step_tolerance: float = 1e-4            # StepperOptions.hpp:37
+    step_size_cutoff: float = 0.0
+    initial_step_size: float = 10000.0
+    max_step_size: float = np.finfo(np.float64).max
+    max_runge_kutta_step_trials: int = 10000
+    max_steps: int = 1000                   # PropagatorOptions.hpp:30
+    max_target_skipping: int = 100
+    loop_protection: bool = True
+    path_limit: float = np.finfo(np.float64).max
+    loop_fraction: float = 0.5
+    surface_tolerance: float = 1e-4
+    near_limit: float = -0.1                # DirectNavigator.hpp:58 (-100 um)
+    abs_pdg: int = 13
+    mass: float = MUON_MASS
+    abs_charge: float = 1.0
+
+    def struct(self) -> capi.Options:
+        o = capi.Options()
+        for f in fields(self):
+            v = getattr(self, f.name)
+            setattr(o, f.name, int(v) if isinstance(v, bool) else v)
+        return o
+
+
+STATE_ARRAYS = {
+    # name: (dtype, per-state width)
+    "state_surface": (np.int32, 1),
+    "state_flags": (np.uint32, 1),
+    "state_path": (np.float64, 1),
+    "state_chi2": (np.float32, 1),
+    "predicted": (np.float64, 6),
+    "predicted_cov": (np.float64, 36),
+    "filtered": (np.float64, 6),
+    "filtered_cov": (np.float64, 36),
+    "smoothed": (np.float64, 6),
+    "smoothed_cov": (np.float64, 36),
+    "jacobian": (np.float64, 36),
+}
+TRACK_ARRAYS = {
+    "status": np.int32, "n_states": np.int32, "chi2": np.float32, "ndf": np.int32,
+    "n_measurements": np.int32, "n_holes": np.int32, "n_outliers": np.int32,
+    "n_steps": np.int32, "n_step_trials": np.int32,
+}
+DEFAULT_STATE_OUTPUTS = ("state_surface", "state_flags", "state_path", "state_chi2",
+                         "smoothed", "smoothed_cov")
+ALL_STATE_OUTPUTS = tuple(STATE_ARRAYS)
+
+
+class FitResult:
+    """Per-track and per-state output arrays (numpy) of one batch."""
+
+    def __init__(self, n_tracks: int, n_slots: int, outputs=DEFAULT_STATE_OUTPUTS):
+        self.n_tracks = n_tracks
+        for name, dt in TRACK_ARRAYS.items():
+            setattr(self, name, np.zeros(n_tracks, dt))
+        self.outputs = tuple(outputs)
+        for name in STATE_ARRAYS:
+            dt, w = STATE_ARRAYS[name]
+            if name in self.outputs:
+                shape = (n_slots,) if w == 1 else (n_slots, w)
+                setattr(self, name, np.zeros(shape, dt))
+            else:
+                setattr(self, name, None)
+
+    def batch_out(self) -> capi.BatchOut:
+        b = capi.BatchOut()
+        for name in capi.BATCH_OUT_FIELDS:
+            arr = getattr(self, name, None)
+            setattr(b, name, arr.ctypes.data if arr is not None else None)
+        return b
+
+
+class KalmanFitter:
+    """Batched Kalman fitter on one HIP device."""
+
+    def __init__(self, detector: Detector, field: ConstantField,
+                 options: KalmanFitterOptions | None = None, device: int = 0):
+        self.lib = capi.load_library()
+        self.detector = detector
+        self.options = options or KalmanFitterOptions()
+        self._surf = detector.surface_table()
+        self._mats = detector.material_table()
+        self._field = field.struct()
+        self._opt = self.options.struct()
+        ctx = C.c_void_p()
+        capi.check(self.lib.acts_kf_create(self._surf, len(detector.surfaces), self._mats,
+                                           len(detector.materials), C.byref(self._field),
+                                           C.byref(self._opt), device, C.byref(ctx)),
+                   "acts_kf_create")
+        self.ctx = ctx
+
+    def fit(self, events: Events, outputs=DEFAULT_STATE_OUTPUTS) -> FitResult:
+        """Fit a batch held in host memory; blocks until the results are back."""
+        res = FitResult(events.n_tracks, int(events.seq_offsets[-1]), outputs)
+        bin_ = events.batch_in()
+        bout = res.batch_out()
+        capi.check(self.lib.acts_kf_fit_batch(self.ctx, C.byref(bin_), C.byref(bout), None),
+                   "acts_kf_fit_batch")
+        capi.check(self.lib.acts_kf_sync(self.ctx), "acts_kf_sync")
+        return res
+
+    def fit_device(self, bin_: capi.BatchIn, bout: capi.BatchOut, stream=None) -> None:
+        """Fit a batch whose buffers are device memory; asynchronous on `stream`."""
+        capi.check(self.lib.acts_kf_fit_batch_device(self.ctx, C.byref(bin_), C.byref(bout),
+                                                     stream), "acts_kf_fit_batch_device")
+
+    def sync(self) -> None:
+        capi.check(self.lib.acts_kf_sync(self.ctx), "acts_kf_sync")
+
+    def close(self) -> None:
+        if getattr(self, "ctx", None):
+            self.lib.acts_kf_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,250 @@
+/*
+ * acts_kf.h -- C ABI of the MI355X batched Kalman track fitter.
+ *
+ * This is the drop-in boundary for the reference call
+ *
+ *   Acts::KalmanFitter<Propagator<EigenStepper<>, DirectNavigator>, traj_t>::fit(
+ *       begin, end, startParameters, kfOptions, surfaceSequence, trackContainer)
+ *   (Core/include/Acts/TrackFitting/KalmanFitter.hpp:721-731)
+ *
+ * batched over many tracks.  Everything is plain C: POD structs, pointers and
+ * sizes.  One call fits a whole batch of independent tracks; per-track results
+ * and error codes come back in structure-of-arrays buffers.
+ *
+ * Conventions (identical to the reference):
+ *   - units: mm, GeV, ns*c (time in mm), magnetic field in GeV/(e*mm)
+ *     (1 T = 0.000299792458, Core/include/Acts/Definitions/Units.hpp:143)
+ *   - bound parameter order: loc0, loc1, phi, theta, q/p, time
+ *     (Core/include/Acts/Definitions/TrackParametrization.hpp:42-53)
+ *   - free parameter order: x, y, z, t, dx, dy, dz, q/p (:67-79)
+ *   - every 6x6 matrix (covariance, jacobian) is stored row-major, 36 doubles.
+ *
+ * Ownership: the caller owns every buffer it passes.  Host buffers handed to
+ * acts_kf_fit_batch must stay valid until acts_kf_sync returns.  The context
+ * owns its device memory (geometry, field, scratch).  One context serialises
+ * its calls on the stream it is given; several contexts may share a device.
+ */
+#ifndef ACTS_KF_H
+#define ACTS_KF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ACTS_KF_API_VERSION 1
+
+/* ---- API return codes (the call itself) -------------------------------- */
+#define ACTS_KF_OK 0
+#define ACTS_KF_ERR_INVALID_ARGUMENT (-1)
+#define ACTS_KF_ERR_DEVICE (-2)         /* a HIP runtime call failed */
+#define ACTS_KF_ERR_OUT_OF_MEMORY (-3)
+#define ACTS_KF_ERR_UNSUPPORTED (-4)
+
+/* ---- per-track fit status: the reference's error enums ----------------- *
+ * code = category | enum value, enum values as in the reference headers.   */
+#define ACTS_KF_STATUS_OK 0
+#define ACTS_KF_CAT_KALMAN 0x100     /* KalmanFitterError.hpp:18-31 */
+#define ACTS_KF_CAT_PROPAGATOR 0x200 /* PropagatorError.hpp:18-26 */
+#define ACTS_KF_CAT_STEPPER 0x300    /* EigenStepperError.hpp:18-26 */
+#define ACTS_KF_CAT_SURFACE 0x400    /* SurfaceError.hpp:18-22 */
+#define ACTS_KF_CAT_FIELD 0x500      /* MagneticFieldError.hpp:18-22 */
+
+#define ACTS_KF_STATUS_UPDATE_FAILED (ACTS_KF_CAT_KALMAN | 1)
+#define ACTS_KF_STATUS_SMOOTH_FAILED (ACTS_KF_CAT_KALMAN | 2)
+#define ACTS_KF_STATUS_NO_MEASUREMENT_FOUND (ACTS_KF_CAT_KALMAN | 4)
+#define ACTS_KF_STATUS_PROPAGATION_FAILURE (ACTS_KF_CAT_PROPAGATOR | 1)
+#define ACTS_KF_STATUS_STEP_COUNT_LIMIT (ACTS_KF_CAT_PROPAGATOR | 2)
+#define ACTS_KF_STATUS_NEXT_TARGET_LIMIT (ACTS_KF_CAT_PROPAGATOR | 3)
+#define ACTS_KF_STATUS_STEP_SIZE_STALLED (ACTS_KF_CAT_STEPPER | 1)
+#define ACTS_KF_STATUS_STEP_INVALID (ACTS_KF_CAT_STEPPER | 2)
+#define ACTS_KF_STATUS_STEP_ADJUSTMENT_FAILED (ACTS_KF_CAT_STEPPER | 3)
+#define ACTS_KF_STATUS_NOT_ON_SURFACE (ACTS_KF_CAT_SURFACE | 1)
+#define ACTS_KF_STATUS_FIELD_OUT_OF_BOUNDS (ACTS_KF_CAT_FIELD | 1)
+
+/* ---- track-state type flags (TrackStateType.hpp:30-49, bit positions) -- */
+#define ACTS_KF_STATE_HAS_MEASUREMENT (1u << 0)
+#define ACTS_KF_STATE_HAS_PARAMETERS (1u << 1)
+#define ACTS_KF_STATE_IS_OUTLIER (1u << 2)
+#define ACTS_KF_STATE_IS_HOLE (1u << 3)
+#define ACTS_KF_STATE_HAS_MATERIAL (1u << 4)
+
+/* ---- geometry ----------------------------------------------------------- */
+#define ACTS_KF_SURFACE_PLANE 0    /* PlaneSurface.cpp */
+#define ACTS_KF_SURFACE_CYLINDER 1 /* CylinderSurface.cpp */
+#define ACTS_KF_SURFACE_DISC 2     /* DiscSurface.cpp */
+
+#define ACTS_KF_SURFACE_SENSITIVE 1u /* Surface::isSensitive() */
+
+/* One surface.  The placement is the surface's local-to-global transform:
+ * rotation columns are the local x, y, z axes expressed in the global frame.
+ * Bounds are not used by the fit itself (the direct navigator intersects with
+ * BoundaryTolerance::Infinite, DirectNavigator.hpp:305-308) but are used by
+ * the geometry helpers and the event generator.                              */
+typedef struct acts_kf_surface {
+  int32_t type;          /* ACTS_KF_SURFACE_* */
+  int32_t material;      /* index into the material table, or -1 */
+  uint32_t flags;        /* ACTS_KF_SURFACE_* flag bits */
+  int32_t reserved;
+  double rotation[9];    /* row-major 3x3 */
+  double translation[3];
+  double bounds[2];      /* cylinder: R, half length z; disc: rmin, rmax */
+} acts_kf_surface;
+
+/* Homogeneous surface material: Material::fromMolarDensity(x0, l0, ar, z,
+ * molar_rho) with a slab thickness (Material.cpp, MaterialSlab.hpp:40-160).
+ * The library derives the electron density and mean excitation energy in
+ * float arithmetic exactly as the reference does.                          */
+typedef struct acts_kf_material_slab {
+  float x0;         /* radiation length [mm] */
+  float l0;         /* nuclear interaction length [mm] */
+  float ar;         /* relative atomic mass */
+  float z;          /* nuclear charge */
+  float molar_rho;  /* molar density [mol/mm^3] */
+  float thickness;  /* [mm] */
+} acts_kf_material_slab;
+
+/* ---- magnetic field ------------------------------------------------------ */
+#define ACTS_KF_FIELD_CONSTANT 0 /* ConstantBField.hpp */
+#define ACTS_KF_FIELD_GRID_XYZ 1 /* fieldMapXYZ + InterpolatedBFieldMap */
+
+typedef struct acts_kf_field {
+  int32_t kind;
+  int32_t reserved;
+  double constant[3];  /* kind == CONSTANT */
+  /* kind == GRID_XYZ: field values on the points of a regular grid,
+   * n[a] points on axis a from point_min[a] to point_max[a] (inclusive).
+   * values[((ix*n[1]) + iy)*n[2] + iz][c], c = Bx, By, Bz.                  */
+  int32_t n[3];
+  int32_t reserved2;
+  double point_min[3];
+  double point_max[3];
+  const double* values; /* host pointer, read once by acts_kf_create */
+} acts_kf_field;
+
+/* ---- options: KalmanFitterOptions + StepperPlainOptions +
+ *      PropagatorPlainOptions + DirectNavigator::Options + particle -------- */
+#define ACTS_KF_PRECISION_FP64 0  /* plain FP64 fit (the drop-in) */
+#define ACTS_KF_PRECISION_MIXED 1 /* FP32 stepping/field, FP64 Kalman algebra */
+
+typedef struct acts_kf_options {
+  int32_t multiple_scattering; /* KalmanFitter.hpp:166, default 1 */
+  int32_t energy_loss;         /* KalmanFitter.hpp:169, default 1 */
+  int32_t use_joseph;          /* GainMatrixUpdater, default 0 */
+  int32_t precision;           /* ACTS_KF_PRECISION_* */
+  double outlier_chi2_cut;     /* predicted chi2 cut; <= 0 disables the finder */
+  /* StepperPlainOptions (StepperOptions.hpp:36-49) */
+  double step_tolerance;       /* 1e-4 */
+  double step_size_cutoff;     /* 0 */
+  double initial_step_size;    /* 10 m */
+  double max_step_size;        /* DBL_MAX */
+  uint32_t max_runge_kutta_step_trials; /* 10000 */
+  /* PropagatorPlainOptions (PropagatorOptions.hpp:26-52) */
+  uint32_t max_steps;          /* 1000 */
+  uint32_t max_target_skipping;/* 100 */
+  int32_t loop_protection;     /* 1 */
+  double path_limit;           /* DBL_MAX */
+  double loop_fraction;        /* 0.5 */
+  double surface_tolerance;    /* 1e-4 */
+  /* DirectNavigator::Options (DirectNavigator.hpp:51-61) */
+  double near_limit;           /* -100 um */
+  /* ParticleHypothesis (ParticleHypothesis.hpp) */
+  int32_t abs_pdg;             /* 13 = muon */
+  float mass;                  /* GeV */
+  float abs_charge;            /* e */
+  int32_t reserved;
+} acts_kf_options;
+
+/* ---- one batch of tracks ------------------------------------------------- */
+typedef struct acts_kf_batch_in {
+  int64_t n_tracks;
+  /* start parameters: bound vector and covariance on the track's start plane
+   * (BoundTrackParameters; the start surface is a PlaneSurface given by its
+   * rotation[9] + translation[3] per track, without material)               */
+  const double* start_params;   /* [n][6] */
+  const double* start_cov;      /* [n][36] */
+  const double* start_plane;    /* [n][12]: rotation row-major, then translation */
+  /* surface sequence for the direct navigator (KalmanFitter.hpp:725,778) */
+  const int64_t* seq_offsets;   /* [n+1] */
+  const int32_t* seq_surfaces;  /* [seq_offsets[n]] indices into the geometry */
+  /* measurements: 2D (loc0, loc1) source links keyed by their surface */
+  const int64_t* meas_offsets;  /* [n+1] */
+  const int32_t* meas_surfaces; /* [meas_offsets[n]] */
+  const double* meas_values;    /* [m][2] */
+  const double* meas_cov;       /* [m][4] row-major 2x2 */
+  /* upper bound on any track's sequence length (sizes the per-state scratch);
+   * 0 = let the library compute it (one extra device->host copy)            */
+  int32_t max_sequence_length;
+  int32_t reserved;
+} acts_kf_batch_in;
+
+/* Output buffers.  Per-state arrays use the sequence layout: the states of
+ * track i occupy slots seq_offsets[i] .. seq_offsets[i] + n_states[i] - 1 in
+ * propagation order (first state first), up to and including the last
+ * measurement state (the track tip, KalmanFitter.hpp:831).  Any per-state
+ * pointer may be NULL to skip that output.                                  */
+typedef struct acts_kf_batch_out {
+  int32_t* status;          /* [n] ACTS_KF_STATUS_* */
+  int32_t* n_states;        /* [n] */
+  float* chi2;              /* [n] sum over measurement states (float, as the track column) */
+  int32_t* ndf;             /* [n] */
+  int32_t* n_measurements;  /* [n] */
+  int32_t* n_holes;         /* [n] */
+  int32_t* n_outliers;      /* [n] */
+  int32_t* n_steps;         /* [n] accepted RK steps (StepperStatistics) */
+  int32_t* n_step_trials;   /* [n] attempted RK steps */
+  int32_t* state_surface;   /* [slots] geometry index of the state's surface */
+  uint32_t* state_flags;    /* [slots] ACTS_KF_STATE_* */
+  double* state_path;       /* [slots] accumulated path length */
+  float* state_chi2;        /* [slots] filtered chi2 (float, VectorMultiTrajectory.hpp:193) */
+  double* predicted;        /* [slots][6] */
+  double* predicted_cov;    /* [slots][36] */
+  double* filtered;         /* [slots][6] */
+  double* filtered_cov;     /* [slots][36] */
+  double* smoothed;         /* [slots][6] */
+  double* smoothed_cov;     /* [slots][36] */
+  double* jacobian;         /* [slots][36] transport jacobian from the previous state */
+} acts_kf_batch_out;
+
+typedef struct acts_kf_ctx acts_kf_ctx;
+
+/* Fill *opt with the reference defaults (see the field comments). */
+void acts_kf_default_options(acts_kf_options* opt);
+
+/* Create a fitting context on HIP device `device`: uploads the geometry,
+ * material table and field once.  Replaces the construction of
+ * KalmanFitter(Propagator(EigenStepper(bField), DirectNavigator())).       */
+int acts_kf_create(const acts_kf_surface* surfaces, int32_t n_surfaces,
+                   const acts_kf_material_slab* materials, int32_t n_materials,
+                   const acts_kf_field* field, const acts_kf_options* options,
+                   int32_t device, acts_kf_ctx** out_ctx);
+
+/* Fit a batch whose buffers (in and out) are host memory.  Asynchronous on
+ * `stream` (a hipStream_t, NULL = the context's own stream); results are in
+ * the output buffers after acts_kf_sync.                                    */
+int acts_kf_fit_batch(acts_kf_ctx* ctx, const acts_kf_batch_in* in,
+                      const acts_kf_batch_out* out, void* stream);
+
+/* Fit a batch whose buffers are all device memory on the context's device
+ * (inputs already resident in HBM).  Asynchronous on `stream`.             */
+int acts_kf_fit_batch_device(acts_kf_ctx* ctx, const acts_kf_batch_in* in,
+                             const acts_kf_batch_out* out, void* stream);
+
+/* Wait for all work the context queued. */
+int acts_kf_sync(acts_kf_ctx* ctx);
+
+/* Release the context and its device memory. */
+void acts_kf_destroy(acts_kf_ctx* ctx);
+
+/* Message for a per-track status code (the reference's error messages). */
+const char* acts_kf_status_string(int32_t status);
+
+/* Message for the last API error of this thread. */
+const char* acts_kf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ACTS_KF_H */

@@ -1,0 +1,160 @@
+"""Python loader of the CPU oracle (liboracle_kf.so).
+
+TEST INFRASTRUCTURE ONLY: the oracle is a CPU restatement of the reference
+Kalman-fit path used to check the HIP fitter (tests/, __graft_entry__.smoke)
+and timed as the CPU baseline (bench.py).  It is never part of the product.
+The ABI structs are shared with the product (include/acts_kf.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from acts_amd import capi
+from acts_amd.fitter import ALL_STATE_OUTPUTS, FitResult
+
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(ORACLE_DIR, "liboracle_kf.so")
+
+_lib = None
+
+
+def build(verbose: bool = False) -> str:
+    """Compile the oracle with its Makefile (g++, -O2, no -march)."""
+    res = subprocess.run(["make", "-C", ORACLE_DIR, "-B", "liboracle_kf.so"],
+                         capture_output=True, text=True)
+    if verbose or res.returncode != 0:
+        print(res.stdout + res.stderr)
+    if res.returncode != 0:
+        raise RuntimeError("oracle build failed")
+    return LIB_PATH
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    lib = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    lib.acts_oracle_fit.restype = C.c_int
+    lib.acts_oracle_fit.argtypes = [C.POINTER(capi.Surface), C.c_int32,
+                                    C.POINTER(capi.MaterialSlab), C.c_int32,
+                                    C.POINTER(capi.Field), C.POINTER(capi.Options),
+                                    C.POINTER(capi.BatchIn), C.POINTER(capi.BatchOut),
+                                    C.c_int32, dp]
+    lib.acts_oracle_gain_update.restype = C.c_int
+    lib.acts_oracle_gain_update.argtypes = [dp, dp, dp, dp, C.c_int32, dp, dp, dp]
+    lib.acts_oracle_smooth.restype = C.c_int
+    lib.acts_oracle_smooth.argtypes = [C.c_int32, dp, dp, dp, dp, dp, dp, dp]
+    lib.acts_oracle_interaction.restype = C.c_float
+    lib.acts_oracle_interaction.argtypes = [C.c_int32, C.POINTER(capi.MaterialSlab),
+                                            C.c_int32, C.c_float, C.c_float, C.c_float]
+    lib.acts_oracle_surface_intersect.restype = C.c_int
+    lib.acts_oracle_surface_intersect.argtypes = [C.POINTER(capi.Surface), dp, dp,
+                                                  C.c_double, dp, C.POINTER(C.c_int32)]
+    lib.acts_oracle_surface_local.restype = C.c_int
+    lib.acts_oracle_surface_local.argtypes = [C.POINTER(capi.Surface), C.c_int32, dp, dp]
+    lib.acts_oracle_propagate.restype = C.c_int
+    lib.acts_oracle_propagate.argtypes = [C.POINTER(capi.Field), C.POINTER(capi.Options),
+                                          dp, dp, dp, C.POINTER(capi.Surface), C.c_double,
+                                          dp, dp, dp, dp, C.POINTER(C.c_int32)]
+    _lib = lib
+    return lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def fit(detector, field, options, events, n_threads: int = 0,
+        outputs=ALL_STATE_OUTPUTS):
+    """Fit `events` on the CPU; returns (FitResult, wall seconds)."""
+    lib = load()
+    res = FitResult(events.n_tracks, int(events.seq_offsets[-1]), outputs)
+    surf = detector.surface_table()
+    mats = detector.material_table()
+    fld = field.struct()
+    opt = options.struct() if hasattr(options, "struct") else options
+    bin_ = events.batch_in()
+    bout = res.batch_out()
+    secs = C.c_double(0.0)
+    rc = lib.acts_oracle_fit(surf, len(detector.surfaces), mats, len(detector.materials),
+                             C.byref(fld), C.byref(opt), C.byref(bin_), C.byref(bout),
+                             n_threads, C.byref(secs))
+    if rc != 0:
+        raise RuntimeError(f"acts_oracle_fit failed: {rc}")
+    return res, secs.value
+
+
+def gain_update(pred, pred_cov, meas, meas_cov, joseph=False):
+    lib = load()
+    pred = np.ascontiguousarray(pred, np.float64)
+    pred_cov = np.ascontiguousarray(pred_cov, np.float64).reshape(36)
+    meas = np.ascontiguousarray(meas, np.float64)
+    meas_cov = np.ascontiguousarray(meas_cov, np.float64).reshape(4)
+    filt = np.zeros(6)
+    fcov = np.zeros(36)
+    chi2 = np.zeros(1)
+    rc = lib.acts_oracle_gain_update(_dp(pred), _dp(pred_cov), _dp(meas), _dp(meas_cov),
+                                     int(joseph), _dp(filt), _dp(fcov), _dp(chi2))
+    return rc, filt, fcov.reshape(6, 6), float(chi2[0])
+
+
+def smooth(predicted, predicted_cov, filtered, filtered_cov, jacobian):
+    lib = load()
+    n = len(predicted)
+    arrs = [np.ascontiguousarray(a, np.float64) for a in
+            (predicted, predicted_cov, filtered, filtered_cov, jacobian)]
+    xs = np.zeros((n, 6))
+    cs = np.zeros((n, 36))
+    rc = lib.acts_oracle_smooth(n, *[_dp(a) for a in arrs], _dp(xs), _dp(cs))
+    return rc, xs, cs.reshape(n, 6, 6)
+
+
+def interaction(mode, slab, abs_pdg, mass, qop, abs_q=1.0):
+    lib = load()
+    ptr = C.byref(slab) if slab is not None else None
+    return lib.acts_oracle_interaction(mode, ptr, abs_pdg, mass, qop, abs_q)
+
+
+def surface_intersect(surface, pos, direction, tol=1e-4):
+    lib = load()
+    paths = np.zeros(2)
+    status = (C.c_int32 * 2)()
+    n = lib.acts_oracle_surface_intersect(C.byref(surface), _dp(np.asarray(pos, float)),
+                                          _dp(np.asarray(direction, float)), tol,
+                                          _dp(paths), status)
+    return n, paths, [status[0], status[1]]
+
+
+def surface_local(surface, to_global, v3):
+    lib = load()
+    inp = np.ascontiguousarray(v3, np.float64)
+    out = np.zeros(3)
+    rc = lib.acts_oracle_surface_local(C.byref(surface), int(to_global), _dp(inp), _dp(out))
+    return rc, out
+
+
+def propagate(field, options, start_plane, start_params, start_cov, target, direction=1.0):
+    """Propagate bound parameters from a start plane to `target` (a capi.Surface)."""
+    lib = load()
+    sp = np.ascontiguousarray(start_plane, np.float64).reshape(12)
+    pp = np.ascontiguousarray(start_params, np.float64)
+    cov = None if start_cov is None else np.ascontiguousarray(start_cov, np.float64).reshape(36)
+    ep = np.zeros(6)
+    ec = np.zeros(36)
+    jac = np.zeros(36)
+    path = np.zeros(1)
+    nsteps = C.c_int32(0)
+    opt = options.struct() if hasattr(options, "struct") else options
+    fld = field.struct() if hasattr(field, "struct") else field
+    rc = lib.acts_oracle_propagate(C.byref(fld), C.byref(opt), _dp(sp), _dp(pp),
+                                   _dp(cov) if cov is not None else None, C.byref(target),
+                                   direction, _dp(ep), _dp(ec), _dp(jac), _dp(path),
+                                   C.byref(nsteps))
+    return rc, ep, ec.reshape(6, 6), jac.reshape(6, 6), float(path[0]), nsteps.value

@@ -1,0 +1,98 @@
+"""Comparison of two batched fit results (HIP fitter vs CPU oracle).
+
+Parameters are compared with |a - b| <= rel * max(|b|, floor) per bound
+component (floors keep near-zero values from demanding absolute agreement
+below the parameters' natural scale); covariances with
+|a_ij - b_ij| <= rel * sqrt(|b_ii b_jj|).
+
+Tolerances of the FP64 fit: 1e-10 for every parameter vector, the
+predicted/filtered covariances and the transport jacobians; 1e-9 for the
+smoothed covariance, whose RTS update inverts the predicted covariance
+(condition numbers up to ~1e13 in the mixed mm/rad/GeV^-1/ns units) and so
+amplifies last-bit differences of the double-precision sin/cos/atan2 of the
+two math libraries by up to ~1e3.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+# loc0 [mm], loc1 [mm], phi, theta [rad], q/p [1/GeV], time [mm]
+PARAM_FLOORS = np.array([1.0, 1.0, 1.0, 1.0, 0.01, 1.0])
+
+
+def param_rel_err(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    scale = np.maximum(np.abs(b), PARAM_FLOORS)
+    return np.abs(a - b) / scale
+
+
+def cov_rel_err(a: np.ndarray, b: np.ndarray) -> np.ndarray:
+    a = a.reshape(-1, 6, 6)
+    b = b.reshape(-1, 6, 6)
+    d = np.sqrt(np.abs(np.einsum("nii->ni", b)))
+    scale = d[:, :, None] * d[:, None, :]
+    scale = np.where(scale > 0, scale, 1.0)
+    return (np.abs(a - b) / scale).reshape(len(a), -1)
+
+
+def compare(got, ref, events, rel: float = 1e-10, rel_smoothed_cov: float = 1e-9) -> dict:
+    """Return a report; `ok` is True if everything agrees within tolerance."""
+    n = events.n_tracks
+    rep = {"n_tracks": n}
+    same_status = got.status == ref.status
+    rep["status_mismatch"] = int(np.count_nonzero(~same_status))
+    both_ok = same_status & (ref.status == 0)
+    rep["n_ok"] = int(np.count_nonzero(both_ok))
+    rep["n_failed_ref"] = int(np.count_nonzero(ref.status != 0))
+    same_ns = got.n_states == ref.n_states
+    rep["n_states_mismatch"] = int(np.count_nonzero(both_ok & ~same_ns))
+    good = both_ok & same_ns
+    for name in ("chi2", "ndf", "n_measurements", "n_holes", "n_outliers"):
+        g, r = getattr(got, name), getattr(ref, name)
+        if name == "chi2":
+            bad = good & (np.abs(g - r) > 1e-5 * np.maximum(np.abs(r), 1.0))
+        else:
+            bad = good & (g != r)
+        rep[f"{name}_mismatch"] = int(np.count_nonzero(bad))
+    # per-state arrays over the states of the good tracks
+    so = events.seq_offsets
+    idx = np.concatenate([np.arange(so[t], so[t] + ref.n_states[t])
+                          for t in np.nonzero(good)[0]]) if good.any() else np.zeros(0, int)
+    seq_bad_slots = np.zeros(0, bool)
+    if ref.state_surface is not None and got.state_surface is not None:
+        seq_bad_slots = (got.state_surface[idx] != ref.state_surface[idx])
+        if ref.state_flags is not None and got.state_flags is not None:
+            seq_bad_slots |= got.state_flags[idx] != ref.state_flags[idx]
+    rep["state_sequence_mismatch_slots"] = int(np.count_nonzero(seq_bad_slots))
+    worst = {}
+    for name in ("predicted", "filtered", "smoothed"):
+        g, r = getattr(got, name), getattr(ref, name)
+        if g is not None and r is not None and len(idx):
+            worst[name] = float(np.max(param_rel_err(g[idx], r[idx])))
+    for name in ("predicted_cov", "filtered_cov", "smoothed_cov", "jacobian"):
+        g, r = getattr(got, name), getattr(ref, name)
+        if g is not None and r is not None and len(idx):
+            if name == "jacobian":
+                worst[name] = float(np.max(np.abs(g[idx] - r[idx]) /
+                                           np.maximum(np.abs(r[idx]), 1.0)))
+            else:
+                worst[name] = float(np.max(cov_rel_err(g[idx], r[idx])))
+    rep["max_rel_err"] = worst
+    # tracks whose smoothed parameters and covariances are bit-identical
+    if ref.smoothed is not None and got.smoothed is not None and len(idx):
+        slot_exact = np.all(got.smoothed[idx] == ref.smoothed[idx], axis=1)
+        if ref.smoothed_cov is not None and got.smoothed_cov is not None:
+            slot_exact &= np.all(got.smoothed_cov[idx] == ref.smoothed_cov[idx], axis=1)
+        owner = np.repeat(np.nonzero(good)[0], ref.n_states[good])
+        bad_tracks = np.unique(owner[~slot_exact])
+        rep["n_bitwise_identical_tracks"] = int(np.count_nonzero(good)) - len(bad_tracks)
+    if ref.state_chi2 is not None and got.state_chi2 is not None and len(idx):
+        rep["max_state_chi2_rel"] = float(np.max(
+            np.abs(got.state_chi2[idx].astype(np.float64) - ref.state_chi2[idx]) /
+            np.maximum(np.abs(ref.state_chi2[idx]), 1.0)))
+    rep["ok"] = (rep["status_mismatch"] == 0 and rep["n_states_mismatch"] == 0
+                 and rep["state_sequence_mismatch_slots"] == 0
+                 and all(v <= (rel_smoothed_cov if k == "smoothed_cov" else rel)
+                         for k, v in worst.items())
+                 and all(rep[f"{k}_mismatch"] == 0 for k in
+                         ("chi2", "ndf", "n_measurements", "n_holes", "n_outliers")))
+    return rep
