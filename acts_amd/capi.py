@@ -10,7 +10,9 @@ import ctypes as C
 import os
 
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG_DIR, "libacts_kf.so")
+# ACTS_KF_LIBRARY selects another build of the same library (e.g. a variant
+# compiled with different flags); default: the in-tree build.
+LIB_PATH = os.environ.get("ACTS_KF_LIBRARY", os.path.join(PKG_DIR, "libacts_kf.so"))
 
 # ---- constants (acts_kf.h) -------------------------------------------------
 OK = 0
@@ -105,7 +107,7 @@ class Options(C.Structure):
         ("abs_pdg", C.c_int32),
         ("mass", C.c_float),
         ("abs_charge", C.c_float),
-        ("reserved", C.c_int32),
+        ("length_sorted", C.c_int32),
     ]
 
 

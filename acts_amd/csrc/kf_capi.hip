@@ -34,13 +34,55 @@ int fail(int code, const std::string& msg) {
 
 constexpr int kBlock = 64;
 
-__global__ __launch_bounds__(kBlock) void fitKernel(
+#ifndef KF_FORWARD_WAVES_PER_EU
+#define KF_FORWARD_WAVES_PER_EU 1
+#endif
+
+// forward filter: one lane per track; lane g fits track perm[g] (tracks
+// ordered by sequence length so that the lanes of a wavefront do similar work)
+__global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKernel(
     DevBatch in, DevOut out, DevScratch sc, const DevSurface* __restrict__ geo,
     const DevSlab* __restrict__ slabs, DevField F, DevOpts o, PhysConst pc,
-    int32_t nSurfaces) {
+    int32_t nSurfaces, const int32_t* __restrict__ perm) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= in.n) return;
+  const int64_t t = perm ? perm[g] : g;
+  forwardTrack(t, g, in, out, sc, geo, slabs, F, o, pc, nSurfaces);
+}
+
+// RTS smoother + track summary + outputs: one lane per track
+__global__ __launch_bounds__(kBlock) void smoothKernel(DevBatch in, DevOut out,
+                                                       DevScratch sc,
+                                                       const int32_t* __restrict__ perm) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool valid = g < in.n;
+  const int64_t t = !valid ? 0 : (perm ? perm[g] : g);
+  smoothTrack(t, g, valid, in, out, sc);
+}
+
+// Counting sort of the tracks by sequence length, longest first.
+__global__ void lengthHistogramKernel(const int64_t* off, int64_t n, int32_t maxLen,
+                                      int32_t* hist) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= in.n) return;
-  fitOneTrack(t, in, out, sc, geo, slabs, F, o, pc, nSurfaces);
+  if (t >= n) return;
+  const int32_t len = min(static_cast<int32_t>(off[t + 1] - off[t]), maxLen);
+  atomicAdd(&hist[len], 1);
+}
+__global__ void lengthScanKernel(int32_t maxLen, int32_t* hist) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int32_t acc = 0;
+  for (int32_t len = maxLen; len >= 0; --len) {
+    const int32_t c = hist[len];
+    hist[len] = acc;
+    acc += c;
+  }
+}
+__global__ void lengthScatterKernel(const int64_t* off, int64_t n, int32_t maxLen,
+                                    int32_t* cursor, int32_t* perm) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int32_t len = min(static_cast<int32_t>(off[t + 1] - off[t]), maxLen);
+  perm[atomicAdd(&cursor[len], 1)] = static_cast<int32_t>(t);
 }
 
 __global__ void maxLengthKernel(const int64_t* off, int64_t n, int32_t* result) {
@@ -98,7 +140,8 @@ struct acts_kf_ctx {
   DevField field{};
   DevOpts opts{};
   PhysConst pc{};
-  DevBuf scratchD, scratchI, lenBuf;
+  DevBuf scratchD, scratchI, scratchT, lenBuf, permBuf, histBuf;
+  bool lengthSorted = true;
   // staging for the host-buffer entry point
   DevBuf inBuf, outBuf;
 };
@@ -128,6 +171,7 @@ void acts_kf_default_options(acts_kf_options* o) {
   o->abs_pdg = 13;
   o->mass = 0.1056583755f;
   o->abs_charge = 1.0f;
+  o->length_sorted = 1;
 }
 
 const char* acts_kf_last_error(void) { return g_lastError.c_str(); }
@@ -258,6 +302,7 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   o.mass = options->mass;
   o.absQ = options->abs_charge;
   c->pc = makePhysConst();
+  c->lengthSorted = options->length_sorted != 0;
   *outCtx = c;
   return ACTS_KF_OK;
 }
@@ -271,7 +316,8 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   const int32_t states = std::max(maxLen, 1);
   const size_t needD = sizeof(double) * static_cast<size_t>(states) * kScratchComps * stride;
   const size_t needI = sizeof(int32_t) * static_cast<size_t>(states) * kIScratchComps * stride;
-  if (c->scratchD.ensure(needD) || c->scratchI.ensure(needI))
+  const size_t needT = sizeof(int32_t) * 2 * stride;
+  if (c->scratchD.ensure(needD) || c->scratchI.ensure(needI) || c->scratchT.ensure(needT))
     return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "scratch allocation failed");
   DevBatch b;
   b.n = n;
@@ -308,11 +354,32 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   DevScratch sc;
   sc.d = static_cast<double*>(c->scratchD.p);
   sc.i = static_cast<int32_t*>(c->scratchI.p);
+  sc.track = static_cast<int32_t*>(c->scratchT.p);
   sc.stride = stride;
   sc.maxStates = states;
   const int64_t blocks = stride / kBlock;
-  hipLaunchKernelGGL(fitKernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream,
-                     b, d, sc, c->geo, c->slabs, c->field, c->opts, c->pc, c->nSurfaces);
+  int32_t* perm = nullptr;
+  if (c->lengthSorted && n > kBlock) {
+    if (c->permBuf.ensure(sizeof(int32_t) * n) ||
+        c->histBuf.ensure(sizeof(int32_t) * (states + 1)))
+      return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "permutation allocation failed");
+    perm = static_cast<int32_t*>(c->permBuf.p);
+    int32_t* hist = static_cast<int32_t*>(c->histBuf.p);
+    KF_HIP(hipMemsetAsync(hist, 0, sizeof(int32_t) * (states + 1), stream));
+    const unsigned nb = static_cast<unsigned>((n + 255) / 256);
+    hipLaunchKernelGGL(lengthHistogramKernel, dim3(nb), dim3(256), 0, stream, in->seq_offsets,
+                       n, states, hist);
+    hipLaunchKernelGGL(lengthScanKernel, dim3(1), dim3(64), 0, stream, states, hist);
+    hipLaunchKernelGGL(lengthScatterKernel, dim3(nb), dim3(256), 0, stream, in->seq_offsets,
+                       n, states, hist, perm);
+    KF_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(fitForwardKernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0,
+                     stream, b, d, sc, c->geo, c->slabs, c->field, c->opts, c->pc,
+                     c->nSurfaces, perm);
+  KF_HIP(hipGetLastError());
+  hipLaunchKernelGGL(smoothKernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream,
+                     b, d, sc, perm);
   KF_HIP(hipGetLastError());
   return ACTS_KF_OK;
 }
@@ -467,7 +534,10 @@ void acts_kf_destroy(acts_kf_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   c->scratchD.release();
   c->scratchI.release();
+  c->scratchT.release();
   c->lenBuf.release();
+  c->permBuf.release();
+  c->histBuf.release();
   c->inBuf.release();
   c->outBuf.release();
   if (c->geo) (void)hipFree(c->geo);

@@ -58,6 +58,11 @@ struct DevSurface {
   int32_t pad;
 };
 
+// Packed symmetric 6x6 matrices: the upper triangle row by row (21 values).
+KF_HD constexpr int sym(int i, int j) {
+  return i <= j ? i * 6 - i * (i - 1) / 2 + (j - i) : j * 6 - j * (j - 1) / 2 + (i - j);
+}
+
 // ---------------------------------------------------------------------------
 // vector helpers (Eigen coefficient order)
 // ---------------------------------------------------------------------------

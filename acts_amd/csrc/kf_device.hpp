@@ -35,16 +35,17 @@ struct DevBatch {
   const double* measCov;
 };
 
-// Scratch of the forward pass, state-major SoA: element c of state k of
-// track t lives at scratch[(k * kScratchComps + c) * stride + t], so that a
-// wavefront writes 64 consecutive doubles per component.
-constexpr int kPred = 0, kPCov = 6, kFilt = 42, kFCov = 48, kJac = 84,
-              kPath = 120, kChi2 = 121, kScratchComps = 122;
+// Scratch of the forward pass (states of every track, read back by the
+// smoother), tiled per wavefront; see sd()/si() for the layout.
+// Covariances are stored as packed upper triangles (kf::sym).
+constexpr int kPred = 0, kPCov = 6, kFilt = 27, kFCov = 33, kJac = 54,
+              kPath = 90, kChi2 = 91, kScratchComps = 92;
 constexpr int kISurf = 0, kIFlags = 1, kIScratchComps = 2;
 
 struct DevScratch {
   double* d;
   int32_t* i;
+  int32_t* track;  // per track: [0] status, [1] tip state index
   int64_t stride;  // >= n, multiple of 64
   int32_t maxStates;
 };
@@ -243,7 +244,7 @@ struct Track {
   double time;
   V3 dir;
   double qop;
-  double cov[36];
+  double cov[21];  // packed symmetric (kf::sym)
   // jacToGlobal: position block rows 0..2 x cols 0..1, direction block
   double jp[6];
   double jd[5];

@@ -108,25 +108,38 @@ __device__ inline void transportToBound(Track& T, const DevSurface& s,
     J[6 * r + 4] = A2[r][3];
     J[6 * r + 5] = (r == 5) ? 1.0 : 0.0;
   }
-  // cov = (J * cov) * J^T
+  // cov = (J * cov) * J^T on the packed symmetric covariance; row 4 of J is
+  // e4 and column 5 is e5, so those products are skipped exactly.
   double JC[36];
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
-      double acc = J[6 * i] * T.cov[j];
+      double acc;
+      if (i == 4) {
+        acc = T.cov[sym(4, j)];
+      } else {
+        acc = J[6 * i] * T.cov[sym(0, j)];
 #pragma unroll
-      for (int k = 1; k < 6; ++k) acc = acc + J[6 * i + k] * T.cov[6 * k + j];
+        for (int k = 1; k < 5; ++k) acc = acc + J[6 * i + k] * T.cov[sym(k, j)];
+        if (i == 5) acc = acc + T.cov[sym(5, j)];
+      }
       JC[6 * i + j] = acc;
     }
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      double acc = JC[6 * i] * J[6 * j];
+    for (int j = i; j < 6; ++j) {
+      double acc;
+      if (j == 4) {
+        acc = JC[6 * i + 4];
+      } else {
+        acc = JC[6 * i] * J[6 * j];
 #pragma unroll
-      for (int k = 1; k < 6; ++k) acc = acc + JC[6 * i + k] * J[6 * j + k];
-      T.cov[6 * i + j] = acc;
+        for (int k = 1; k < 5; ++k) acc = acc + JC[6 * i + k] * J[6 * j + k];
+        if (j == 5) acc = acc + JC[6 * i + 5];
+      }
+      T.cov[sym(i, j)] = acc;
     }
   // reinitializeJacobians
   resetTransport(T);
@@ -175,9 +188,9 @@ __device__ inline int materialInteraction(Track& T, const DevSurface& s,
   nextP = fmax(pc.minP, nextP);
   const float signedQ = static_cast<float>(copysign(absQ, static_cast<double>(qopF)));
   T.qop = signedQ / nextP;
-  T.cov[14] = fmax(0., T.cov[14] + varPhi);
-  T.cov[21] = fmax(0., T.cov[21] + varTheta);
-  T.cov[28] = fmax(0., T.cov[28] + varQop);
+  T.cov[sym(2, 2)] = fmax(0., T.cov[sym(2, 2)] + varPhi);
+  T.cov[sym(3, 3)] = fmax(0., T.cov[sym(3, 3)] + varTheta);
+  T.cov[sym(4, 4)] = fmax(0., T.cov[sym(4, 4)] + varQop);
   return 0;
 }
 
@@ -193,19 +206,21 @@ __device__ inline void inverse2(double a, double b, double c, double d,
   out[3] = a * invdet;
 }
 
-// GainMatrixUpdaterImpl.hpp:21-91 for a 2D (loc0, loc1) measurement.
-__device__ inline int gainUpdate(const double x[6], const double P[36],
+// GainMatrixUpdaterImpl.hpp:21-91 for a 2D (loc0, loc1) measurement; P and
+// C are packed symmetric.
+__device__ inline int gainUpdate(const double x[6], const double P[21],
                                  const double m[2], const double V[4],
-                                 bool joseph, double f[6], double C[36],
+                                 bool joseph, double f[6], double C[21],
                                  double& chi2) {
   double Si[4];
-  inverse2(P[0] + V[0], P[1] + V[1], P[6] + V[2], P[7] + V[3], Si);
+  inverse2(P[sym(0, 0)] + V[0], P[sym(0, 1)] + V[1], P[sym(1, 0)] + V[2],
+           P[sym(1, 1)] + V[3], Si);
   double K[12];
   bool nan = false;
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
-    K[2 * i] = P[6 * i] * Si[0] + P[6 * i + 1] * Si[2];
-    K[2 * i + 1] = P[6 * i] * Si[1] + P[6 * i + 1] * Si[3];
+    K[2 * i] = P[sym(i, 0)] * Si[0] + P[sym(i, 1)] * Si[2];
+    K[2 * i + 1] = P[sym(i, 0)] * Si[1] + P[sym(i, 1)] * Si[3];
     nan = nan || isnan(K[2 * i]) || isnan(K[2 * i + 1]);
   }
   if (nan) return ACTS_KF_STATUS_UPDATE_FAILED;
@@ -219,10 +234,10 @@ __device__ inline int gainUpdate(const double x[6], const double P[36],
       const double t0 = (i == 0 ? 1.0 : 0.0) - K[2 * i];
       const double t1 = (i == 1 ? 1.0 : 0.0) - K[2 * i + 1];
 #pragma unroll
-      for (int j = 0; j < 6; ++j) {
-        double acc = t0 * P[j] + t1 * P[6 + j];
-        if (i >= 2) acc = acc + P[6 * i + j];
-        C[6 * i + j] = acc;
+      for (int j = i; j < 6; ++j) {
+        double acc = t0 * P[sym(0, j)] + t1 * P[sym(1, j)];
+        if (i >= 2) acc = acc + P[sym(i, j)];
+        C[sym(i, j)] = acc;
       }
     }
   } else {
@@ -236,22 +251,22 @@ __device__ inline int gainUpdate(const double x[6], const double P[36],
     for (int i = 0; i < 6; ++i)
 #pragma unroll
       for (int j = 0; j < 6; ++j) {
-        double acc = tmp[6 * i] * P[j];
+        double acc = tmp[6 * i] * P[sym(0, j)];
 #pragma unroll
-        for (int k = 1; k < 6; ++k) acc = acc + tmp[6 * i + k] * P[6 * k + j];
+        for (int k = 1; k < 6; ++k) acc = acc + tmp[6 * i + k] * P[sym(k, j)];
         TP[6 * i + j] = acc;
       }
 #pragma unroll
     for (int i = 0; i < 6; ++i)
 #pragma unroll
-      for (int j = 0; j < 6; ++j) {
+      for (int j = i; j < 6; ++j) {
         double acc = TP[6 * i] * tmp[6 * j];
 #pragma unroll
         for (int k = 1; k < 6; ++k) acc = acc + TP[6 * i + k] * tmp[6 * j + k];
         const double kv0 = K[2 * i] * V[0] + K[2 * i + 1] * V[2];
         const double kv1 = K[2 * i] * V[1] + K[2 * i + 1] * V[3];
         const double b = kv0 * K[2 * j] + kv1 * K[2 * j + 1];
-        C[6 * i + j] = acc + b;
+        C[sym(i, j)] = acc + b;
       }
   }
   // chi2 on the filtered residual with ((I - H K) V)^-1
@@ -267,11 +282,12 @@ __device__ inline int gainUpdate(const double x[6], const double P[36],
 }
 
 // calculatePredictedChi2 (TrackHelpers.hpp:676-694), 2D measurement
-__device__ inline double predictedChi2(const double x[6], const double P[36],
+__device__ inline double predictedChi2(const double x[6], const double P[21],
                                        const double m[2], const double V[4]) {
   const double r0 = m[0] - x[0], r1 = m[1] - x[1];
   double Si[4];
-  inverse2(P[0] + V[0], P[1] + V[1], P[6] + V[2], P[7] + V[3], Si);
+  inverse2(P[sym(0, 0)] + V[0], P[sym(0, 1)] + V[1], P[sym(1, 0)] + V[2],
+           P[sym(1, 1)] + V[3], Si);
   const double row0 = r0 * Si[0] + r1 * Si[2];
   const double row1 = r0 * Si[1] + r1 * Si[3];
   return row0 * r0 + row1 * r1;
@@ -343,23 +359,31 @@ __device__ inline bool inverse6(const double* A, double* X) {
   return true;
 }
 
-// GainMatrixSmoother::calculate (GainMatrixSmoother.cpp:18-97)
-__device__ inline int smoothStep(const double xf[6], const double Cf[36],
-                                 const double xpN[6], const double CpN[36],
-                                 const double xsN[6], const double CsN[36],
+// GainMatrixSmoother::calculate (GainMatrixSmoother.cpp:18-97); covariances
+// packed symmetric, the jacobian full.
+__device__ inline int smoothStep(const double xf[6], const double Cf[21],
+                                 const double xpN[6], const double CpN[21],
+                                 const double xsN[6], const double CsN[21],
                                  const double JN[36], double xs[6],
-                                 double Cs[36]) {
+                                 double Cs[21]) {
   double CfJt[36], Cpi[36], G[36];
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
-      double acc = Cf[6 * i] * JN[6 * j];
+      double acc = Cf[sym(i, 0)] * JN[6 * j];
 #pragma unroll
-      for (int k = 1; k < 6; ++k) acc = acc + Cf[6 * i + k] * JN[6 * j + k];
+      for (int k = 1; k < 6; ++k) acc = acc + Cf[sym(i, k)] * JN[6 * j + k];
       CfJt[6 * i + j] = acc;
     }
-  inverse6(CpN, Cpi);
+  {
+    double Cp[36];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) Cp[6 * i + j] = CpN[sym(i, j)];
+    inverse6(Cp, Cpi);
+  }
   bool nan = false;
 #pragma unroll
   for (int i = 0; i < 6; ++i)
@@ -385,26 +409,26 @@ __device__ inline int smoothStep(const double xf[6], const double Cf[36],
     xs[i] = xf[i] + acc;
   }
   normalizePhiTheta(xs[2], xs[3]);
-  double D[36], GD[36];
+  double D[21], GD[36];
 #pragma unroll
-  for (int i = 0; i < 36; ++i) D[i] = CsN[i] - CpN[i];
+  for (int i = 0; i < 21; ++i) D[i] = CsN[i] - CpN[i];
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
-      double acc = G[6 * i] * D[j];
+      double acc = G[6 * i] * D[sym(0, j)];
 #pragma unroll
-      for (int k = 1; k < 6; ++k) acc = acc + G[6 * i + k] * D[6 * k + j];
+      for (int k = 1; k < 6; ++k) acc = acc + G[6 * i + k] * D[sym(k, j)];
       GD[6 * i + j] = acc;
     }
 #pragma unroll
   for (int i = 0; i < 6; ++i)
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
+    for (int j = i; j < 6; ++j) {
       double acc = GD[6 * i] * G[6 * j];
 #pragma unroll
       for (int k = 1; k < 6; ++k) acc = acc + GD[6 * i + k] * G[6 * j + k];
-      Cs[6 * i + j] = Cf[6 * i + j] + acc;
+      Cs[sym(i, j)] = Cf[sym(i, j)] + acc;
     }
   return 0;
 }

@@ -5,7 +5,8 @@
 // lane that reaches its next surface waits for the other lanes of its
 // wavefront before the (expensive) surface update runs for all of them
 // together.  The order of operations seen by each track is unchanged.
-// Backward pass: GainMatrixSmoother from the tip (GainMatrixSmoother.hpp:43-99)
+// Backward pass (a separate kernel, so that each pass gets its own register
+// allocation): GainMatrixSmoother from the tip (GainMatrixSmoother.hpp:43-99)
 // and calculateTrackQuantities (TrackHelpers.hpp:421-445).
 #pragma once
 
@@ -13,14 +14,21 @@
 
 namespace kf {
 
-__device__ inline double& sd(const DevScratch& sc, int k, int c, int64_t t) {
-  return sc.d[(static_cast<int64_t>(k) * kScratchComps + c) * sc.stride + t];
+// Scratch of lane g: wavefront tile w = g / 64 holds its states contiguously,
+// element c of state k at ((w * maxStates + k) * comps + c) * 64 + g % 64.
+// A wavefront's stores of one component are 512 consecutive bytes, and the
+// states of one wavefront stay within a few pages.
+__device__ inline double& sd(const DevScratch& sc, int k, int c, int64_t g) {
+  const int64_t w = g >> 6;
+  return sc.d[((w * sc.maxStates + k) * kScratchComps + c) * 64 + (g & 63)];
 }
-__device__ inline int32_t& si(const DevScratch& sc, int k, int c, int64_t t) {
-  return sc.i[(static_cast<int64_t>(k) * kIScratchComps + c) * sc.stride + t];
+__device__ inline int32_t& si(const DevScratch& sc, int k, int c, int64_t g) {
+  const int64_t w = g >> 6;
+  return sc.i[((w * sc.maxStates + k) * kIScratchComps + c) * 64 + (g & 63)];
 }
 
-__device__ inline void fitOneTrack(const int64_t t, const DevBatch& in,
+// t: the track (input/output index); g: the lane's scratch column.
+__device__ inline void forwardTrack(const int64_t t, const int64_t g, const DevBatch& in,
                                    const DevOut& out, const DevScratch& sc,
                                    const DevSurface* __restrict__ geo,
                                    const DevSlab* __restrict__ slabs,
@@ -39,13 +47,8 @@ __device__ inline void fitOneTrack(const int64_t t, const DevBatch& in,
     badIndex = badIndex || sid < 0 || sid >= nSurfaces;
   }
   if (badIndex) {
-    if (out.status) out.status[t] = ACTS_KF_STATUS_PROPAGATION_FAILURE;
-    if (out.nStates) out.nStates[t] = 0;
-    if (out.chi2) out.chi2[t] = 0.f;
-    if (out.ndf) out.ndf[t] = 0;
-    if (out.nMeas) out.nMeas[t] = 0;
-    if (out.nHoles) out.nHoles[t] = 0;
-    if (out.nOutliers) out.nOutliers[t] = 0;
+    sc.track[g] = ACTS_KF_STATUS_PROPAGATION_FAILURE;
+    sc.track[sc.stride + g] = -1;
     if (out.nSteps) out.nSteps[t] = 0;
     if (out.nTrials) out.nTrials[t] = 0;
     return;
@@ -71,7 +74,9 @@ __device__ inline void fitOneTrack(const int64_t t, const DevBatch& in,
     T.time = bp[5];
     T.qop = bp[4];
 #pragma unroll
-    for (int i = 0; i < 36; ++i) T.cov[i] = in.startCov[36 * t + i];
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = i; j < 6; ++j) T.cov[sym(i, j)] = in.startCov[36 * t + 6 * i + j];
     T.jp[0] = pl[0]; T.jp[1] = pl[1];
     T.jp[2] = pl[3]; T.jp[3] = pl[4];
     T.jp[4] = pl[6]; T.jp[5] = pl[7];
@@ -193,12 +198,12 @@ __device__ inline void fitOneTrack(const int64_t t, const DevBatch& in,
           uint32_t flags = ACTS_KF_STATE_HAS_PARAMETERS;
           if (hasMat) flags |= ACTS_KF_STATE_HAS_MATERIAL;
 #pragma unroll
-          for (int i = 0; i < 6; ++i) sd(sc, k, kPred + i, t) = x[i];
+          for (int i = 0; i < 6; ++i) sd(sc, k, kPred + i, g) = x[i];
 #pragma unroll
-          for (int i = 0; i < 36; ++i) sd(sc, k, kPCov + i, t) = T.cov[i];
+          for (int i = 0; i < 21; ++i) sd(sc, k, kPCov + i, g) = T.cov[i];
 #pragma unroll
-          for (int i = 0; i < 36; ++i) sd(sc, k, kJac + i, t) = J[i];
-          sd(sc, k, kPath, t) = T.pathAcc;
+          for (int i = 0; i < 36; ++i) sd(sc, k, kJac + i, g) = J[i];
+          sd(sc, k, kPath, g) = T.pathAcc;
           float chi2f = 0.f;
           bool measured = false;
           if (mi >= 0) {
@@ -210,7 +215,7 @@ __device__ inline void fitOneTrack(const int64_t t, const DevBatch& in,
             bool outlier = false;
             if (o.chi2Cut > 0) outlier = predictedChi2(x, T.cov, mvv, mcc) > o.chi2Cut;
             if (!outlier) {
-              double f[6], C[36], chi2 = 0;
+              double f[6], C[21], chi2 = 0;
               const int us = gainUpdate(x, T.cov, mvv, mcc, o.useJoseph != 0, f, C, chi2);
               if (us) {
                 status = us;
@@ -218,9 +223,9 @@ __device__ inline void fitOneTrack(const int64_t t, const DevBatch& in,
                 measured = true;
                 chi2f = static_cast<float>(chi2);
 #pragma unroll
-                for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, t) = f[i];
+                for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, g) = f[i];
 #pragma unroll
-                for (int i = 0; i < 36; ++i) sd(sc, k, kFCov + i, t) = C[i];
+                for (int i = 0; i < 21; ++i) sd(sc, k, kFCov + i, g) = C[i];
                 // EigenStepper::update with the filtered parameters
                 const double sinTh = sin(f[3]);
                 T.dir = mk(cos(f[2]) * sinTh, sin(f[2]) * sinTh, cos(f[3]));
@@ -228,7 +233,7 @@ __device__ inline void fitOneTrack(const int64_t t, const DevBatch& in,
                 T.time = f[5];
                 T.qop = f[4];
 #pragma unroll
-                for (int i = 0; i < 36; ++i) T.cov[i] = C[i];
+                for (int i = 0; i < 21; ++i) T.cov[i] = C[i];
                 double b2f[6];
                 positionJacobians(s, T.pos, b2f, nullptr);
 #pragma unroll
@@ -245,13 +250,13 @@ __device__ inline void fitOneTrack(const int64_t t, const DevBatch& in,
           if (!status) {
             if (!measured) {
 #pragma unroll
-              for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, t) = x[i];
+              for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, g) = x[i];
 #pragma unroll
-              for (int i = 0; i < 36; ++i) sd(sc, k, kFCov + i, t) = T.cov[i];
+              for (int i = 0; i < 21; ++i) sd(sc, k, kFCov + i, g) = T.cov[i];
             }
-            sd(sc, k, kChi2, t) = chi2f;
-            si(sc, k, kISurf, t) = reached;
-            si(sc, k, kIFlags, t) = static_cast<int32_t>(flags);
+            sd(sc, k, kChi2, g) = chi2f;
+            si(sc, k, kISurf, g) = reached;
+            si(sc, k, kIFlags, g) = static_cast<int32_t>(flags);
             nStates = k + 1;
             if (mi >= 0) tip = k;
             if (hasMat) status = materialInteraction(T, s, slabs[s.mat], pc, o);
@@ -276,75 +281,101 @@ __device__ inline void fitOneTrack(const int64_t t, const DevBatch& in,
   if (out.nSteps) out.nSteps[t] = T.nSteps;
   if (out.nTrials) out.nTrials[t] = T.nTrials;
   if (!status && nMeasStates == 0) status = ACTS_KF_STATUS_NO_MEASUREMENT_FOUND;
+  sc.track[g] = status;
+  sc.track[sc.stride + g] = tip;
+}
 
-  // ---- smoother + track summary, tip backwards ----
+// Expand a packed symmetric matrix into a row-major 6x6 output.
+__device__ inline void storeSym(double* dst, const double* p) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) dst[6 * i + j] = p[sym(i, j)];
+}
+
+// GainMatrixSmoother::operator() from the tip backwards, the track summary
+// (calculateTrackQuantities) and the per-state outputs.
+// `valid` is false for the padding lanes of the last wavefront: they take
+// part in the wavefront-wide reduction but touch no memory.
+__device__ inline void smoothTrack(const int64_t t, const int64_t g, const bool valid,
+                                   const DevBatch& in, const DevOut& out,
+                                   const DevScratch& sc) {
+  int status = valid ? sc.track[g] : 0;
+  const int tip = (!valid || status) ? -1 : sc.track[sc.stride + g];
   int nHoles = 0, nOut = 0, nMeasT = 0, ndf = 0;
   float chi2sum = 0.f;
-  if (!status) {
-    double xsN[6], CsN[36];
-    const int64_t base = s0;
-    for (int k = tip; k >= 0; --k) {
-      double xf[6], Cf[36], xs[6], Cs[36];
+  // All lanes walk the same state index k (from the wavefront's largest tip
+  // down), so every scratch load of a wavefront hits one contiguous row.
+  int kTop = tip;
 #pragma unroll
-      for (int i = 0; i < 6; ++i) xf[i] = sd(sc, k, kFilt + i, t);
+  for (int off = 32; off > 0; off >>= 1) kTop = max(kTop, __shfl_xor(kTop, off));
+  const int64_t base = valid ? in.seqOff[t] : 0;
+  double xsN[6], CsN[21];
+  bool active = tip >= 0;
+  for (int k = kTop; k >= 0; --k) {
+    if (!active || k > tip) continue;
+    double xf[6], Cf[21], xs[6], Cs[21];
 #pragma unroll
-      for (int i = 0; i < 36; ++i) Cf[i] = sd(sc, k, kFCov + i, t);
-      if (k == tip) {
+    for (int i = 0; i < 6; ++i) xf[i] = sd(sc, k, kFilt + i, g);
 #pragma unroll
-        for (int i = 0; i < 6; ++i) xs[i] = xf[i];
+    for (int i = 0; i < 21; ++i) Cf[i] = sd(sc, k, kFCov + i, g);
+    if (k == tip) {
 #pragma unroll
-        for (int i = 0; i < 36; ++i) Cs[i] = Cf[i];
-      } else {
-        double xpN[6], CpN[36], JN[36];
+      for (int i = 0; i < 6; ++i) xs[i] = xf[i];
 #pragma unroll
-        for (int i = 0; i < 6; ++i) xpN[i] = sd(sc, k + 1, kPred + i, t);
+      for (int i = 0; i < 21; ++i) Cs[i] = Cf[i];
+    } else {
+      double xpN[6], CpN[21], JN[36];
 #pragma unroll
-        for (int i = 0; i < 36; ++i) CpN[i] = sd(sc, k + 1, kPCov + i, t);
+      for (int i = 0; i < 6; ++i) xpN[i] = sd(sc, k + 1, kPred + i, g);
 #pragma unroll
-        for (int i = 0; i < 36; ++i) JN[i] = sd(sc, k + 1, kJac + i, t);
-        const int r = smoothStep(xf, Cf, xpN, CpN, xsN, CsN, JN, xs, Cs);
-        if (r) {
-          status = r;
-          break;
-        }
+      for (int i = 0; i < 21; ++i) CpN[i] = sd(sc, k + 1, kPCov + i, g);
+#pragma unroll
+      for (int i = 0; i < 36; ++i) JN[i] = sd(sc, k + 1, kJac + i, g);
+      const int r = smoothStep(xf, Cf, xpN, CpN, xsN, CsN, JN, xs, Cs);
+      if (r) {
+        status = r;
+        active = false;
+        continue;
       }
-      const uint32_t flags = static_cast<uint32_t>(si(sc, k, kIFlags, t));
-      const float c2 = static_cast<float>(sd(sc, k, kChi2, t));
-      if (flags & ACTS_KF_STATE_IS_HOLE) {
-        ++nHoles;
-      } else if (flags & ACTS_KF_STATE_IS_OUTLIER) {
-        ++nOut;
-      } else if (flags & ACTS_KF_STATE_HAS_MEASUREMENT) {
-        ++nMeasT;
-        chi2sum += c2;
-        ndf += 2;
-      }
-      const int64_t slot = base + k;
-      if (out.smoothed)
-        for (int i = 0; i < 6; ++i) out.smoothed[6 * slot + i] = xs[i];
-      if (out.smoothedCov)
-        for (int i = 0; i < 36; ++i) out.smoothedCov[36 * slot + i] = Cs[i];
-      if (out.filtered)
-        for (int i = 0; i < 6; ++i) out.filtered[6 * slot + i] = xf[i];
-      if (out.filteredCov)
-        for (int i = 0; i < 36; ++i) out.filteredCov[36 * slot + i] = Cf[i];
-      if (out.predicted)
-        for (int i = 0; i < 6; ++i) out.predicted[6 * slot + i] = sd(sc, k, kPred + i, t);
-      if (out.predictedCov)
-        for (int i = 0; i < 36; ++i)
-          out.predictedCov[36 * slot + i] = sd(sc, k, kPCov + i, t);
-      if (out.jacobian)
-        for (int i = 0; i < 36; ++i) out.jacobian[36 * slot + i] = sd(sc, k, kJac + i, t);
-      if (out.stateSurface) out.stateSurface[slot] = si(sc, k, kISurf, t);
-      if (out.stateFlags) out.stateFlags[slot] = flags;
-      if (out.statePath) out.statePath[slot] = sd(sc, k, kPath, t);
-      if (out.stateChi2) out.stateChi2[slot] = c2;
-#pragma unroll
-      for (int i = 0; i < 6; ++i) xsN[i] = xs[i];
-#pragma unroll
-      for (int i = 0; i < 36; ++i) CsN[i] = Cs[i];
     }
+    const uint32_t flags = static_cast<uint32_t>(si(sc, k, kIFlags, g));
+    const float c2 = static_cast<float>(sd(sc, k, kChi2, g));
+    if (flags & ACTS_KF_STATE_IS_HOLE) {
+      ++nHoles;
+    } else if (flags & ACTS_KF_STATE_IS_OUTLIER) {
+      ++nOut;
+    } else if (flags & ACTS_KF_STATE_HAS_MEASUREMENT) {
+      ++nMeasT;
+      chi2sum += c2;
+      ndf += 2;
+    }
+    const int64_t slot = base + k;
+    if (out.smoothed)
+      for (int i = 0; i < 6; ++i) out.smoothed[6 * slot + i] = xs[i];
+    if (out.smoothedCov) storeSym(out.smoothedCov + 36 * slot, Cs);
+    if (out.filtered)
+      for (int i = 0; i < 6; ++i) out.filtered[6 * slot + i] = xf[i];
+    if (out.filteredCov) storeSym(out.filteredCov + 36 * slot, Cf);
+    if (out.predicted)
+      for (int i = 0; i < 6; ++i) out.predicted[6 * slot + i] = sd(sc, k, kPred + i, g);
+    if (out.predictedCov) {
+      double P[21];
+      for (int i = 0; i < 21; ++i) P[i] = sd(sc, k, kPCov + i, g);
+      storeSym(out.predictedCov + 36 * slot, P);
+    }
+    if (out.jacobian)
+      for (int i = 0; i < 36; ++i) out.jacobian[36 * slot + i] = sd(sc, k, kJac + i, g);
+    if (out.stateSurface) out.stateSurface[slot] = si(sc, k, kISurf, g);
+    if (out.stateFlags) out.stateFlags[slot] = flags;
+    if (out.statePath) out.statePath[slot] = sd(sc, k, kPath, g);
+    if (out.stateChi2) out.stateChi2[slot] = c2;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) xsN[i] = xs[i];
+#pragma unroll
+    for (int i = 0; i < 21; ++i) CsN[i] = Cs[i];
   }
+  if (!valid) return;
   if (out.status) out.status[t] = status;
   if (out.nStates) out.nStates[t] = status ? 0 : tip + 1;
   if (out.chi2) out.chi2[t] = status ? 0.f : chi2sum;

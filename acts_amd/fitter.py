@@ -42,6 +42,7 @@ class KalmanFitterOptions:
     abs_pdg: int = 13
     mass: float = MUON_MASS
     abs_charge: float = 1.0
+    length_sorted: bool = True              # scheduling only; results unchanged
 
     def struct(self) -> capi.Options:
         o = capi.Options()

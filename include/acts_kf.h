@@ -153,7 +153,10 @@ typedef struct acts_kf_options {
   int32_t abs_pdg;             /* 13 = muon */
   float mass;                  /* GeV */
   float abs_charge;            /* e */
-  int32_t reserved;
+  /* scheduling (no reference counterpart): 1 = fit the tracks of a batch
+   * in order of decreasing sequence length (default), 0 = input order.
+   * Results do not depend on it.                                           */
+  int32_t length_sorted;
 } acts_kf_options;
 
 /* ---- one batch of tracks ------------------------------------------------- */
