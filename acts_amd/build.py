@@ -55,5 +55,25 @@ def build_library(verbose: bool = False, extra_flags: list[str] | None = None) -
     return LIB_PATH
 
 
+FACADE_SRC = os.path.join(REPO_DIR, "tests", "cpp", "facade_check.cpp")
+FACADE_BIN = os.path.join(REPO_DIR, "tests", "cpp", "facade_check")
+
+
+def build_facade_check(verbose: bool = False) -> str:
+    """Compile the C++ facade example against the built library (host only)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I", os.path.join(REPO_DIR, "include"),
+           "-o", FACADE_BIN + ".tmp", FACADE_SRC, "-L", PKG_DIR, "-lacts_kf",
+           "-Wl,-rpath," + PKG_DIR, "-Wl,-rpath,$ORIGIN/../../acts_amd"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building facade_check")
+    os.replace(FACADE_BIN + ".tmp", FACADE_BIN)
+    return FACADE_BIN
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_facade_check(verbose=True))

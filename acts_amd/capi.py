@@ -108,6 +108,8 @@ class Options(C.Structure):
         ("mass", C.c_float),
         ("abs_charge", C.c_float),
         ("length_sorted", C.c_int32),
+        ("sched_inner_steps", C.c_int32),
+        ("sched_wait_64ths", C.c_int32),
     ]
 
 

@@ -33,21 +33,57 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int kBlock = 64;
+// the track queue counts in int32 and may overshoot n by one per lane
+constexpr int64_t kMaxBatch = (int64_t{1} << 31) - (int64_t{1} << 24);
 
 #ifndef KF_FORWARD_WAVES_PER_EU
 #define KF_FORWARD_WAVES_PER_EU 1
 #endif
 
-// forward filter: one lane per track; lane g fits track perm[g] (tracks
-// ordered by sequence length so that the lanes of a wavefront do similar work)
+// Forward filter, persistent: each wavefront keeps its 64 lanes busy with
+// tracks taken from a global queue (`next`).  Per outer iteration: idle lanes
+// take new tracks; stepping lanes step until no lane steps any more, or at
+// least `waitFrac`/64 of the busy lanes wait at a surface, or `maxInner`
+// steps have been taken; then the waiting lanes run the surface update
+// together, and finished lanes write their summary and go idle.
 __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKernel(
-    DevBatch in, DevOut out, DevScratch sc, const DevSurface* __restrict__ geo,
-    const DevSlab* __restrict__ slabs, DevField F, DevOpts o, PhysConst pc,
-    int32_t nSurfaces, const int32_t* __restrict__ perm) {
-  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= in.n) return;
-  const int64_t t = perm ? perm[g] : g;
-  forwardTrack(t, g, in, out, sc, geo, slabs, F, o, pc, nSurfaces);
+    FwdArgs a, int32_t* __restrict__ next, int32_t maxInner, int32_t waitFrac) {
+  const int lane = static_cast<int>(threadIdx.x) & 63;
+  const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const int64_t n = a.in.n;
+  Lane L;
+  int phase = kPhaseIdle;
+  while (true) {
+    const uint64_t idle = __ballot(phase == kPhaseIdle);
+    if (idle) {
+      const int cnt = __popcll(idle);
+      const int leader = __ffsll(static_cast<long long>(idle)) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(next, cnt);
+      base = __shfl(base, leader);
+      if (phase == kPhaseIdle) {
+        const int64_t t = static_cast<int64_t>(base) + __popcll(idle & lanesBelow);
+        phase = (t < n) ? laneInit(L, t, a) : kPhaseRetired;
+      }
+    }
+    if (phase == kPhaseFinished) {
+      laneFinish(L, a);
+      phase = kPhaseIdle;
+    }
+    const uint64_t busy = __ballot(phase != kPhaseRetired);
+    if (!busy) break;
+    for (int it = 0; it < maxInner; ++it) {
+      if (phase == kPhaseStepping) phase = laneStep(L, a);
+      const int nStep = __popcll(__ballot(phase == kPhaseStepping));
+      const int nWait = __popcll(__ballot(phase == kPhaseSurface || phase == kPhaseFinished));
+      if (nStep == 0 || 64 * nWait >= waitFrac * (nStep + nWait)) break;
+    }
+    if (phase == kPhaseSurface) phase = laneFilter(L, a);
+    if (phase == kPhaseFinished) {
+      laneFinish(L, a);
+      phase = kPhaseIdle;
+    }
+  }
 }
 
 // RTS smoother + track summary + outputs: one lane per track
@@ -57,7 +93,12 @@ __global__ __launch_bounds__(kBlock) void smoothKernel(DevBatch in, DevOut out,
   const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool valid = g < in.n;
   const int64_t t = !valid ? 0 : (perm ? perm[g] : g);
-  smoothTrack(t, g, valid, in, out, sc);
+  smoothTrack(t, valid, in, out, sc);
+}
+
+__global__ void zeroKernel(int32_t* p, int32_t count) {
+  const int i = static_cast<int>(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < count) p[i] = 0;
 }
 
 // Counting sort of the tracks by sequence length, longest first.
@@ -140,8 +181,12 @@ struct acts_kf_ctx {
   DevField field{};
   DevOpts opts{};
   PhysConst pc{};
-  DevBuf scratchD, scratchI, scratchT, lenBuf, permBuf, histBuf;
+  DevBuf scratchD, scratchI, scratchT, lenBuf, permBuf, histBuf, queueBuf;
   bool lengthSorted = true;
+  // persistent forward kernel: resident wavefronts and scheduling knobs
+  int64_t residentWaves = 0;
+  int32_t innerSteps = 16;
+  int32_t waitFrac = 32;
   // staging for the host-buffer entry point
   DevBuf inBuf, outBuf;
 };
@@ -303,6 +348,20 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   o.absQ = options->abs_charge;
   c->pc = makePhysConst();
   c->lengthSorted = options->length_sorted != 0;
+  if (options->sched_inner_steps > 0) c->innerSteps = options->sched_inner_steps;
+  if (options->sched_wait_64ths > 0) c->waitFrac = std::min(options->sched_wait_64ths, 64);
+  {
+    int blocksPerCu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocksPerCu, fitForwardKernel, kBlock,
+                                                     0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) !=
+            hipSuccess ||
+        blocksPerCu <= 0 || cus <= 0) {
+      acts_kf_destroy(c);
+      return fail(ACTS_KF_ERR_DEVICE, "occupancy query failed");
+    }
+    c->residentWaves = static_cast<int64_t>(blocksPerCu) * cus;
+  }
   *outCtx = c;
   return ACTS_KF_OK;
 }
@@ -312,12 +371,14 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
                      int32_t maxLen) {
   const int64_t n = in->n_tracks;
   if (n == 0) return ACTS_KF_OK;
-  const int64_t stride = (n + kBlock - 1) / kBlock * kBlock;
+  if (n > kMaxBatch) return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "batch too large");
+  const int64_t stride = n;
   const int32_t states = std::max(maxLen, 1);
-  const size_t needD = sizeof(double) * static_cast<size_t>(states) * kScratchComps * stride;
-  const size_t needI = sizeof(int32_t) * static_cast<size_t>(states) * kIScratchComps * stride;
-  const size_t needT = sizeof(int32_t) * 2 * stride;
-  if (c->scratchD.ensure(needD) || c->scratchI.ensure(needI) || c->scratchT.ensure(needT))
+  const size_t needD = sizeof(double) * static_cast<size_t>(states) * kScratchComps * n;
+  const size_t needI = sizeof(int32_t) * static_cast<size_t>(states) * kIScratchComps * n;
+  const size_t needT = sizeof(int32_t) * 2 * n;
+  if (c->scratchD.ensure(needD) || c->scratchI.ensure(needI) || c->scratchT.ensure(needT) ||
+      c->queueBuf.ensure(sizeof(int32_t)))
     return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "scratch allocation failed");
   DevBatch b;
   b.n = n;
@@ -357,7 +418,7 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   sc.track = static_cast<int32_t*>(c->scratchT.p);
   sc.stride = stride;
   sc.maxStates = states;
-  const int64_t blocks = stride / kBlock;
+  const int64_t blocks = (n + kBlock - 1) / kBlock;
   int32_t* perm = nullptr;
   if (c->lengthSorted && n > kBlock) {
     if (c->permBuf.ensure(sizeof(int32_t) * n) ||
@@ -374,9 +435,21 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
                        n, states, hist, perm);
     KF_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(fitForwardKernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0,
-                     stream, b, d, sc, c->geo, c->slabs, c->field, c->opts, c->pc,
-                     c->nSurfaces, perm);
+  FwdArgs fa;
+  fa.in = b;
+  fa.out = d;
+  fa.sc = sc;
+  fa.geo = c->geo;
+  fa.slabs = c->slabs;
+  fa.F = c->field;
+  fa.o = c->opts;
+  fa.pc = c->pc;
+  fa.nSurfaces = c->nSurfaces;
+  int32_t* queue = static_cast<int32_t*>(c->queueBuf.p);
+  hipLaunchKernelGGL(zeroKernel, dim3(1), dim3(64), 0, stream, queue, 1);
+  const int64_t waves = std::min(blocks, c->residentWaves);
+  hipLaunchKernelGGL(fitForwardKernel, dim3(static_cast<unsigned>(waves)), dim3(kBlock), 0,
+                     stream, fa, queue, c->innerSteps, c->waitFrac);
   KF_HIP(hipGetLastError());
   hipLaunchKernelGGL(smoothKernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream,
                      b, d, sc, perm);
@@ -538,6 +611,7 @@ void acts_kf_destroy(acts_kf_ctx* c) {
   c->lenBuf.release();
   c->permBuf.release();
   c->histBuf.release();
+  c->queueBuf.release();
   c->inBuf.release();
   c->outBuf.release();
   if (c->geo) (void)hipFree(c->geo);

@@ -1,10 +1,13 @@
 // kf_kernel.hpp -- the fit kernel: one lane per track.
 //
 // Forward pass: Propagator::propagate with the KalmanFitter actor
-// (Propagator.ipp:25-210, KalmanFitter.hpp:339-671), restructured so that a
-// lane that reaches its next surface waits for the other lanes of its
-// wavefront before the (expensive) surface update runs for all of them
-// together.  The order of operations seen by each track is unchanged.
+// (Propagator.ipp:25-210, KalmanFitter.hpp:339-671), cut into phases (init,
+// step, surface update, finish) that a persistent kernel schedules per
+// wavefront: lanes that reach their next surface wait until enough lanes of
+// the wavefront are waiting, then the (expensive) surface update runs for
+// all of them together, and a lane whose track is done takes the next track
+// from a global queue.  The order of operations seen by each track is
+// unchanged.
 // Backward pass (a separate kernel, so that each pass gets its own register
 // allocation): GainMatrixSmoother from the tip (GainMatrixSmoother.hpp:43-99)
 // and calculateTrackQuantities (TrackHelpers.hpp:421-445).
@@ -14,57 +17,103 @@
 
 namespace kf {
 
-// Scratch of lane g: wavefront tile w = g / 64 holds its states contiguously,
-// element c of state k at ((w * maxStates + k) * comps + c) * 64 + g % 64.
-// A wavefront's stores of one component are 512 consecutive bytes, and the
-// states of one wavefront stay within a few pages.
-__device__ inline double& sd(const DevScratch& sc, int k, int c, int64_t g) {
-  const int64_t w = g >> 6;
-  return sc.d[((w * sc.maxStates + k) * kScratchComps + c) * 64 + (g & 63)];
+// Scratch record of state k of track t: kScratchComps doubles (and
+// kIScratchComps ints) stored contiguously, so that a lane writes and reads
+// whole cache lines of its own track.
+__device__ inline double& sd(const DevScratch& sc, int k, int c, int64_t t) {
+  return sc.d[(t * sc.maxStates + k) * kScratchComps + c];
 }
-__device__ inline int32_t& si(const DevScratch& sc, int k, int c, int64_t g) {
-  const int64_t w = g >> 6;
-  return sc.i[((w * sc.maxStates + k) * kIScratchComps + c) * 64 + (g & 63)];
+__device__ inline int32_t& si(const DevScratch& sc, int k, int c, int64_t t) {
+  return sc.i[(t * sc.maxStates + k) * kIScratchComps + c];
 }
 
-// t: the track (input/output index); g: the lane's scratch column.
-__device__ inline void forwardTrack(const int64_t t, const int64_t g, const DevBatch& in,
-                                   const DevOut& out, const DevScratch& sc,
-                                   const DevSurface* __restrict__ geo,
-                                   const DevSlab* __restrict__ slabs,
-                                   const DevField& F, const DevOpts& o,
-                                   const PhysConst& pc, const int32_t nSurfaces) {
+// Phases of a lane of the persistent forward kernel.
+enum LanePhase : int {
+  kPhaseIdle = 0,     // needs a track
+  kPhaseStepping = 1, // propagating towards its target surface
+  kPhaseSurface = 2,  // reached its target surface, filter pending
+  kPhaseFinished = 3, // forward pass over, summary pending
+  kPhaseRetired = 4,  // no tracks left
+};
+
+struct SeqAt {
+  const int32_t* p;
+  int64_t s0;
+  __device__ int operator()(int i) const { return p[s0 + i]; }
+};
+
+// Everything one track needs between two phases of the forward pass.
+struct Lane {
+  int64_t t;
+  int64_t m0;
+  int nMeas, nUnique;
+  SeqAt seq;
+  Track T;
+  Nav nv;
+  double propLimit, kfLimit;
+  int nMeasStates, nStates, tip;
+  bool finished;
+  int status;
+  uint32_t iter;
+  int reached;
+};
+
+struct FwdArgs {
+  DevBatch in;
+  DevOut out;
+  DevScratch sc;
+  const DevSurface* __restrict__ geo;
+  const DevSlab* __restrict__ slabs;
+  DevField F;
+  DevOpts o;
+  PhysConst pc;
+  int32_t nSurfaces;
+};
+
+// EigenStepper::initialize (EigenStepper.ipp:37-70), DirectNavigator
+// initialisation and the pre-propagation act of the KalmanFitter actor.
+// Returns the lane's next phase.
+__device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a) {
+  const DevBatch& in = a.in;
+  const DevOpts& o = a.o;
+  L.t = t;
   const int64_t s0 = in.seqOff[t];
   const int seqLen = static_cast<int>(in.seqOff[t + 1] - s0);
-  const int64_t m0 = in.measOff[t];
-  const int nMeas = static_cast<int>(in.measOff[t + 1] - m0);
-  auto seqAt = [&](int i) { return in.seqSurf[s0 + i]; };
+  L.m0 = in.measOff[t];
+  L.nMeas = static_cast<int>(in.measOff[t + 1] - L.m0);
+  L.seq.p = in.seqSurf;
+  L.seq.s0 = s0;
+  L.status = 0;
+  L.nMeasStates = 0;
+  L.nStates = 0;
+  L.tip = -1;
+  L.iter = 0;
+  L.reached = -1;
+  L.finished = false;
+  L.T.nSteps = 0;
+  L.T.nTrials = 0;
   // surface indices come from the caller: reject the track if any is out of
   // range instead of reading outside the geometry table
   bool badIndex = false;
   for (int i = 0; i < seqLen; ++i) {
-    const int sid = seqAt(i);
-    badIndex = badIndex || sid < 0 || sid >= nSurfaces;
+    const int sid = L.seq(i);
+    badIndex = badIndex || sid < 0 || sid >= a.nSurfaces;
   }
   if (badIndex) {
-    sc.track[g] = ACTS_KF_STATUS_PROPAGATION_FAILURE;
-    sc.track[sc.stride + g] = -1;
-    if (out.nSteps) out.nSteps[t] = 0;
-    if (out.nTrials) out.nTrials[t] = 0;
-    return;
+    L.status = ACTS_KF_STATUS_PROPAGATION_FAILURE;
+    return kPhaseFinished;
   }
-
   // unique measurement surfaces (KalmanFitter.hpp:750-755, first one wins)
   int nUnique = 0;
-  for (int i = 0; i < nMeas; ++i) {
-    const int si_ = in.measSurf[m0 + i];
+  for (int i = 0; i < L.nMeas; ++i) {
+    const int si_ = in.measSurf[L.m0 + i];
     bool dup = false;
-    for (int j = 0; j < i; ++j) dup = dup || (in.measSurf[m0 + j] == si_);
+    for (int j = 0; j < i; ++j) dup = dup || (in.measSurf[L.m0 + j] == si_);
     nUnique += dup ? 0 : 1;
   }
+  L.nUnique = nUnique;
 
-  // EigenStepper::initialize (EigenStepper.ipp:37-70)
-  Track T;
+  Track& T = L.T;
   {
     const double* bp = in.startParams + 6 * t;
     const double* pl = in.startPlane + 12 * t;
@@ -83,206 +132,190 @@ __device__ inline void forwardTrack(const int64_t t, const int64_t g, const DevB
     directionB2F(T.dir, T.jd);
     resetTransport(T);
     T.pathAcc = 0;
-    T.nSteps = 0;
-    T.nTrials = 0;
     T.cNav = kNotSet;
     T.cActor = kNotSet;
     T.cUser = o.maxStepSize;
     T.cAcc = o.initialStepSize;
   }
+  L.nv.seqPos = -1;
+  L.nv.seqLen = seqLen;
+  L.nv.navBreak = false;
+  L.nv.tSurf = -1;
+  L.nv.tIdx = 0;
 
-  Nav nv;
-  nv.seqPos = -1;
-  nv.seqLen = seqLen;
-  nv.navBreak = false;
-  nv.tSurf = -1;
-  nv.tIdx = 0;
-
-  double propLimit = o.pathLimit;
-  loopProtection(T, F, o, propLimit, false);
-  double kfLimit = DBL_MAX;
-  int nMeasStates = 0, nStates = 0, tip = -1;
-  bool finished = false;
-  int status = 0;
-
+  L.propLimit = o.pathLimit;
+  loopProtection(T, a.F, o, L.propLimit, false);
+  L.kfLimit = DBL_MAX;
   // pre-propagation act (the start surface carries neither material nor a
   // measurement, so only the finalisation checks run)
-  if (kfLimit == DBL_MAX) loopProtection(T, F, o, kfLimit, true);
-  {
-    const bool complete = nMeasStates == nUnique;
-    const bool pathReached = pathAbort(T, kfLimit, o.surfaceTolerance);
-    finished = complete || pathReached;
-  }
-  bool done = finished || pathAbort(T, propLimit, o.surfaceTolerance);
-  if (!done) {
-    status = getNextTarget(nv, T, geo, o, seqAt);
-    done = status != 0;
-  }
-  uint32_t iter = 0;
+  loopProtection(T, a.F, o, L.kfLimit, true);
+  L.finished = (L.nMeasStates == L.nUnique) || pathAbort(T, L.kfLimit, o.surfaceTolerance);
+  if (L.finished || pathAbort(T, L.propLimit, o.surfaceTolerance)) return kPhaseFinished;
+  L.status = getNextTarget(L.nv, T, a.geo, o, L.seq);
+  return L.status ? kPhaseFinished : kPhaseStepping;
+}
 
-  while (!done) {
-    // ---- stepping until this lane reaches a surface or stops ----
-    int reached = -1;
-    while (true) {
-      if (iter >= o.maxSteps) {
-        status = ACTS_KF_STATUS_STEP_COUNT_LIMIT;
-        done = true;
-        break;
-      }
-      ++iter;
-      const int sst = rkStep(T, F, o);
-      if (sst) {
-        status = sst;
-        done = true;
-        break;
-      }
-      T.cNav = kNotSet;
-      T.cActor = kNotSet;
-      if (nv.tSurf >= 0) {
-        const int ps = updateSurfaceStatus(T, geo[nv.tSurf], nv.tIdx, o.surfaceTolerance);
-        if (ps == kOnSurface && !nv.navBreak) reached = nv.tSurf;
-        if (ps != kReachable) nv.tSurf = -1;
-      }
-      if (reached >= 0) break;
-      // act without a current surface: only the finalisation checks
-      if (!finished) {
-        if (kfLimit == DBL_MAX) loopProtection(T, F, o, kfLimit, true);
-        const bool complete = nMeasStates == nUnique;
-        const bool pathReached = pathAbort(T, kfLimit, o.surfaceTolerance);
-        finished = complete || pathReached;
-      }
-      if (finished || pathAbort(T, propLimit, o.surfaceTolerance)) {
-        done = true;
-        break;
-      }
-      if (nv.tSurf < 0) {
-        T.cNav = kNotSet;
-        status = getNextTarget(nv, T, geo, o, seqAt);
-        if (status) {
-          done = true;
-          break;
-        }
-      }
-    }
-    if (reached < 0) break;
-
-    // ---- KalmanFitter::Actor::filter on the reached surface ----
-    if (!finished) {
-      if (kfLimit == DBL_MAX) loopProtection(T, F, o, kfLimit, true);
-      const DevSurface& s = geo[reached];
-      int mi = -1;
-      for (int i = 0; i < nMeas; ++i)
-        if (mi < 0 && in.measSurf[m0 + i] == reached) mi = i;
-      const bool hasMat = s.mat >= 0;
-      const bool sensitive = (s.flags & ACTS_KF_SURFACE_SENSITIVE) != 0;
-      const bool preceding = nMeasStates > 0;
-      if (mi >= 0 || (preceding && sensitive) || hasMat) {
-        double l0 = 0, l1 = 0;
-        const bool onS = globalToLocal(s, T.pos, l0, l1);
-        double J[36];
-        transportToBound(T, s, onS, J);
-        if (!onS) {
-          status = ACTS_KF_STATUS_NOT_ON_SURFACE;
-        } else if (nStates >= sc.maxStates) {
-          status = ACTS_KF_STATUS_PROPAGATION_FAILURE;
-        } else {
-          // bound parameters (transformFreeToBoundParameters)
-          double x[6];
-          x[0] = l0;
-          x[1] = l1;
-          x[2] = atan2(T.dir.y, T.dir.x);
-          x[3] = atan2(perp(T.dir), T.dir.z);
-          x[4] = T.qop;
-          x[5] = T.time;
-          const int k = nStates;
-          uint32_t flags = ACTS_KF_STATE_HAS_PARAMETERS;
-          if (hasMat) flags |= ACTS_KF_STATE_HAS_MATERIAL;
-#pragma unroll
-          for (int i = 0; i < 6; ++i) sd(sc, k, kPred + i, g) = x[i];
-#pragma unroll
-          for (int i = 0; i < 21; ++i) sd(sc, k, kPCov + i, g) = T.cov[i];
-#pragma unroll
-          for (int i = 0; i < 36; ++i) sd(sc, k, kJac + i, g) = J[i];
-          sd(sc, k, kPath, g) = T.pathAcc;
-          float chi2f = 0.f;
-          bool measured = false;
-          if (mi >= 0) {
-            flags |= ACTS_KF_STATE_HAS_MEASUREMENT;
-            const double* mv = in.measVal + 2 * (m0 + mi);
-            const double* mc = in.measCov + 4 * (m0 + mi);
-            const double mvv[2] = {mv[0], mv[1]};
-            const double mcc[4] = {mc[0], mc[1], mc[2], mc[3]};
-            bool outlier = false;
-            if (o.chi2Cut > 0) outlier = predictedChi2(x, T.cov, mvv, mcc) > o.chi2Cut;
-            if (!outlier) {
-              double f[6], C[21], chi2 = 0;
-              const int us = gainUpdate(x, T.cov, mvv, mcc, o.useJoseph != 0, f, C, chi2);
-              if (us) {
-                status = us;
-              } else {
-                measured = true;
-                chi2f = static_cast<float>(chi2);
-#pragma unroll
-                for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, g) = f[i];
-#pragma unroll
-                for (int i = 0; i < 21; ++i) sd(sc, k, kFCov + i, g) = C[i];
-                // EigenStepper::update with the filtered parameters
-                const double sinTh = sin(f[3]);
-                T.dir = mk(cos(f[2]) * sinTh, sin(f[2]) * sinTh, cos(f[3]));
-                T.pos = localToGlobal(s, f[0], f[1]);
-                T.time = f[5];
-                T.qop = f[4];
-#pragma unroll
-                for (int i = 0; i < 21; ++i) T.cov[i] = C[i];
-                double b2f[6];
-                positionJacobians(s, T.pos, b2f, nullptr);
-#pragma unroll
-                for (int i = 0; i < 6; ++i) T.jp[i] = b2f[i];
-                directionB2F(T.dir, T.jd);
-                ++nMeasStates;
-              }
-            } else {
-              flags |= ACTS_KF_STATE_IS_OUTLIER;
-            }
-          } else if (sensitive && preceding) {
-            flags |= ACTS_KF_STATE_IS_HOLE;
-          }
-          if (!status) {
-            if (!measured) {
-#pragma unroll
-              for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, g) = x[i];
-#pragma unroll
-              for (int i = 0; i < 21; ++i) sd(sc, k, kFCov + i, g) = T.cov[i];
-            }
-            sd(sc, k, kChi2, g) = chi2f;
-            si(sc, k, kISurf, g) = reached;
-            si(sc, k, kIFlags, g) = static_cast<int32_t>(flags);
-            nStates = k + 1;
-            if (mi >= 0) tip = k;
-            if (hasMat) status = materialInteraction(T, s, slabs[s.mat], pc, o);
-          }
-        }
-      }
-      if (status) {
-        done = true;
-      } else {
-        const bool complete = nMeasStates == nUnique;
-        const bool pathReached = pathAbort(T, kfLimit, o.surfaceTolerance);
-        finished = complete || pathReached;
-      }
-    }
-    if (done) break;
-    if (finished || pathAbort(T, propLimit, o.surfaceTolerance)) break;
+// One iteration of the propagation loop (Propagator.ipp:25-210): step, target
+// status, the actor's finalisation checks, aborters, next target.
+__device__ inline int laneStep(Lane& L, const FwdArgs& a) {
+  const DevOpts& o = a.o;
+  Track& T = L.T;
+  if (L.iter >= o.maxSteps) {
+    L.status = ACTS_KF_STATUS_STEP_COUNT_LIMIT;
+    return kPhaseFinished;
+  }
+  ++L.iter;
+  const int sst = rkStep(T, a.F, o);
+  if (sst) {
+    L.status = sst;
+    return kPhaseFinished;
+  }
+  T.cNav = kNotSet;
+  T.cActor = kNotSet;
+  if (L.nv.tSurf >= 0) {
+    const int ps = updateSurfaceStatus(T, a.geo[L.nv.tSurf], L.nv.tIdx, o.surfaceTolerance);
+    if (ps == kOnSurface && !L.nv.navBreak) L.reached = L.nv.tSurf;
+    if (ps != kReachable) L.nv.tSurf = -1;
+  }
+  if (L.reached >= 0) return kPhaseSurface;
+  // act without a current surface: only the finalisation checks
+  if (!L.finished) {
+    if (L.kfLimit == DBL_MAX) loopProtection(T, a.F, o, L.kfLimit, true);
+    L.finished = (L.nMeasStates == L.nUnique) || pathAbort(T, L.kfLimit, o.surfaceTolerance);
+  }
+  if (L.finished || pathAbort(T, L.propLimit, o.surfaceTolerance)) return kPhaseFinished;
+  if (L.nv.tSurf < 0) {
     T.cNav = kNotSet;
-    status = getNextTarget(nv, T, geo, o, seqAt);
-    if (status) break;
+    L.status = getNextTarget(L.nv, T, a.geo, o, L.seq);
+    if (L.status) return kPhaseFinished;
   }
+  return kPhaseStepping;
+}
 
-  if (out.nSteps) out.nSteps[t] = T.nSteps;
-  if (out.nTrials) out.nTrials[t] = T.nTrials;
-  if (!status && nMeasStates == 0) status = ACTS_KF_STATUS_NO_MEASUREMENT_FOUND;
-  sc.track[g] = status;
-  sc.track[sc.stride + g] = tip;
+// KalmanFitter::Actor::filter on the reached surface (KalmanFitter.hpp:
+// 339-671), then the rest of the propagation-loop iteration.
+__device__ inline int laneFilter(Lane& L, const FwdArgs& a) {
+  const DevBatch& in = a.in;
+  const DevOpts& o = a.o;
+  const DevScratch& sc = a.sc;
+  Track& T = L.T;
+  const int reached = L.reached;
+  const int64_t t = L.t;
+  L.reached = -1;
+  if (!L.finished) {
+    if (L.kfLimit == DBL_MAX) loopProtection(T, a.F, o, L.kfLimit, true);
+    const DevSurface& s = a.geo[reached];
+    int mi = -1;
+    for (int i = 0; i < L.nMeas; ++i)
+      if (mi < 0 && in.measSurf[L.m0 + i] == reached) mi = i;
+    const bool hasMat = s.mat >= 0;
+    const bool sensitive = (s.flags & ACTS_KF_SURFACE_SENSITIVE) != 0;
+    const bool preceding = L.nMeasStates > 0;
+    if (mi >= 0 || (preceding && sensitive) || hasMat) {
+      double l0 = 0, l1 = 0;
+      const bool onS = globalToLocal(s, T.pos, l0, l1);
+      double J[36];
+      transportToBound(T, s, onS, J);
+      if (!onS) {
+        L.status = ACTS_KF_STATUS_NOT_ON_SURFACE;
+      } else if (L.nStates >= sc.maxStates) {
+        L.status = ACTS_KF_STATUS_PROPAGATION_FAILURE;
+      } else {
+        // bound parameters (transformFreeToBoundParameters)
+        double x[6];
+        x[0] = l0;
+        x[1] = l1;
+        x[2] = atan2(T.dir.y, T.dir.x);
+        x[3] = atan2(perp(T.dir), T.dir.z);
+        x[4] = T.qop;
+        x[5] = T.time;
+        const int k = L.nStates;
+        uint32_t flags = ACTS_KF_STATE_HAS_PARAMETERS;
+        if (hasMat) flags |= ACTS_KF_STATE_HAS_MATERIAL;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) sd(sc, k, kPred + i, t) = x[i];
+#pragma unroll
+        for (int i = 0; i < 21; ++i) sd(sc, k, kPCov + i, t) = T.cov[i];
+#pragma unroll
+        for (int i = 0; i < 36; ++i) sd(sc, k, kJac + i, t) = J[i];
+        sd(sc, k, kPath, t) = T.pathAcc;
+        float chi2f = 0.f;
+        bool measured = false;
+        if (mi >= 0) {
+          flags |= ACTS_KF_STATE_HAS_MEASUREMENT;
+          const double* mv = in.measVal + 2 * (L.m0 + mi);
+          const double* mc = in.measCov + 4 * (L.m0 + mi);
+          const double mvv[2] = {mv[0], mv[1]};
+          const double mcc[4] = {mc[0], mc[1], mc[2], mc[3]};
+          bool outlier = false;
+          if (o.chi2Cut > 0) outlier = predictedChi2(x, T.cov, mvv, mcc) > o.chi2Cut;
+          if (!outlier) {
+            double f[6], C[21], chi2 = 0;
+            const int us = gainUpdate(x, T.cov, mvv, mcc, o.useJoseph != 0, f, C, chi2);
+            if (us) {
+              L.status = us;
+            } else {
+              measured = true;
+              chi2f = static_cast<float>(chi2);
+#pragma unroll
+              for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, t) = f[i];
+#pragma unroll
+              for (int i = 0; i < 21; ++i) sd(sc, k, kFCov + i, t) = C[i];
+              // EigenStepper::update with the filtered parameters
+              const double sinTh = sin(f[3]);
+              T.dir = mk(cos(f[2]) * sinTh, sin(f[2]) * sinTh, cos(f[3]));
+              T.pos = localToGlobal(s, f[0], f[1]);
+              T.time = f[5];
+              T.qop = f[4];
+#pragma unroll
+              for (int i = 0; i < 21; ++i) T.cov[i] = C[i];
+              double b2f[6];
+              positionJacobians(s, T.pos, b2f, nullptr);
+#pragma unroll
+              for (int i = 0; i < 6; ++i) T.jp[i] = b2f[i];
+              directionB2F(T.dir, T.jd);
+              ++L.nMeasStates;
+            }
+          } else {
+            flags |= ACTS_KF_STATE_IS_OUTLIER;
+          }
+        } else if (sensitive && preceding) {
+          flags |= ACTS_KF_STATE_IS_HOLE;
+        }
+        if (!L.status) {
+          if (!measured) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, t) = x[i];
+#pragma unroll
+            for (int i = 0; i < 21; ++i) sd(sc, k, kFCov + i, t) = T.cov[i];
+          }
+          sd(sc, k, kChi2, t) = chi2f;
+          si(sc, k, kISurf, t) = reached;
+          si(sc, k, kIFlags, t) = static_cast<int32_t>(flags);
+          L.nStates = k + 1;
+          if (mi >= 0) L.tip = k;
+          if (hasMat) L.status = materialInteraction(T, s, a.slabs[s.mat], a.pc, o);
+        }
+      }
+    }
+    if (L.status) return kPhaseFinished;
+    L.finished = (L.nMeasStates == L.nUnique) || pathAbort(T, L.kfLimit, o.surfaceTolerance);
+  }
+  if (L.finished || pathAbort(T, L.propLimit, o.surfaceTolerance)) return kPhaseFinished;
+  T.cNav = kNotSet;
+  L.status = getNextTarget(L.nv, T, a.geo, o, L.seq);
+  return L.status ? kPhaseFinished : kPhaseStepping;
+}
+
+// End of the forward pass of a track: counters and the smoother's inputs.
+__device__ inline void laneFinish(Lane& L, const FwdArgs& a) {
+  const int64_t t = L.t;
+  int status = L.status;
+  if (a.out.nSteps) a.out.nSteps[t] = L.T.nSteps;
+  if (a.out.nTrials) a.out.nTrials[t] = L.T.nTrials;
+  if (!status && L.nMeasStates == 0) status = ACTS_KF_STATUS_NO_MEASUREMENT_FOUND;
+  a.sc.track[t] = status;
+  a.sc.track[a.sc.stride + t] = L.tip;
 }
 
 // Expand a packed symmetric matrix into a row-major 6x6 output.
@@ -297,15 +330,14 @@ __device__ inline void storeSym(double* dst, const double* p) {
 // (calculateTrackQuantities) and the per-state outputs.
 // `valid` is false for the padding lanes of the last wavefront: they take
 // part in the wavefront-wide reduction but touch no memory.
-__device__ inline void smoothTrack(const int64_t t, const int64_t g, const bool valid,
-                                   const DevBatch& in, const DevOut& out,
-                                   const DevScratch& sc) {
-  int status = valid ? sc.track[g] : 0;
-  const int tip = (!valid || status) ? -1 : sc.track[sc.stride + g];
+__device__ inline void smoothTrack(const int64_t t, const bool valid, const DevBatch& in,
+                                   const DevOut& out, const DevScratch& sc) {
+  int status = valid ? sc.track[t] : 0;
+  const int tip = (!valid || status) ? -1 : sc.track[sc.stride + t];
   int nHoles = 0, nOut = 0, nMeasT = 0, ndf = 0;
   float chi2sum = 0.f;
   // All lanes walk the same state index k (from the wavefront's largest tip
-  // down), so every scratch load of a wavefront hits one contiguous row.
+  // down), which keeps the lanes of a wavefront converged.
   int kTop = tip;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) kTop = max(kTop, __shfl_xor(kTop, off));
@@ -316,9 +348,9 @@ __device__ inline void smoothTrack(const int64_t t, const int64_t g, const bool 
     if (!active || k > tip) continue;
     double xf[6], Cf[21], xs[6], Cs[21];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) xf[i] = sd(sc, k, kFilt + i, g);
+    for (int i = 0; i < 6; ++i) xf[i] = sd(sc, k, kFilt + i, t);
 #pragma unroll
-    for (int i = 0; i < 21; ++i) Cf[i] = sd(sc, k, kFCov + i, g);
+    for (int i = 0; i < 21; ++i) Cf[i] = sd(sc, k, kFCov + i, t);
     if (k == tip) {
 #pragma unroll
       for (int i = 0; i < 6; ++i) xs[i] = xf[i];
@@ -327,11 +359,11 @@ __device__ inline void smoothTrack(const int64_t t, const int64_t g, const bool 
     } else {
       double xpN[6], CpN[21], JN[36];
 #pragma unroll
-      for (int i = 0; i < 6; ++i) xpN[i] = sd(sc, k + 1, kPred + i, g);
+      for (int i = 0; i < 6; ++i) xpN[i] = sd(sc, k + 1, kPred + i, t);
 #pragma unroll
-      for (int i = 0; i < 21; ++i) CpN[i] = sd(sc, k + 1, kPCov + i, g);
+      for (int i = 0; i < 21; ++i) CpN[i] = sd(sc, k + 1, kPCov + i, t);
 #pragma unroll
-      for (int i = 0; i < 36; ++i) JN[i] = sd(sc, k + 1, kJac + i, g);
+      for (int i = 0; i < 36; ++i) JN[i] = sd(sc, k + 1, kJac + i, t);
       const int r = smoothStep(xf, Cf, xpN, CpN, xsN, CsN, JN, xs, Cs);
       if (r) {
         status = r;
@@ -339,8 +371,8 @@ __device__ inline void smoothTrack(const int64_t t, const int64_t g, const bool 
         continue;
       }
     }
-    const uint32_t flags = static_cast<uint32_t>(si(sc, k, kIFlags, g));
-    const float c2 = static_cast<float>(sd(sc, k, kChi2, g));
+    const uint32_t flags = static_cast<uint32_t>(si(sc, k, kIFlags, t));
+    const float c2 = static_cast<float>(sd(sc, k, kChi2, t));
     if (flags & ACTS_KF_STATE_IS_HOLE) {
       ++nHoles;
     } else if (flags & ACTS_KF_STATE_IS_OUTLIER) {
@@ -358,17 +390,17 @@ __device__ inline void smoothTrack(const int64_t t, const int64_t g, const bool 
       for (int i = 0; i < 6; ++i) out.filtered[6 * slot + i] = xf[i];
     if (out.filteredCov) storeSym(out.filteredCov + 36 * slot, Cf);
     if (out.predicted)
-      for (int i = 0; i < 6; ++i) out.predicted[6 * slot + i] = sd(sc, k, kPred + i, g);
+      for (int i = 0; i < 6; ++i) out.predicted[6 * slot + i] = sd(sc, k, kPred + i, t);
     if (out.predictedCov) {
       double P[21];
-      for (int i = 0; i < 21; ++i) P[i] = sd(sc, k, kPCov + i, g);
+      for (int i = 0; i < 21; ++i) P[i] = sd(sc, k, kPCov + i, t);
       storeSym(out.predictedCov + 36 * slot, P);
     }
     if (out.jacobian)
-      for (int i = 0; i < 36; ++i) out.jacobian[36 * slot + i] = sd(sc, k, kJac + i, g);
-    if (out.stateSurface) out.stateSurface[slot] = si(sc, k, kISurf, g);
+      for (int i = 0; i < 36; ++i) out.jacobian[36 * slot + i] = sd(sc, k, kJac + i, t);
+    if (out.stateSurface) out.stateSurface[slot] = si(sc, k, kISurf, t);
     if (out.stateFlags) out.stateFlags[slot] = flags;
-    if (out.statePath) out.statePath[slot] = sd(sc, k, kPath, g);
+    if (out.statePath) out.statePath[slot] = sd(sc, k, kPath, t);
     if (out.stateChi2) out.stateChi2[slot] = c2;
 #pragma unroll
     for (int i = 0; i < 6; ++i) xsN[i] = xs[i];

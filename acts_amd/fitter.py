@@ -42,7 +42,10 @@ class KalmanFitterOptions:
     abs_pdg: int = 13
     mass: float = MUON_MASS
     abs_charge: float = 1.0
-    length_sorted: bool = True              # scheduling only; results unchanged
+    # scheduling only; results do not depend on these
+    length_sorted: bool = True
+    sched_inner_steps: int = 0              # 0 = library default
+    sched_wait_64ths: int = 0               # 0 = library default
 
     def struct(self) -> capi.Options:
         o = capi.Options()

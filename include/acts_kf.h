@@ -153,10 +153,16 @@ typedef struct acts_kf_options {
   int32_t abs_pdg;             /* 13 = muon */
   float mass;                  /* GeV */
   float abs_charge;            /* e */
-  /* scheduling (no reference counterpart): 1 = fit the tracks of a batch
-   * in order of decreasing sequence length (default), 0 = input order.
-   * Results do not depend on it.                                           */
+  /* scheduling (no reference counterpart; results do not depend on it):
+   * length_sorted     1 = the smoother visits tracks in order of decreasing
+   *                   sequence length (default), 0 = input order;
+   * sched_inner_steps most RK steps the forward kernel takes before the
+   *                   lanes waiting at a surface are updated (0 = 16);
+   * sched_wait_64ths  the surface update runs early once this many 64ths of
+   *                   the busy lanes of a wavefront wait (0 = 32).          */
   int32_t length_sorted;
+  int32_t sched_inner_steps;
+  int32_t sched_wait_64ths;
 } acts_kf_options;
 
 /* ---- one batch of tracks ------------------------------------------------- */
@@ -212,7 +218,10 @@ typedef struct acts_kf_batch_out {
 
 typedef struct acts_kf_ctx acts_kf_ctx;
 
-/* Fill *opt with the reference defaults (see the field comments). */
+/* Fill *opt with the reference defaults (see the field comments): the
+ * default-constructed KalmanFitterOptions / PropagatorPlainOptions /
+ * StepperPlainOptions (KalmanFitter.hpp:118-175, PropagatorOptions.hpp:26-52,
+ * StepperOptions.hpp:36-49) and ParticleHypothesis::muon().                */
 void acts_kf_default_options(acts_kf_options* opt);
 
 /* Create a fitting context on HIP device `device`: uploads the geometry,
@@ -225,25 +234,38 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t n_surfaces,
 
 /* Fit a batch whose buffers (in and out) are host memory.  Asynchronous on
  * `stream` (a hipStream_t, NULL = the context's own stream); results are in
- * the output buffers after acts_kf_sync.                                    */
+ * the output buffers after acts_kf_sync.  Replaces one call of
+ * KalmanFitter::fit (KalmanFitter.hpp:721-731, the DirectNavigator overload
+ * with the surface sequence) per track of the batch, as issued by
+ * KalmanFitterFunctionImpl (Examples/Algorithms/TrackFitting/src/
+ * KalmanFitterFunction.cpp:150-157).                                        */
 int acts_kf_fit_batch(acts_kf_ctx* ctx, const acts_kf_batch_in* in,
                       const acts_kf_batch_out* out, void* stream);
 
 /* Fit a batch whose buffers are all device memory on the context's device
- * (inputs already resident in HBM).  Asynchronous on `stream`.             */
+ * (inputs already resident in HBM).  Asynchronous on `stream`.  Same
+ * reference call as acts_kf_fit_batch (KalmanFitter.hpp:721-731).          */
 int acts_kf_fit_batch_device(acts_kf_ctx* ctx, const acts_kf_batch_in* in,
                              const acts_kf_batch_out* out, void* stream);
 
-/* Wait for all work the context queued. */
+/* Wait for all work the context queued.  The reference's fit() returns
+ * with its result (KalmanFitter.hpp:721-731); a batched, asynchronous fit
+ * makes that completion point explicit.                                    */
 int acts_kf_sync(acts_kf_ctx* ctx);
 
-/* Release the context and its device memory. */
+/* Release the context and its device memory (the end of life of the
+ * KalmanFitter / Propagator / field objects of the reference).             */
 void acts_kf_destroy(acts_kf_ctx* ctx);
 
-/* Message for a per-track status code (the reference's error messages). */
+/* Message for a per-track status code: the reference's error messages
+ * (std::error_category::message of KalmanFitterError.cpp:21,
+ * PropagatorError.cpp, EigenStepperError.cpp, SurfaceError.cpp,
+ * MagneticFieldError.cpp).                                                  */
 const char* acts_kf_status_string(int32_t status);
 
-/* Message for the last API error of this thread. */
+/* Message for the last API error of this thread (the reference reports
+ * configuration errors by exceptions, VoidFitterComponents.hpp:22-27; a C
+ * ABI returns a code and keeps the message here).                           */
 const char* acts_kf_last_error(void);
 
 #ifdef __cplusplus
