@@ -6,13 +6,13 @@ GainMatrixSmoother, run as hand-written HIP kernels (one lane per track)
 behind the C ABI of include/acts_kf.h.
 """
 from . import capi  # noqa: F401
-from .detector import (ConstantField, Detector, barrel_detector,  # noqa: F401
+from .detector import (ConstantField, Detector, FieldMap, barrel_detector,  # noqa: F401
                        barrel_endcap_detector)
 from .events import Events, generate, gen_config, workload  # noqa: F401
 from .fitter import FitResult, KalmanFitter, KalmanFitterOptions  # noqa: F401
 
 __all__ = [
-    "capi", "ConstantField", "Detector", "barrel_detector", "barrel_endcap_detector",
+    "capi", "ConstantField", "Detector", "FieldMap", "barrel_detector", "barrel_endcap_detector",
     "Events", "generate", "gen_config", "workload", "FitResult", "KalmanFitter",
     "KalmanFitterOptions",
 ]

@@ -171,6 +171,16 @@ class GenConfig(C.Structure):
     ]
 
 
+class Solenoid(C.Structure):
+    _fields_ = [
+        ("radius", C.c_double),
+        ("length", C.c_double),
+        ("n_coils", C.c_int32),
+        ("reserved", C.c_int32),
+        ("b_mag_center", C.c_double),
+    ]
+
+
 # symbol -> (restype, argtypes); every declaration in include/*.h
 SIGNATURES = {
     "acts_kf_default_options": (None, [C.POINTER(Options)]),
@@ -192,6 +202,9 @@ SIGNATURES = {
                                  C.POINTER(C.c_int64)]),
     "acts_kf_gen_fetch": (None, [C.c_void_p] + [C.c_void_p] * 12),
     "acts_kf_gen_free": (None, [C.c_void_p]),
+    "acts_kf_solenoid_map": (C.c_int, [C.POINTER(Solenoid), C.POINTER(C.c_int32),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.c_int32]),
 }
 
 _lib = None

@@ -13,6 +13,7 @@
 
 #include "../../include/acts_kf_gen.h"
 #include "kf_device.hpp"
+#include "kf_field_host.hpp"
 
 using namespace kf;
 
@@ -52,7 +53,8 @@ struct GenWorld {
   std::vector<DevSurface> surf;
   std::vector<DevSlab> slab;
   std::vector<uint32_t> flags;
-  double B[3];
+  DevField F{};
+  std::vector<double> grid;  // host copy of a field map (F.v points here)
   float mass, absQ;
   PhysConst pc;
 };
@@ -67,23 +69,31 @@ bool insideBounds(const DevSurface& s, const V3& pos) {
   return true;
 }
 
-// One RKN4 step of at most h (shrunk until the error estimate is accepted);
-// returns the step taken.
-double rkStepTruth(V3& pos, V3& dir, double qop, const double* B, double h,
+// One RKN4 step of at most h (shrunk until the error estimate is accepted),
+// with the field evaluated at the three RKN4 points; returns the step taken,
+// or NaN if the field lookup failed (the particle left the field map).
+double rkStepTruth(V3& pos, V3& dir, double qop, const DevField& F, double h,
                    double& accuracy) {
-  const V3 Bv = mk(B[0], B[1], B[2]);
-  const V3 c1 = cross(dir, Bv);
+  V3 B1, B2, B3;
+  if (fieldAt(F, pos, B1)) return NAN;
+  const V3 c1 = cross(dir, B1);
   const V3 k1 = mk(qop * c1.x, qop * c1.y, qop * c1.z);
   double h2 = 0, err = 0;
   V3 k2, k3, k4;
   for (int trial = 0; trial < 100; ++trial) {
     h2 = h * h;
     const double hh = 0.5 * h;
-    V3 a = cross(mk(dir.x + hh * k1.x, dir.y + hh * k1.y, dir.z + hh * k1.z), Bv);
+    const V3 p1 = mk(pos.x + hh * dir.x + h2 * 0.125 * k1.x, pos.y + hh * dir.y + h2 * 0.125 * k1.y,
+                     pos.z + hh * dir.z + h2 * 0.125 * k1.z);
+    if (fieldAt(F, p1, B2)) return NAN;
+    V3 a = cross(mk(dir.x + hh * k1.x, dir.y + hh * k1.y, dir.z + hh * k1.z), B2);
     k2 = mk(qop * a.x, qop * a.y, qop * a.z);
-    a = cross(mk(dir.x + hh * k2.x, dir.y + hh * k2.y, dir.z + hh * k2.z), Bv);
+    a = cross(mk(dir.x + hh * k2.x, dir.y + hh * k2.y, dir.z + hh * k2.z), B2);
     k3 = mk(qop * a.x, qop * a.y, qop * a.z);
-    a = cross(mk(dir.x + h * k3.x, dir.y + h * k3.y, dir.z + h * k3.z), Bv);
+    const V3 p2 = mk(pos.x + h * dir.x + h2 * 0.5 * k3.x, pos.y + h * dir.y + h2 * 0.5 * k3.y,
+                     pos.z + h * dir.z + h2 * 0.5 * k3.z);
+    if (fieldAt(F, p2, B3)) return NAN;
+    a = cross(mk(dir.x + h * k3.x, dir.y + h * k3.y, dir.z + h * k3.z), B3);
     k4 = mk(qop * a.x, qop * a.y, qop * a.z);
     err = h2 * (std::fabs(k1.x - k2.x - k3.x + k4.x) + std::fabs(k1.y - k2.y - k3.y + k4.y) +
                 std::fabs(k1.z - k2.z - k3.z + k4.z));
@@ -171,7 +181,9 @@ void generateTrack(const GenWorld& w, const acts_kf_gen_config& cfg, int64_t ind
     std::vector<char> crossed(nS, 0);
     double accuracy = 10000.0;
     double path = 0.0;
-    const double Bn = std::sqrt(w.B[0] * w.B[0] + w.B[1] * w.B[1] + w.B[2] * w.B[2]);
+    V3 B0;
+    if (fieldAt(w.F, pos, B0)) continue;  // vertex outside the field map
+    const double Bn = std::sqrt(B0.x * B0.x + B0.y * B0.y + B0.z * B0.z);
     const double loopLimit = Bn > 0 ? 0.5 * kTwoPi * p / Bn : 1e9;
     int target = -1;
     for (int it = 0; it < 2000 && path < loopLimit; ++it) {
@@ -207,7 +219,8 @@ void generateTrack(const GenWorld& w, const acts_kf_gen_config& cfg, int64_t ind
       }
       if (std::fabs(dist) >= kOnSurfaceTolerance) {
         const double h = std::min(dist, accuracy);
-        const double taken = rkStepTruth(pos, dir, qop, w.B, h, accuracy);
+        const double taken = rkStepTruth(pos, dir, qop, w.F, h, accuracy);
+        if (std::isnan(taken)) break;  // left the field map: the track ends
         const double pp = 1.0 / std::fabs(qop);
         const double mp = w.mass / pp;
         time += taken * std::sqrt(1 + mp * mp);
@@ -274,6 +287,89 @@ void generateTrack(const GenWorld& w, const acts_kf_gen_config& cfg, int64_t ind
   }
 }
 
+// SolenoidBField (SolenoidBField.cpp:24-162).  The reference passes k^2 to
+// Boost's ellint_1/ellint_2, which take the modulus, so the integrals here
+// are K and E of modulus k^2.
+void ellipticKE(double k, double& K, double& E) {
+  // arithmetic-geometric mean (Abramowitz & Stegun 17.6)
+  double a = 1.0, b = std::sqrt(1.0 - k * k), c = k;
+  double sum = 0.5 * c * c, pow2 = 0.5;
+  // stop once c has reached the rounding level of a: further terms are
+  // below 1e-30, and a - b only oscillates in the last bit from there on
+  for (int i = 0; i < 64 && std::fabs(c) > 1e-15 * a; ++i) {
+    const double an = 0.5 * (a + b);
+    const double bn = std::sqrt(a * b);
+    c = 0.5 * (a - b);
+    a = an;
+    b = bn;
+    pow2 *= 2.0;
+    sum += pow2 * c * c;
+  }
+  K = kPi / (2.0 * a);
+  E = K * (1.0 - sum);
+}
+
+struct Solenoid {
+  double radius, length, dz, R2, scale;
+  int nCoils;
+  Solenoid(double radius_, double length_, int nCoils_, double bMagCenter)
+      : radius(radius_), length(length_), nCoils(nCoils_) {
+    dz = length / nCoils;
+    R2 = radius * radius;
+    double br, bz;
+    multiCoil(0.0, 0.0, 1.0, br, bz);
+    scale = bMagCenter / std::sqrt(br * br + bz * bz);
+  }
+  double k2(double r, double z) const {
+    return 4 * radius * r / ((radius + r) * (radius + r) + z * z);
+  }
+  void multiCoil(double r, double z, double sc, double& br, double& bz) const {
+    br = 0;
+    bz = 0;
+    for (int coil = 0; coil < nCoils; ++coil) {
+      const double zs = z + length * 0.5 - dz * (coil + 0.5);
+      double vr, vz;
+      coilField(r, zs, sc, vr, vz);
+      br += vr;
+      bz += vz;
+    }
+  }
+  // B_r and B_z of one coil, the elliptic integrals shared between the two
+  void coilField(double rs, double z, double sc, double& vr, double& vz) const {
+    const double r = std::fabs(rs);
+    if (r == 0) {
+      vr = 0.;
+      vz = sc / 2. * R2 / (std::sqrt(R2 + z * z) * (R2 + z * z));
+      return;
+    }
+    const double k_2 = k2(r, z);
+    const double k = std::sqrt(k_2);
+    double K, E;
+    ellipticKE(k_2, K, E);
+    const double cr = sc * k * z / (4 * kPi * std::sqrt(radius * r * r * r));
+    const double Bsr = (2. - k_2) / (2. - 2. * k_2) * E - K;
+    vr = r / rs * cr * Bsr;
+    const double cz = sc * k / (4 * kPi * std::sqrt(radius * r));
+    const double Bsz = ((radius + r) * k_2 - 2. * r) / (2. * r * (1. - k_2)) * E + K;
+    vz = cz * Bsz;
+  }
+  void rzField(double r, double z, double& br, double& bz) const {
+    multiCoil(r, z, scale, br, bz);
+  }
+  // getField: (0, 0, B_z) plus B_r along the normalised (x, y, 0)
+  void compose(double x, double y, double r, double br, double bz, double* out) const {
+    out[0] = 0;
+    out[1] = 0;
+    out[2] = bz;
+    if (r != 0.) {
+      const double n = std::sqrt((x * x + y * y) + 0.0 * 0.0);
+      out[0] = out[0] + (x / n) * br;
+      out[1] = out[1] + (y / n) * br;
+      out[2] = out[2] + (0.0 / n) * br;
+    }
+  }
+};
+
 }  // namespace
 
 struct acts_kf_gen {
@@ -289,8 +385,16 @@ int acts_kf_gen_run(const acts_kf_gen_config* cfg, const acts_kf_surface* surfac
                     const acts_kf_options* particle, acts_kf_gen** out) {
   if (!cfg || !surfaces || nSurfaces <= 0 || !field || !particle || !out || cfg->n_tracks < 0)
     return ACTS_KF_ERR_INVALID_ARGUMENT;
-  if (field->kind != ACTS_KF_FIELD_CONSTANT) return ACTS_KF_ERR_UNSUPPORTED;
   GenWorld w;
+  if (field->kind == ACTS_KF_FIELD_GRID_XYZ) {
+    if (!buildGridField(*field, w.F, w.grid)) return ACTS_KF_ERR_INVALID_ARGUMENT;
+    w.F.v = w.grid.data();
+  } else if (field->kind == ACTS_KF_FIELD_CONSTANT) {
+    w.F.kind = ACTS_KF_FIELD_CONSTANT;
+    for (int i = 0; i < 3; ++i) w.F.B[i] = field->constant[i];
+  } else {
+    return ACTS_KF_ERR_UNSUPPORTED;
+  }
   w.surf.resize(nSurfaces);
   w.flags.resize(nSurfaces);
   for (int i = 0; i < nSurfaces; ++i) {
@@ -320,7 +424,6 @@ int acts_kf_gen_run(const acts_kf_gen_config* cfg, const acts_kf_surface* surfac
     d.meanI = static_cast<float>(static_cast<double>(1e-9 * 16.0L) * std::pow(m.z, 0.9f));
     d.vacuum = m.ar <= 0.f;
   }
-  for (int i = 0; i < 3; ++i) w.B[i] = field->constant[i];
   w.mass = particle->mass;
   w.absQ = particle->abs_charge;
   w.pc.me = static_cast<float>(static_cast<double>(1e-3 * 0.5109989461L));
@@ -394,5 +497,60 @@ void acts_kf_gen_fetch(const acts_kf_gen* g, double* startParams, double* startC
 }
 
 void acts_kf_gen_free(acts_kf_gen* g) { delete g; }
+
+int acts_kf_solenoid_map(const acts_kf_solenoid* cfg, const int32_t n[3],
+                         const double pmin[3], const double pmax[3], double* values,
+                         int32_t nThreads) {
+  if (!cfg || !n || !pmin || !pmax || !values || cfg->n_coils <= 0 || !(cfg->radius > 0) ||
+      !(cfg->length > 0))
+    return ACTS_KF_ERR_INVALID_ARGUMENT;
+  for (int a = 0; a < 3; ++a)
+    if (n[a] < 2 || !(pmax[a] > pmin[a])) return ACTS_KF_ERR_INVALID_ARGUMENT;
+  const Solenoid sol(cfg->radius, cfg->length, cfg->n_coils, cfg->b_mag_center);
+  std::vector<double> xs[3];
+  for (int a = 0; a < 3; ++a) {
+    const double step = (pmax[a] - pmin[a]) / static_cast<double>(n[a] - 1);
+    xs[a].resize(n[a]);
+    for (int i = 0; i < n[a]; ++i) xs[a][i] = pmin[a] + static_cast<double>(i) * step;
+  }
+  // The (B_r, B_z) part depends on (r, z) only: evaluate it once per distinct
+  // transverse radius of the grid.
+  std::vector<double> rOf(static_cast<size_t>(n[0]) * n[1]);
+  for (int i = 0; i < n[0]; ++i)
+    for (int j = 0; j < n[1]; ++j) rOf[i * n[1] + j] = std::sqrt(xs[0][i] * xs[0][i] + xs[1][j] * xs[1][j]);
+  std::vector<double> radii(rOf);
+  std::sort(radii.begin(), radii.end());
+  radii.erase(std::unique(radii.begin(), radii.end()), radii.end());
+  const int64_t nr = static_cast<int64_t>(radii.size()), nz = n[2];
+  std::vector<double> table(static_cast<size_t>(2 * nr * nz));
+  int nt = nThreads > 0 ? nThreads : static_cast<int>(std::thread::hardware_concurrency());
+  nt = std::max(1, std::min(nt, 64));
+  std::atomic<int64_t> next{0};
+  auto work = [&]() {
+    for (int64_t r = next++; r < nr; r = next++)
+      for (int64_t k = 0; k < nz; ++k)
+        sol.rzField(radii[r], xs[2][k], table[2 * (r * nz + k)], table[2 * (r * nz + k) + 1]);
+  };
+  std::vector<std::thread> pool;
+  for (int i = 0; i < nt; ++i) pool.emplace_back(work);
+  for (auto& th : pool) th.join();
+  std::atomic<int64_t> nextRow{0};
+  auto fill = [&]() {
+    for (int64_t i = nextRow++; i < n[0]; i = nextRow++)
+      for (int j = 0; j < n[1]; ++j) {
+        const double r = rOf[i * n[1] + j];
+        const int64_t ri = std::lower_bound(radii.begin(), radii.end(), r) - radii.begin();
+        for (int k = 0; k < n[2]; ++k) {
+          const double br = table[2 * (ri * nz + k)], bz = table[2 * (ri * nz + k) + 1];
+          double* out = values + 3 * ((i * n[1] + j) * n[2] + k);
+          sol.compose(xs[0][i], xs[1][j], r, br, bz, out);
+        }
+      }
+  };
+  pool.clear();
+  for (int i = 0; i < nt; ++i) pool.emplace_back(fill);
+  for (auto& th : pool) th.join();
+  return ACTS_KF_OK;
+}
 
 }  // extern "C"

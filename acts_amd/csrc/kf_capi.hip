@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "kf_field_host.hpp"
 #include "kf_kernel.hpp"
 
 using namespace kf;
@@ -179,6 +180,7 @@ struct acts_kf_ctx {
   DevSurface* geo = nullptr;
   DevSlab* slabs = nullptr;
   DevField field{};
+  double* fieldValues = nullptr;
   DevOpts opts{};
   PhysConst pc{};
   DevBuf scratchD, scratchI, scratchT, lenBuf, permBuf, histBuf, queueBuf;
@@ -257,8 +259,16 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
     return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: null or empty argument");
   if (nMaterials > 0 && !materials)
     return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: materials is null");
-  if (field->kind != ACTS_KF_FIELD_CONSTANT)
+  if (field->kind == ACTS_KF_FIELD_GRID_XYZ) {
+    if (!field->values)
+      return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: field grid values are null");
+    for (int a = 0; a < 3; ++a)
+      if (field->n[a] < 2 || field->n[a] > (1 << 20) ||
+          !(field->point_max[a] > field->point_min[a]))
+        return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: bad field grid axis");
+  } else if (field->kind != ACTS_KF_FIELD_CONSTANT) {
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: field kind not supported");
+  }
   if (options->precision != ACTS_KF_PRECISION_FP64)
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: precision mode not supported");
   for (int32_t i = 0; i < nSurfaces; ++i) {
@@ -326,6 +336,17 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
     return cleanup(ACTS_KF_ERR_DEVICE, "hipMemcpy geometry failed");
   c->field.kind = field->kind;
   for (int i = 0; i < 3; ++i) c->field.B[i] = field->constant[i];
+  if (field->kind == ACTS_KF_FIELD_GRID_XYZ) {
+    std::vector<double> padded;
+    if (!buildGridField(*field, c->field, padded))
+      return cleanup(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: bad field grid");
+    if (hipMalloc(&c->fieldValues, sizeof(double) * padded.size()) != hipSuccess)
+      return cleanup(ACTS_KF_ERR_OUT_OF_MEMORY, "hipMalloc field map failed");
+    if (hipMemcpy(c->fieldValues, padded.data(), sizeof(double) * padded.size(),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(ACTS_KF_ERR_DEVICE, "hipMemcpy field map failed");
+    c->field.v = c->fieldValues;
+  }
   DevOpts& o = c->opts;
   o.stepTolerance = options->step_tolerance;
   o.stepSizeCutOff = options->step_size_cutoff;
@@ -616,6 +637,7 @@ void acts_kf_destroy(acts_kf_ctx* c) {
   c->outBuf.release();
   if (c->geo) (void)hipFree(c->geo);
   if (c->slabs) (void)hipFree(c->slabs);
+  if (c->fieldValues) (void)hipFree(c->fieldValues);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -221,10 +221,20 @@ KF_HD float theta0(const PhysConst& c, const DevSlab& s, int absPdg, float m,
 // ---------------------------------------------------------------------------
 // field
 // ---------------------------------------------------------------------------
+// Magnetic field: constant, or an xyz grid as built by fieldMapXYZ
+// (BFieldMapUtils.cpp:109-186): per axis nb bins of width `width` starting at
+// `min`, bin b (1-based) holding the value of grid point b - 1; lookups are
+// valid on [min, upper) with upper = the last grid point.
 struct DevField {
   int32_t kind;
   int32_t pad;
   double B[3];
+  int32_t nb[3];
+  int32_t pad2;
+  double min[3], width[3], upper[3];
+  // values of points 0..nb per axis, the extra point nb (the overflow bin)
+  // zero: [(ix * (nb1 + 1) + iy) * (nb2 + 1) + iz][3]
+  const double* __restrict__ v;
 };
 
 }  // namespace kf

@@ -280,9 +280,52 @@ __device__ inline void resetTransport(Track& T) {
   for (int i = 0; i < 7; ++i) T.der[i] = 0.0;
 }
 
-__device__ inline V3 fieldAt(const DevField& F, const V3& p) {
-  (void)p;
-  return mk(F.B[0], F.B[1], F.B[2]);
+// InterpolatedBFieldMap::getField (InterpolatedBFieldMap.hpp:335-366,
+// Grid.hpp:470-494, Interpolation.hpp:96-120): the cell of the point's bin,
+// the last axis collapsed first with (1 - f) * a + f * b.  Returns 0 or
+// FIELD_OUT_OF_BOUNDS.
+KF_HD int gridField(const DevField& F, const V3& p, V3& out) {
+  const double x[3] = {p.x, p.y, p.z};
+  int lo[3];
+  double f[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (!(x[a] >= F.min[a]) || x[a] >= F.upper[a]) return ACTS_KF_STATUS_FIELD_OUT_OF_BOUNDS;
+    // 1 <= bin <= nb: x >= min, and x < upper up to one rounding
+    int bin = static_cast<int>(floor((x[a] - F.min[a]) / F.width[a]) + 1);
+    bin = bin < 1 ? 1 : (bin > F.nb[a] ? F.nb[a] : bin);
+    lo[a] = bin - 1;
+    const double ll = F.min[a] + static_cast<double>(bin - 1) * F.width[a];
+    const double ur = F.min[a] + static_cast<double>(bin) * F.width[a];
+    f[a] = (x[a] - ll) / (ur - ll);
+  }
+  const int64_t sy = F.nb[2] + 1;
+  const int64_t sx = static_cast<int64_t>(F.nb[1] + 1) * (F.nb[2] + 1);
+  const double* base = F.v + 3 * (lo[0] * sx + lo[1] * sy + lo[2]);
+  // corners (x, y) in canonical order, each with its z pair contiguous
+  double c[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double* q = base + 3 * (((i >> 1) & 1) * sx + (i & 1) * sy);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[i][k] = (1 - f[2]) * q[k] + f[2] * q[3 + k];
+  }
+  double d[2][3];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d[i][k] = (1 - f[1]) * c[2 * i][k] + f[1] * c[2 * i + 1][k];
+  out = mk((1 - f[0]) * d[0][0] + f[0] * d[1][0], (1 - f[0]) * d[0][1] + f[0] * d[1][1],
+           (1 - f[0]) * d[0][2] + f[0] * d[1][2]);
+  return 0;
+}
+
+KF_HD int fieldAt(const DevField& F, const V3& p, V3& out) {
+  if (F.kind == ACTS_KF_FIELD_CONSTANT) {
+    out = mk(F.B[0], F.B[1], F.B[2]);
+    return 0;
+  }
+  return gridField(F, p, out);
 }
 
 // EigenStepperDefaultExtension.hpp:40-58
@@ -304,7 +347,8 @@ __device__ inline int rkStep(Track& T, const DevField& F, const DevOpts& o) {
   const V3 pos = T.pos;
   const V3 dir = T.dir;
   const double qop = T.qop;
-  const V3 B1 = fieldAt(F, pos);
+  V3 B1;
+  if (const int fs = fieldAt(F, pos, B1)) return fs;
   const V3 c1 = cross(dir, B1);
   const V3 k1 = mk(qop * c1.x, qop * c1.y, qop * c1.z);
 
@@ -321,14 +365,14 @@ __device__ inline int rkStep(Track& T, const DevField& F, const DevOpts& o) {
     const V3 p1 = mk((pos.x + half_h * dir.x) + h28 * k1.x,
                      (pos.y + half_h * dir.y) + h28 * k1.y,
                      (pos.z + half_h * dir.z) + h28 * k1.z);
-    B2 = fieldAt(F, p1);
+    if (const int fs = fieldAt(F, p1, B2)) return fs;
     k2 = kEval(qop, dir, half_h, k1, B2);
     k3 = kEval(qop, dir, half_h, k2, B2);
     const double h22 = h2 * 0.5;
     const V3 p2 = mk((pos.x + h * dir.x) + h22 * k3.x,
                      (pos.y + h * dir.y) + h22 * k3.y,
                      (pos.z + h * dir.z) + h22 * k3.z);
-    B3 = fieldAt(F, p2);
+    if (const int fs = fieldAt(F, p2, B3)) return fs;
     k4 = kEval(qop, dir, h, k3, B3);
     const double e0 = ((k1.x - k2.x) - k3.x) + k4.x;
     const double e1 = ((k1.y - k2.y) - k3.y) + k4.y;

@@ -505,7 +505,9 @@ __device__ inline void loopProtection(const Track& T, const DevField& F,
                                       bool release) {
   if (!o.loopProtection) return;
   if (release) limit = DBL_MAX;
-  const V3 B = fieldAt(F, T.pos);
+  // a failed lookup leaves the loop protection off (LoopProtection.hpp)
+  V3 B;
+  if (fieldAt(F, T.pos, B)) return;
   const double Bn = sqrt(dot(B, B));
   if (Bn == 0) return;
   const double p = copysign(static_cast<double>(o.absQ), T.qop) / T.qop;

@@ -10,6 +10,8 @@ from __future__ import annotations
 from dataclasses import dataclass, field
 import math
 
+import ctypes as C
+
 import numpy as np
 
 from . import capi
@@ -137,6 +139,53 @@ class ConstantField:
         f.kind = capi.FIELD_CONSTANT
         f.constant[0], f.constant[1], f.constant[2] = self.bx, self.by, self.bz
         return f
+
+
+class FieldMap:
+    """Field values on the points of a regular xyz grid (the input of the
+    reference's fieldMapXYZ, BFieldMapUtils.cpp:109-186): n[a] points per
+    axis from point_min[a] to point_max[a], values[(ix*ny + iy)*nz + iz]."""
+
+    def __init__(self, n, point_min, point_max, values: np.ndarray):
+        self.n = tuple(int(v) for v in n)
+        self.point_min = tuple(float(v) for v in point_min)
+        self.point_max = tuple(float(v) for v in point_max)
+        self.values = np.ascontiguousarray(values, dtype=np.float64).reshape(-1, 3)
+        assert self.values.shape[0] == self.n[0] * self.n[1] * self.n[2]
+
+    def struct(self) -> capi.Field:
+        f = capi.Field()
+        f.kind = capi.FIELD_GRID_XYZ
+        for a in range(3):
+            f.n[a] = self.n[a]
+            f.point_min[a] = self.point_min[a]
+            f.point_max[a] = self.point_max[a]
+        f.values = self.values.ctypes.data_as(C.POINTER(C.c_double))
+        return f
+
+    def points(self, axis: int) -> np.ndarray:
+        n, lo, hi = self.n[axis], self.point_min[axis], self.point_max[axis]
+        return lo + np.arange(n) * ((hi - lo) / (n - 1))
+
+    @classmethod
+    def solenoid(cls, n=(201, 201, 401), point_min=(-2000.0, -2000.0, -4000.0),
+                 point_max=(2000.0, 2000.0, 4000.0), radius=1200.0, length=6000.0,
+                 n_coils=100, b_center=2.0 * TESLA, n_threads=0) -> "FieldMap":
+        """SolenoidBField sampled on the grid points (library sampler)."""
+        lib = capi.load_library()
+        cfg = capi.Solenoid(radius=radius, length=length, n_coils=n_coils,
+                            b_mag_center=b_center)
+        values = np.zeros((n[0] * n[1] * n[2], 3), np.float64)
+        nn = (C.c_int32 * 3)(*n)
+        lo = (C.c_double * 3)(*point_min)
+        hi = (C.c_double * 3)(*point_max)
+        capi.check(lib.acts_kf_solenoid_map(C.byref(cfg), nn, lo, hi,
+                                            values.ctypes.data_as(C.POINTER(C.c_double)),
+                                            n_threads), "acts_kf_solenoid_map")
+        fm = cls(n, point_min, point_max, values)
+        fm.solenoid_config = dict(radius=radius, length=length, n_coils=n_coils,
+                                  b_center=b_center)
+        return fm
 
 
 def curvilinear_rotation(direction) -> np.ndarray:

@@ -17,7 +17,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import capi
-from .detector import (ConstantField, Detector, MUON_MASS, NS, UM,
+from .detector import (ConstantField, Detector, FieldMap, MUON_MASS, NS, UM,
                        barrel_detector, barrel_endcap_detector)
 
 
@@ -162,15 +162,26 @@ WORKLOADS = {
     # 100,000 muons, barrel + 2x7 endcap discs, |eta| < 2.5
     "endcap": dict(detector=barrel_endcap_detector, n_tracks=100000, eta=(-2.5, 2.5),
                    pt=(1.0, 10.0)),
+    # 500,000 muons, pT log-uniform 0.5-100 GeV, in a 201x201x401 field map
+    # (x, y in +-2 m, z in +-4 m) sampled from a 2 T solenoid of radius 1.2 m
+    # and length 6 m (100 coils)
+    "solenoid": dict(detector=barrel_endcap_detector, n_tracks=500000, eta=(-2.5, 2.5),
+                     pt=(0.5, 100.0), pt_log=True, field="solenoid"),
 }
+
+FIELD_MAP_POINTS = (201, 201, 401)
 
 
 def workload(name: str, n_tracks: int | None = None, seed: int = 42,
-             n_threads: int = 0):
+             n_threads: int = 0, field_map_points=None):
     """(detector, field, events) for a named benchmark workload."""
     w = WORKLOADS[name]
     det = w["detector"]()
-    field = ConstantField()
+    if w.get("field") == "solenoid":
+        field = FieldMap.solenoid(n=field_map_points or FIELD_MAP_POINTS, n_threads=n_threads)
+    else:
+        field = ConstantField()
     n = w["n_tracks"] if n_tracks is None else n_tracks
-    cfg = gen_config(n, seed, pt=w["pt"], eta=w["eta"], n_threads=n_threads)
+    cfg = gen_config(n, seed, pt=w["pt"], pt_log=w.get("pt_log", False), eta=w["eta"],
+                     n_threads=n_threads)
     return det, field, generate(det, field, cfg)

@@ -72,6 +72,28 @@ void acts_kf_gen_fetch(const acts_kf_gen* g, double* start_params, double* start
 
 void acts_kf_gen_free(acts_kf_gen* g);
 
+/* ---- field maps -----------------------------------------------------------
+ * SolenoidBField::Config (Core/include/Acts/MagneticField/SolenoidBField.hpp):
+ * a finite solenoid of n_coils current loops, scaled so that the field at
+ * the origin is b_mag_center.                                               */
+typedef struct acts_kf_solenoid {
+  double radius;        /* mm */
+  double length;        /* mm */
+  int32_t n_coils;
+  int32_t reserved;
+  double b_mag_center;  /* field at the centre, GeV/(e mm) */
+} acts_kf_solenoid;
+
+/* Sample SolenoidBField::getField (Core/src/MagneticField/SolenoidBField.cpp:
+ * 24-162, including its use of k^2 as the elliptic modulus) on the points of
+ * a regular grid, point i of axis a at point_min[a] + i * (point_max[a] -
+ * point_min[a]) / (n[a] - 1), into values[((ix*n[1]) + iy)*n[2] + iz][3]
+ * (the layout of acts_kf_field).  The fieldMapXYZ (BFieldMapUtils.cpp:
+ * 109-186) step of the reference's map construction.                        */
+int acts_kf_solenoid_map(const acts_kf_solenoid* solenoid, const int32_t n[3],
+                         const double point_min[3], const double point_max[3],
+                         double* values, int32_t n_threads);
+
 #ifdef __cplusplus
 }
 #endif

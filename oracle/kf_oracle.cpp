@@ -445,13 +445,157 @@ int freeToBound(const Vec8& f, const Surface& s, Vec6& out) {
 // ===========================================================================
 // Field
 // ===========================================================================
+// fieldMapXYZ (BFieldMapUtils.cpp:109-186): axis a has nb[a] bins of width
+// (max - min) / nb on [min, max) with max one step past the last point
+// (Helpers.hpp:289-310, Axis.hpp:45); bin b (1-based) has lower edge
+// min + (b - 1) * width and holds the value of point b - 1.
+void FieldGrid::set(const int n[3], const double pmin[3], const double pmax[3],
+                    const double* values) {
+  for (int a = 0; a < 3; ++a) {
+    nb[a] = n[a];
+    min[a] = pmin[a];
+    const double step = (pmax[a] - pmin[a]) / static_cast<double>(n[a] - 1);
+    max[a] = pmax[a] + step;
+    width[a] = (max[a] - min[a]) / static_cast<double>(n[a]);
+  }
+  v.assign(values, values + 3 * static_cast<size_t>(n[0]) * n[1] * n[2]);
+}
+
+// InterpolatedBFieldMap::getField (InterpolatedBFieldMap.hpp:335-366):
+// inside iff lowerLeft <= x < lower edge of the last bin (:223-229, 298-310);
+// the cell of the point's bin; Acts::interpolate (Interpolation.hpp:96-120)
+// collapses the last axis first with (1 - f) * a + f * b.
+int FieldGrid::get(const V3& pos, V3& out) const {
+  int lo[3];
+  double ll[3], ur[3];
+  for (int a = 0; a < 3; ++a) {
+    const double upper = min[a] + static_cast<double>(nb[a] - 1) * width[a];
+    // (a NaN coordinate, undefined in the reference, counts as outside)
+    if (!(pos[a] >= min[a]) || pos[a] >= upper) return ACTS_KF_STATUS_FIELD_OUT_OF_BOUNDS;
+    // Axis::getBin (Axis.hpp:232-235), 1-based, here inside [1, nb - 1]
+    const int bin = static_cast<int>(std::floor((pos[a] - min[a]) / width[a]) + 1);
+    lo[a] = bin - 1;  // grid point of the lower corner
+    ll[a] = min[a] + static_cast<double>(bin - 1) * width[a];
+    ur[a] = min[a] + static_cast<double>(bin) * width[a];
+  }
+  // corners in the canonical order: bit 2 = x, bit 1 = y, bit 0 = z
+  double c[8][3];
+  for (int i = 0; i < 8; ++i) {
+    const int ix = lo[0] + ((i >> 2) & 1), iy = lo[1] + ((i >> 1) & 1), iz = lo[2] + (i & 1);
+    // rounding can put a point just below the last grid point into the last
+    // bin, whose upper corner is the overflow bin: zero (setExteriorBins)
+    if (ix >= nb[0] || iy >= nb[1] || iz >= nb[2]) {
+      c[i][0] = c[i][1] = c[i][2] = 0.0;
+      continue;
+    }
+    const double* p = &v[3 * ((static_cast<size_t>(ix) * nb[1] + iy) * nb[2] + iz)];
+    c[i][0] = p[0];
+    c[i][1] = p[1];
+    c[i][2] = p[2];
+  }
+  int n = 8;
+  for (int a = 2; a >= 0; --a) {
+    const double f = (pos[a] - ll[a]) / (ur[a] - ll[a]);
+    for (int i = 0; i < n / 2; ++i)
+      for (int k = 0; k < 3; ++k) c[i][k] = (1 - f) * c[2 * i][k] + f * c[2 * i + 1][k];
+    n /= 2;
+  }
+  out = v3(c[0][0], c[0][1], c[0][2]);
+  return 0;
+}
+
 int Field::get(const V3& pos, V3& out) const {
-  (void)pos;
   if (kind == ACTS_KF_FIELD_CONSTANT) {
     out = v3(B[0], B[1], B[2]);
     return 0;
   }
+  if (kind == ACTS_KF_FIELD_GRID_XYZ) return grid.get(pos, out);
   return ACTS_KF_STATUS_FIELD_OUT_OF_BOUNDS;
+}
+
+// ===========================================================================
+// SolenoidBField (SolenoidBField.cpp:24-162), used to sample field maps.
+// The reference passes k^2 to Boost's ellint_1/ellint_2, which take the
+// modulus: the integrals evaluated are K and E of modulus k^2.
+// ===========================================================================
+// complete elliptic integrals K(k), E(k) of modulus k by the arithmetic-
+// geometric mean (Abramowitz & Stegun 17.6)
+void ellipticKE(double k, double& K, double& E) {
+  double a = 1.0, b = std::sqrt(1.0 - k * k), c = k;
+  double sum = 0.5 * c * c, pow2 = 0.5;
+  // stop once c has reached the rounding level of a: further terms are
+  // below 1e-30, and a - b only oscillates in the last bit from there on
+  for (int i = 0; i < 64 && std::fabs(c) > 1e-15 * a; ++i) {
+    const double an = 0.5 * (a + b);
+    const double bn = std::sqrt(a * b);
+    c = 0.5 * (a - b);
+    a = an;
+    b = bn;
+    pow2 *= 2.0;
+    sum += pow2 * c * c;
+  }
+  K = kPi / (2.0 * a);
+  E = K * (1.0 - sum);
+}
+
+Solenoid::Solenoid(double radius_, double length_, int nCoils_, double bMagCenter)
+    : radius(radius_), length(length_), nCoils(nCoils_) {
+  dz = length / nCoils;
+  R2 = radius * radius;
+  scale = 1.0;
+  double br, bz;
+  multiCoil(0.0, 0.0, 1.0, br, bz);
+  scale = bMagCenter / std::sqrt(br * br + bz * bz);
+}
+
+double Solenoid::k2(double r, double z) const {
+  return 4 * radius * r / ((radius + r) * (radius + r) + z * z);
+}
+
+double Solenoid::Br(double rs, double z, double sc) const {
+  const double r = std::abs(rs);
+  if (r == 0) return 0.;
+  const double k_2 = k2(r, z);
+  const double k = std::sqrt(k_2);
+  const double constant = sc * k * z / (4 * kPi * std::sqrt(radius * r * r * r));
+  double K, E;
+  ellipticKE(k_2, K, E);
+  const double B = (2. - k_2) / (2. - 2. * k_2) * E - K;
+  return r / rs * constant * B;
+}
+
+double Solenoid::Bz(double rs, double z, double sc) const {
+  const double r = std::abs(rs);
+  if (r == 0) return sc / 2. * R2 / (std::sqrt(R2 + z * z) * (R2 + z * z));
+  const double k_2 = k2(r, z);
+  const double k = std::sqrt(k_2);
+  const double constant = sc * k / (4 * kPi * std::sqrt(radius * r));
+  double K, E;
+  ellipticKE(k_2, K, E);
+  const double B = ((radius + r) * k_2 - 2. * r) / (2. * r * (1. - k_2)) * E + K;
+  return constant * B;
+}
+
+void Solenoid::multiCoil(double r, double z, double sc, double& br, double& bz) const {
+  br = 0;
+  bz = 0;
+  for (int coil = 0; coil < nCoils; ++coil) {
+    const double zs = z + length * 0.5 - dz * (coil + 0.5);
+    br += Br(r, zs, sc);
+    bz += Bz(r, zs, sc);
+  }
+}
+
+V3 Solenoid::field(const V3& pos) const {
+  const double r = perp(pos);
+  double br, bz;
+  multiCoil(r, pos[2], scale, br, bz);
+  V3 out = v3(0, 0, bz);
+  if (r != 0.) {
+    const V3 rDir = normalized(v3(pos[0], pos[1], 0));
+    out = v3(out[0] + rDir[0] * br, out[1] + rDir[1] * br, out[2] + rDir[2] * br);
+  }
+  return out;
 }
 
 // ===========================================================================

@@ -289,6 +289,23 @@ struct FieldGrid {
   int nb[3];        // number of bins per axis (= number of points)
   double min[3], max[3], width[3];
   std::vector<double> v;  // values at grid points [(i*nb1 + j)*nb2 + k][3]
+  void set(const int n[3], const double pmin[3], const double pmax[3], const double* values);
+  int get(const V3& pos, V3& out) const;
+};
+
+void ellipticKE(double k, double& K, double& E);
+
+// SolenoidBField with its Config (radius, length, nCoils, bMagCenter)
+struct Solenoid {
+  double radius, length;
+  int nCoils;
+  double dz, R2, scale;
+  Solenoid(double radius, double length, int nCoils, double bMagCenter);
+  double k2(double r, double z) const;
+  double Br(double r, double z, double scale) const;
+  double Bz(double r, double z, double scale) const;
+  void multiCoil(double r, double z, double scale, double& br, double& bz) const;
+  V3 field(const V3& pos) const;
 };
 
 struct Field {

@@ -63,8 +63,48 @@ def load():
     lib.acts_oracle_propagate.argtypes = [C.POINTER(capi.Field), C.POINTER(capi.Options),
                                           dp, dp, dp, C.POINTER(capi.Surface), C.c_double,
                                           dp, dp, dp, dp, C.POINTER(C.c_int32)]
+    lib.acts_oracle_field_eval.restype = C.c_int
+    lib.acts_oracle_field_eval.argtypes = [C.POINTER(capi.Field), C.c_int64, dp, dp,
+                                           C.POINTER(C.c_int32)]
+    lib.acts_oracle_solenoid_eval.restype = C.c_int
+    lib.acts_oracle_solenoid_eval.argtypes = [C.c_double, C.c_double, C.c_int32, C.c_double,
+                                              C.c_int64, dp, dp]
+    lib.acts_oracle_elliptic.restype = None
+    lib.acts_oracle_elliptic.argtypes = [C.c_double, dp, dp]
     _lib = lib
     return lib
+
+
+def field_eval(field, positions):
+    """(B [n,3], status [n]) of the oracle's field lookup at `positions`."""
+    lib = load()
+    pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 3)
+    out = np.zeros_like(pos)
+    status = np.zeros(len(pos), np.int32)
+    fld = field.struct() if hasattr(field, "struct") else field
+    rc = lib.acts_oracle_field_eval(C.byref(fld), len(pos), _dp(pos), _dp(out),
+                                    status.ctypes.data_as(C.POINTER(C.c_int32)))
+    if rc:
+        raise RuntimeError(f"acts_oracle_field_eval failed ({rc})")
+    return out, status
+
+
+def solenoid_eval(radius, length, n_coils, b_center, positions):
+    lib = load()
+    pos = np.ascontiguousarray(positions, np.float64).reshape(-1, 3)
+    out = np.zeros_like(pos)
+    rc = lib.acts_oracle_solenoid_eval(radius, length, n_coils, b_center, len(pos),
+                                       _dp(pos), _dp(out))
+    if rc:
+        raise RuntimeError(f"acts_oracle_solenoid_eval failed ({rc})")
+    return out
+
+
+def elliptic(k):
+    lib = load()
+    K, E = C.c_double(), C.c_double()
+    lib.acts_oracle_elliptic(k, C.byref(K), C.byref(E))
+    return K.value, E.value
 
 
 def _dp(a: np.ndarray):

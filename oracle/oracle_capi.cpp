@@ -38,6 +38,16 @@ void buildGeometry(Geometry& geo, const acts_kf_surface* surfaces, int n,
   }
   geo.field.kind = field->kind;
   for (int i = 0; i < 3; ++i) geo.field.B[i] = field->constant[i];
+  if (field->kind == ACTS_KF_FIELD_GRID_XYZ)
+    geo.field.grid.set(field->n, field->point_min, field->point_max, field->values);
+}
+
+bool fieldSupported(const acts_kf_field* f) {
+  if (f->kind == ACTS_KF_FIELD_CONSTANT) return true;
+  if (f->kind != ACTS_KF_FIELD_GRID_XYZ || !f->values) return false;
+  for (int a = 0; a < 3; ++a)
+    if (f->n[a] < 2 || !(f->point_max[a] > f->point_min[a])) return false;
+  return true;
 }
 
 void loadTrack(const acts_kf_batch_in* in, int64_t t, TrackIn& tr) {
@@ -104,7 +114,7 @@ int acts_oracle_fit(const acts_kf_surface* surfaces, int32_t n_surfaces,
                     const acts_kf_batch_in* in, const acts_kf_batch_out* out,
                     int32_t n_threads, double* seconds) {
   if (!surfaces || !field || !options || !in || !out) return ACTS_KF_ERR_INVALID_ARGUMENT;
-  if (field->kind != ACTS_KF_FIELD_CONSTANT) return ACTS_KF_ERR_UNSUPPORTED;
+  if (!fieldSupported(field)) return ACTS_KF_ERR_UNSUPPORTED;
   Geometry geo;
   buildGeometry(geo, surfaces, n_surfaces, materials, n_materials, field, options);
   if (n_threads <= 0) n_threads = static_cast<int32_t>(std::thread::hardware_concurrency());
@@ -282,5 +292,39 @@ int acts_oracle_propagate(const acts_kf_field* field, const acts_kf_options* opt
   if (n_steps) *n_steps = ns;
   return 0;
 }
+
+// Field lookups: the oracle's InterpolatedBFieldMap / ConstantBField.
+int acts_oracle_field_eval(const acts_kf_field* field, int64_t n, const double* pos,
+                           double* out, int32_t* status) {
+  if (!field || !pos || !out || !status || !fieldSupported(field))
+    return ACTS_KF_ERR_INVALID_ARGUMENT;
+  Field f;
+  f.kind = field->kind;
+  for (int i = 0; i < 3; ++i) f.B[i] = field->constant[i];
+  if (field->kind == ACTS_KF_FIELD_GRID_XYZ)
+    f.grid.set(field->n, field->point_min, field->point_max, field->values);
+  for (int64_t i = 0; i < n; ++i) {
+    V3 B = v3(0, 0, 0);
+    status[i] = f.get(v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]), B);
+    for (int k = 0; k < 3; ++k) out[3 * i + k] = B[k];
+  }
+  return 0;
+}
+
+// SolenoidBField::getField at n positions.
+int acts_oracle_solenoid_eval(double radius, double length, int32_t nCoils, double bCenter,
+                              int64_t n, const double* pos, double* out) {
+  if (!pos || !out || nCoils <= 0) return ACTS_KF_ERR_INVALID_ARGUMENT;
+  const Solenoid s(radius, length, nCoils, bCenter);
+  for (int64_t i = 0; i < n; ++i) {
+    const V3 B = s.field(v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]));
+    for (int k = 0; k < 3; ++k) out[3 * i + k] = B[k];
+  }
+  return 0;
+}
+
+// Complete elliptic integrals K, E of modulus k (what SolenoidBField calls
+// ellint_1/ellint_2 with).
+void acts_oracle_elliptic(double k, double* K, double* E) { ellipticKE(k, *K, *E); }
 
 }  // extern "C"

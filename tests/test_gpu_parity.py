@@ -49,3 +49,14 @@ def test_endcap_tracks(oracle_mod):
     rep = compare(got, ref, ev, REL)
     _report(rep)
     assert rep["ok"], rep
+
+
+def test_solenoid_field_map(oracle_mod):
+    """Barrel + discs in a 201x201x401 map of a 2 T solenoid (trilinear
+    interpolation), pT 0.5-100 GeV; includes tracks that leave the map."""
+    det, field, ev = A.workload("solenoid", n_tracks=4000, seed=13)
+    got, ref = _fit_both(oracle_mod, det, field, ev)
+    rep = compare(got, ref, ev, REL)
+    _report(rep)
+    assert rep["n_ok"] >= 3900
+    assert rep["ok"], rep
