@@ -47,6 +47,7 @@ constexpr int64_t kMaxBatch = (int64_t{1} << 31) - (int64_t{1} << 24);
 // least `waitFrac`/64 of the busy lanes wait at a surface, or `maxInner`
 // steps have been taken; then the waiting lanes run the surface update
 // together, and finished lanes write their summary and go idle.
+template <bool Mixed>
 __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKernel(
     FwdArgs a, int32_t* __restrict__ next, int32_t maxInner, int32_t waitFrac) {
   const int lane = static_cast<int>(threadIdx.x) & 63;
@@ -64,7 +65,7 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
       base = __shfl(base, leader);
       if (phase == kPhaseIdle) {
         const int64_t t = static_cast<int64_t>(base) + __popcll(idle & lanesBelow);
-        phase = (t < n) ? laneInit(L, t, a) : kPhaseRetired;
+        phase = (t < n) ? laneInit<Mixed>(L, t, a) : kPhaseRetired;
       }
     }
     if (phase == kPhaseFinished) {
@@ -74,12 +75,12 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
     const uint64_t busy = __ballot(phase != kPhaseRetired);
     if (!busy) break;
     for (int it = 0; it < maxInner; ++it) {
-      if (phase == kPhaseStepping) phase = laneStep(L, a);
+      if (phase == kPhaseStepping) phase = laneStep<Mixed>(L, a);
       const int nStep = __popcll(__ballot(phase == kPhaseStepping));
       const int nWait = __popcll(__ballot(phase == kPhaseSurface || phase == kPhaseFinished));
       if (nStep == 0 || 64 * nWait >= waitFrac * (nStep + nWait)) break;
     }
-    if (phase == kPhaseSurface) phase = laneFilter(L, a);
+    if (phase == kPhaseSurface) phase = laneFilter<Mixed>(L, a);
     if (phase == kPhaseFinished) {
       laneFinish(L, a);
       phase = kPhaseIdle;
@@ -181,6 +182,8 @@ struct acts_kf_ctx {
   DevSlab* slabs = nullptr;
   DevField field{};
   double* fieldValues = nullptr;
+  float* fieldValuesF = nullptr;
+  bool mixed = false;
   DevOpts opts{};
   PhysConst pc{};
   DevBuf scratchD, scratchI, scratchT, lenBuf, permBuf, histBuf, queueBuf;
@@ -269,7 +272,8 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   } else if (field->kind != ACTS_KF_FIELD_CONSTANT) {
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: field kind not supported");
   }
-  if (options->precision != ACTS_KF_PRECISION_FP64)
+  if (options->precision != ACTS_KF_PRECISION_FP64 &&
+      options->precision != ACTS_KF_PRECISION_MIXED)
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: precision mode not supported");
   for (int32_t i = 0; i < nSurfaces; ++i) {
     const acts_kf_surface& s = surfaces[i];
@@ -336,16 +340,33 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
     return cleanup(ACTS_KF_ERR_DEVICE, "hipMemcpy geometry failed");
   c->field.kind = field->kind;
   for (int i = 0; i < 3; ++i) c->field.B[i] = field->constant[i];
+  c->mixed = options->precision == ACTS_KF_PRECISION_MIXED;
   if (field->kind == ACTS_KF_FIELD_GRID_XYZ) {
     std::vector<double> padded;
     if (!buildGridField(*field, c->field, padded))
       return cleanup(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: bad field grid");
-    if (hipMalloc(&c->fieldValues, sizeof(double) * padded.size()) != hipSuccess)
-      return cleanup(ACTS_KF_ERR_OUT_OF_MEMORY, "hipMalloc field map failed");
-    if (hipMemcpy(c->fieldValues, padded.data(), sizeof(double) * padded.size(),
-                  hipMemcpyHostToDevice) != hipSuccess)
-      return cleanup(ACTS_KF_ERR_DEVICE, "hipMemcpy field map failed");
-    c->field.v = c->fieldValues;
+    if (!c->mixed) {
+      if (hipMalloc(&c->fieldValues, sizeof(double) * padded.size()) != hipSuccess)
+        return cleanup(ACTS_KF_ERR_OUT_OF_MEMORY, "hipMalloc field map failed");
+      if (hipMemcpy(c->fieldValues, padded.data(), sizeof(double) * padded.size(),
+                    hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(ACTS_KF_ERR_DEVICE, "hipMemcpy field map failed");
+      c->field.v = c->fieldValues;
+    } else {
+      // the mixed-precision fit reads an FP32 copy of the map
+      std::vector<float> pf(padded.begin(), padded.end());
+      for (int a = 0; a < 3; ++a) {
+        c->field.minF[a] = static_cast<float>(c->field.min[a]);
+        c->field.widthF[a] = static_cast<float>(c->field.width[a]);
+        c->field.upperF[a] = static_cast<float>(c->field.upper[a]);
+      }
+      if (hipMalloc(&c->fieldValuesF, sizeof(float) * pf.size()) != hipSuccess)
+        return cleanup(ACTS_KF_ERR_OUT_OF_MEMORY, "hipMalloc field map failed");
+      if (hipMemcpy(c->fieldValuesF, pf.data(), sizeof(float) * pf.size(),
+                    hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(ACTS_KF_ERR_DEVICE, "hipMemcpy field map failed");
+      c->field.vf = c->fieldValuesF;
+    }
   }
   DevOpts& o = c->opts;
   o.stepTolerance = options->step_tolerance;
@@ -373,8 +394,12 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   if (options->sched_wait_64ths > 0) c->waitFrac = std::min(options->sched_wait_64ths, 64);
   {
     int blocksPerCu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocksPerCu, fitForwardKernel, kBlock,
-                                                     0) != hipSuccess ||
+    const hipError_t oe =
+        c->mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocksPerCu,
+                                                                fitForwardKernel<true>, kBlock, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocksPerCu,
+                                                                fitForwardKernel<false>, kBlock, 0);
+    if (oe != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) !=
             hipSuccess ||
         blocksPerCu <= 0 || cus <= 0) {
@@ -469,8 +494,12 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   int32_t* queue = static_cast<int32_t*>(c->queueBuf.p);
   hipLaunchKernelGGL(zeroKernel, dim3(1), dim3(64), 0, stream, queue, 1);
   const int64_t waves = std::min(blocks, c->residentWaves);
-  hipLaunchKernelGGL(fitForwardKernel, dim3(static_cast<unsigned>(waves)), dim3(kBlock), 0,
-                     stream, fa, queue, c->innerSteps, c->waitFrac);
+  if (c->mixed)
+    hipLaunchKernelGGL(fitForwardKernel<true>, dim3(static_cast<unsigned>(waves)), dim3(kBlock),
+                       0, stream, fa, queue, c->innerSteps, c->waitFrac);
+  else
+    hipLaunchKernelGGL(fitForwardKernel<false>, dim3(static_cast<unsigned>(waves)), dim3(kBlock),
+                       0, stream, fa, queue, c->innerSteps, c->waitFrac);
   KF_HIP(hipGetLastError());
   hipLaunchKernelGGL(smoothKernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream,
                      b, d, sc, perm);
@@ -638,6 +667,7 @@ void acts_kf_destroy(acts_kf_ctx* c) {
   if (c->geo) (void)hipFree(c->geo);
   if (c->slabs) (void)hipFree(c->slabs);
   if (c->fieldValues) (void)hipFree(c->fieldValues);
+  if (c->fieldValuesF) (void)hipFree(c->fieldValuesF);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

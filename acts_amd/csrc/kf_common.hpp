@@ -235,6 +235,10 @@ struct DevField {
   // values of points 0..nb per axis, the extra point nb (the overflow bin)
   // zero: [(ix * (nb1 + 1) + iy) * (nb2 + 1) + iz][3]
   const double* __restrict__ v;
+  // the same map in float for the mixed-precision fit
+  float minF[3], widthF[3], upperF[3];
+  int32_t pad3;
+  const float* __restrict__ vf;
 };
 
 }  // namespace kf

@@ -1,7 +1,7 @@
 // kf_fit.hpp -- per-track forward filter + RTS smoother on the device.
 #pragma once
 
-#include "kf_device.hpp"
+#include "kf_mixed.hpp"
 
 namespace kf {
 
@@ -500,6 +500,7 @@ __device__ inline int getNextTarget(Nav& nv, Track& T, const DevSurface* geo,
 }
 
 // LoopProtection.hpp:19-67 (forward)
+template <bool Mixed = false>
 __device__ inline void loopProtection(const Track& T, const DevField& F,
                                       const DevOpts& o, double& limit,
                                       bool release) {
@@ -507,7 +508,13 @@ __device__ inline void loopProtection(const Track& T, const DevField& F,
   if (release) limit = DBL_MAX;
   // a failed lookup leaves the loop protection off (LoopProtection.hpp)
   V3 B;
-  if (fieldAt(F, T.pos, B)) return;
+  if constexpr (Mixed) {
+    F3 b;
+    if (fieldAtF(F, tof(T.pos), b)) return;
+    B = mk(b.x, b.y, b.z);
+  } else {
+    if (fieldAt(F, T.pos, B)) return;
+  }
   const double Bn = sqrt(dot(B, B));
   if (Bn == 0) return;
   const double p = copysign(static_cast<double>(o.absQ), T.qop) / T.qop;

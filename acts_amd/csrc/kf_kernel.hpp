@@ -73,6 +73,7 @@ struct FwdArgs {
 // EigenStepper::initialize (EigenStepper.ipp:37-70), DirectNavigator
 // initialisation and the pre-propagation act of the KalmanFitter actor.
 // Returns the lane's next phase.
+template <bool Mixed>
 __device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a) {
   const DevBatch& in = a.in;
   const DevOpts& o = a.o;
@@ -144,11 +145,11 @@ __device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a) {
   L.nv.tIdx = 0;
 
   L.propLimit = o.pathLimit;
-  loopProtection(T, a.F, o, L.propLimit, false);
+  loopProtection<Mixed>(T, a.F, o, L.propLimit, false);
   L.kfLimit = DBL_MAX;
   // pre-propagation act (the start surface carries neither material nor a
   // measurement, so only the finalisation checks run)
-  loopProtection(T, a.F, o, L.kfLimit, true);
+  loopProtection<Mixed>(T, a.F, o, L.kfLimit, true);
   L.finished = (L.nMeasStates == L.nUnique) || pathAbort(T, L.kfLimit, o.surfaceTolerance);
   if (L.finished || pathAbort(T, L.propLimit, o.surfaceTolerance)) return kPhaseFinished;
   L.status = getNextTarget(L.nv, T, a.geo, o, L.seq);
@@ -157,6 +158,7 @@ __device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a) {
 
 // One iteration of the propagation loop (Propagator.ipp:25-210): step, target
 // status, the actor's finalisation checks, aborters, next target.
+template <bool Mixed>
 __device__ inline int laneStep(Lane& L, const FwdArgs& a) {
   const DevOpts& o = a.o;
   Track& T = L.T;
@@ -165,7 +167,7 @@ __device__ inline int laneStep(Lane& L, const FwdArgs& a) {
     return kPhaseFinished;
   }
   ++L.iter;
-  const int sst = rkStep(T, a.F, o);
+  const int sst = Mixed ? rkStepMixed(T, a.F, o) : rkStep(T, a.F, o);
   if (sst) {
     L.status = sst;
     return kPhaseFinished;
@@ -180,7 +182,7 @@ __device__ inline int laneStep(Lane& L, const FwdArgs& a) {
   if (L.reached >= 0) return kPhaseSurface;
   // act without a current surface: only the finalisation checks
   if (!L.finished) {
-    if (L.kfLimit == DBL_MAX) loopProtection(T, a.F, o, L.kfLimit, true);
+    if (L.kfLimit == DBL_MAX) loopProtection<Mixed>(T, a.F, o, L.kfLimit, true);
     L.finished = (L.nMeasStates == L.nUnique) || pathAbort(T, L.kfLimit, o.surfaceTolerance);
   }
   if (L.finished || pathAbort(T, L.propLimit, o.surfaceTolerance)) return kPhaseFinished;
@@ -194,6 +196,7 @@ __device__ inline int laneStep(Lane& L, const FwdArgs& a) {
 
 // KalmanFitter::Actor::filter on the reached surface (KalmanFitter.hpp:
 // 339-671), then the rest of the propagation-loop iteration.
+template <bool Mixed>
 __device__ inline int laneFilter(Lane& L, const FwdArgs& a) {
   const DevBatch& in = a.in;
   const DevOpts& o = a.o;
@@ -203,7 +206,7 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a) {
   const int64_t t = L.t;
   L.reached = -1;
   if (!L.finished) {
-    if (L.kfLimit == DBL_MAX) loopProtection(T, a.F, o, L.kfLimit, true);
+    if (L.kfLimit == DBL_MAX) loopProtection<Mixed>(T, a.F, o, L.kfLimit, true);
     const DevSurface& s = a.geo[reached];
     int mi = -1;
     for (int i = 0; i < L.nMeas; ++i)

@@ -1,0 +1,241 @@
+// kf_mixed.hpp -- the FP32 stepping of the mixed-precision fit.
+//
+// Same algorithm as rkStep (EigenStepper::step, EigenStepper.ipp:169-388,
+// and the transport matrix of EigenStepperDefaultExtension.hpp:137-228), with
+// the Runge-Kutta stages, the error estimate, the field lookups (FP32 copy
+// of the field map, FP32 trilinear interpolation) and the transport-matrix
+// update computed in float.  The track state (position, direction, q/p,
+// time, path length, jacobian blocks) stays in double: each step's float
+// increments are added to it, so rounding does not accumulate at float
+// precision over a track.  Everything per surface (covariance transport,
+// gain-matrix update, material, smoother) stays FP64.
+#pragma once
+
+#include "kf_device.hpp"
+
+namespace kf {
+
+struct F3 {
+  float x, y, z;
+};
+__device__ inline F3 mkf(float a, float b, float c) { return F3{a, b, c}; }
+__device__ inline F3 tof(const V3& v) {
+  return F3{static_cast<float>(v.x), static_cast<float>(v.y), static_cast<float>(v.z)};
+}
+__device__ inline F3 crossf(const F3& a, const F3& b) {
+  return F3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// Trilinear lookup in the FP32 map (same cell, corner order and collapse
+// order as gridField).
+__device__ inline int gridFieldF(const DevField& F, const F3& p, F3& out) {
+  const float x[3] = {p.x, p.y, p.z};
+  int lo[3];
+  float f[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float mn = F.minF[a], w = F.widthF[a];
+    if (!(x[a] >= mn) || x[a] >= F.upperF[a]) return ACTS_KF_STATUS_FIELD_OUT_OF_BOUNDS;
+    int bin = static_cast<int>(floorf((x[a] - mn) / w) + 1.f);
+    bin = bin < 1 ? 1 : (bin > F.nb[a] ? F.nb[a] : bin);
+    lo[a] = bin - 1;
+    const float ll = mn + static_cast<float>(bin - 1) * w;
+    f[a] = (x[a] - ll) / w;
+  }
+  const int64_t sy = F.nb[2] + 1;
+  const int64_t sx = static_cast<int64_t>(F.nb[1] + 1) * (F.nb[2] + 1);
+  const float* base = F.vf + 3 * (lo[0] * sx + lo[1] * sy + lo[2]);
+  float c[4][3];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float* q = base + 3 * (((i >> 1) & 1) * sx + (i & 1) * sy);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) c[i][k] = (1.f - f[2]) * q[k] + f[2] * q[3 + k];
+  }
+  float d[2][3];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) d[i][k] = (1.f - f[1]) * c[2 * i][k] + f[1] * c[2 * i + 1][k];
+  out = mkf((1.f - f[0]) * d[0][0] + f[0] * d[1][0], (1.f - f[0]) * d[0][1] + f[0] * d[1][1],
+            (1.f - f[0]) * d[0][2] + f[0] * d[1][2]);
+  return 0;
+}
+
+__device__ inline int fieldAtF(const DevField& F, const F3& p, F3& out) {
+  if (F.kind == ACTS_KF_FIELD_CONSTANT) {
+    out = mkf(static_cast<float>(F.B[0]), static_cast<float>(F.B[1]),
+              static_cast<float>(F.B[2]));
+    return 0;
+  }
+  return gridFieldF(F, p, out);
+}
+
+__device__ inline float stepScalingF(float tol, float err) {
+  float x = tol / err;
+  x = sqrtf(sqrtf(x));
+  return fminf(fmaxf(x, 0.25f), 4.0f);
+}
+
+__device__ inline int rkStepMixed(Track& T, const DevField& F, const DevOpts& o) {
+  const F3 pos = tof(T.pos);
+  const F3 dir = tof(T.dir);
+  const float qop = static_cast<float>(T.qop);
+  const float tol = static_cast<float>(o.stepTolerance);
+  F3 B1;
+  if (const int fs = fieldAtF(F, pos, B1)) return fs;
+  const F3 c1 = crossf(dir, B1);
+  const F3 k1 = mkf(qop * c1.x, qop * c1.y, qop * c1.z);
+
+  const double initialH = stepValue(T);
+  float h = static_cast<float>(initialH);
+  float h2, half_h, err;
+  F3 B2, B3, k2, k3, k4;
+  int nTrials = 0;
+  while (true) {
+    ++nTrials;
+    h2 = h * h;
+    half_h = h * 0.5f;
+    const float h28 = h2 * 0.125f;
+    const F3 p1 = mkf((pos.x + half_h * dir.x) + h28 * k1.x, (pos.y + half_h * dir.y) + h28 * k1.y,
+                      (pos.z + half_h * dir.z) + h28 * k1.z);
+    if (const int fs = fieldAtF(F, p1, B2)) return fs;
+    F3 a = crossf(mkf(dir.x + half_h * k1.x, dir.y + half_h * k1.y, dir.z + half_h * k1.z), B2);
+    k2 = mkf(qop * a.x, qop * a.y, qop * a.z);
+    a = crossf(mkf(dir.x + half_h * k2.x, dir.y + half_h * k2.y, dir.z + half_h * k2.z), B2);
+    k3 = mkf(qop * a.x, qop * a.y, qop * a.z);
+    const float h22 = h2 * 0.5f;
+    const F3 p2 = mkf((pos.x + h * dir.x) + h22 * k3.x, (pos.y + h * dir.y) + h22 * k3.y,
+                      (pos.z + h * dir.z) + h22 * k3.z);
+    if (const int fs = fieldAtF(F, p2, B3)) return fs;
+    a = crossf(mkf(dir.x + h * k3.x, dir.y + h * k3.y, dir.z + h * k3.z), B3);
+    k4 = mkf(qop * a.x, qop * a.y, qop * a.z);
+    const float e0 = ((k1.x - k2.x) - k3.x) + k4.x;
+    const float e1 = ((k1.y - k2.y) - k3.y) + k4.y;
+    const float e2 = ((k1.z - k2.z) - k3.z) + k4.z;
+    err = h2 * ((fabsf(e0) + fabsf(e1)) + fabsf(e2));
+    err = fmaxf(1e-20f, err);
+    if (err <= 4.0f * tol) break;
+    h *= stepScalingF(tol, err);
+    if (fabs(static_cast<double>(h)) < fabs(o.stepSizeCutOff))
+      return ACTS_KF_STATUS_STEP_SIZE_STALLED;
+    if (static_cast<uint32_t>(nTrials) > o.maxTrials) return ACTS_KF_STATUS_STEP_ADJUSTMENT_FAILED;
+  }
+
+  const float m = o.mass;
+  const float p = copysignf(o.absQ, qop) / qop;
+  const float mp = m / p;
+  const float dtds = sqrtf(1.f + mp * mp);
+  T.time += static_cast<double>(h * dtds);
+  T.der[3] = dtds;
+
+  // transport matrix D (only the non-trivial blocks)
+  const float h6 = h / 6.f;
+  const float hh6 = (h * h) / 6.f;
+  const float qh2 = qop * half_h;
+  const float qh = qop * h;
+  const F3 dk1dL = crossf(dir, B1);
+  F3 dk2dL, dk3dL, dk4dL;
+  {
+    const F3 a = crossf(mkf(dir.x + half_h * k1.x, dir.y + half_h * k1.y, dir.z + half_h * k1.z), B2);
+    const F3 b = crossf(dk1dL, B2);
+    dk2dL = mkf(a.x + qh2 * b.x, a.y + qh2 * b.y, a.z + qh2 * b.z);
+  }
+  {
+    const F3 a = crossf(mkf(dir.x + half_h * k2.x, dir.y + half_h * k2.y, dir.z + half_h * k2.z), B2);
+    const F3 b = crossf(dk2dL, B2);
+    dk3dL = mkf(a.x + qh2 * b.x, a.y + qh2 * b.y, a.z + qh2 * b.z);
+  }
+  {
+    const F3 a = crossf(mkf(dir.x + h * k3.x, dir.y + h * k3.y, dir.z + h * k3.z), B3);
+    const F3 b = crossf(dk3dL, B3);
+    dk4dL = mkf(a.x + qh * b.x, a.y + qh * b.y, a.z + qh * b.z);
+  }
+  float dk1dT[9], dk2dT[9], dk3dT[9], dk4dT[9];
+  dk1dT[0] = 0.f;          dk1dT[1] = qop * B1.z;  dk1dT[2] = qop * -B1.y;
+  dk1dT[3] = qop * -B1.z;  dk1dT[4] = 0.f;         dk1dT[5] = qop * B1.x;
+  dk1dT[6] = qop * B1.y;   dk1dT[7] = qop * -B1.x; dk1dT[8] = 0.f;
+  auto stage = [&](const float* prev, float hh, const F3& B, float* out) {
+    float m3[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) m3[3 * i + j] = (i == j ? 1.f : 0.f) + hh * prev[3 * i + j];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const F3 c = crossf(mkf(m3[j], m3[3 + j], m3[6 + j]), B);
+      out[j] = qop * c.x;
+      out[3 + j] = qop * c.y;
+      out[6 + j] = qop * c.z;
+    }
+  };
+  stage(dk1dT, half_h, B2, dk2dT);
+  stage(dk2dT, half_h, B2, dk3dT);
+  stage(dk3dT, h, B3, dk4dT);
+
+  float dF[12], dG[12];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int e = 3 * i + j;
+      const float id = (i == j ? 1.f : 0.f);
+      dF[4 * i + j] = (id + h6 * ((dk1dT[e] + dk2dT[e]) + dk3dT[e])) * h;
+      dG[4 * i + j] = id + h6 * ((dk1dT[e] + 2.f * (dk2dT[e] + dk3dT[e])) + dk4dT[e]);
+    }
+  }
+  dF[3] = hh6 * ((dk1dL.x + dk2dL.x) + dk3dL.x);
+  dF[7] = hh6 * ((dk1dL.y + dk2dL.y) + dk3dL.y);
+  dF[11] = hh6 * ((dk1dL.z + dk2dL.z) + dk3dL.z);
+  dG[3] = h6 * ((dk1dL.x + 2.f * (dk2dL.x + dk3dL.x)) + dk4dL.x);
+  dG[7] = h6 * ((dk1dL.y + 2.f * (dk2dL.y + dk3dL.y)) + dk4dL.y);
+  dG[11] = h6 * ((dk1dL.z + 2.f * (dk2dL.z + dk3dL.z)) + dk4dL.z);
+  const float D37 = h * m * m / (p * p * qop * dtds);
+
+  float br[12];
+#pragma unroll
+  for (int i = 0; i < 12; ++i) br[i] = static_cast<float>(T.br[i]);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float pt = (dF[4 * i + 0] * br[j] + dF[4 * i + 1] * br[4 + j]) + dF[4 * i + 2] * br[8 + j];
+      float pb = (dG[4 * i + 0] * br[j] + dG[4 * i + 1] * br[4 + j]) + dG[4 * i + 2] * br[8 + j];
+      if (j == 3) {
+        pt = pt + dF[4 * i + 3];
+        pb = pb + dG[4 * i + 3];
+      }
+      T.tr[4 * i + j] += static_cast<double>(pt);
+      T.br[4 * i + j] = static_cast<double>(pb);
+    }
+  }
+  T.tr37 += static_cast<double>(D37);
+
+  const float h26 = h2 / 6.f;
+  T.pos.x += static_cast<double>(h * dir.x + h26 * ((k1.x + k2.x) + k3.x));
+  T.pos.y += static_cast<double>(h * dir.y + h26 * ((k1.y + k2.y) + k3.y));
+  T.pos.z += static_cast<double>(h * dir.z + h26 * ((k1.z + k2.z) + k3.z));
+  const V3 nd = mk(T.dir.x + static_cast<double>(h6 * ((k1.x + 2.f * (k2.x + k3.x)) + k4.x)),
+                   T.dir.y + static_cast<double>(h6 * ((k1.y + 2.f * (k2.y + k3.y)) + k4.y)),
+                   T.dir.z + static_cast<double>(h6 * ((k1.z + 2.f * (k2.z + k3.z)) + k4.z)));
+  // renormalise with a float reciprocal norm: the correction is O(h^2 k^2)
+  const float inv = rsqrtf(static_cast<float>(dot(nd, nd)));
+  T.dir = mk(nd.x * inv, nd.y * inv, nd.z * inv);
+  T.der[0] = T.dir.x;
+  T.der[1] = T.dir.y;
+  T.der[2] = T.dir.z;
+  T.der[4] = k4.x;
+  T.der[5] = k4.y;
+  T.der[6] = k4.z;
+  T.pathAcc += static_cast<double>(h);
+  ++T.nSteps;
+  T.nTrials += nTrials;
+
+  const double nextAccuracy = fabs(static_cast<double>(h * stepScalingF(tol, err)));
+  const double previousAccuracy = fabs(T.cAcc);
+  const double initialStepLength = fabs(initialH);
+  if (nextAccuracy < initialStepLength || nextAccuracy > previousAccuracy) T.cAcc = nextAccuracy;
+  return 0;
+}
+
+}  // namespace kf

@@ -96,3 +96,56 @@ def compare(got, ref, events, rel: float = 1e-10, rel_smoothed_cov: float = 1e-9
                  and all(rep[f"{k}_mismatch"] == 0 for k in
                          ("chi2", "ndf", "n_measurements", "n_holes", "n_outliers")))
     return rep
+
+
+def error_report(got, ref, events) -> dict:
+    """Errors of a reduced-precision fit against the FP64 reference fit.
+
+    Over the tracks that both fit with the same state sequence: max and RMS
+    of the relative parameter error (floors as in `compare`) of the smoothed
+    parameters, and of the smoothed covariance (relative to sqrt(C_ii C_jj)).
+    """
+    same = (got.status == ref.status) & (ref.status == 0) & (got.n_states == ref.n_states)
+    so = events.seq_offsets
+    idx = np.concatenate([np.arange(so[t], so[t] + ref.n_states[t])
+                          for t in np.nonzero(same)[0]]) if same.any() else np.zeros(0, int)
+    if len(idx) and got.state_surface is not None:
+        seq_ok = np.ones(len(idx), bool)
+        seq_ok &= got.state_surface[idx] == ref.state_surface[idx]
+        idx = idx[seq_ok]
+    rep = {"n_tracks": int(events.n_tracks),
+           "status_mismatch": int(np.count_nonzero(got.status != ref.status)),
+           "n_compared_states": int(len(idx))}
+    if len(idx):
+        pe = param_rel_err(got.smoothed[idx], ref.smoothed[idx])
+        rep["smoothed_max_rel"] = pe.max(axis=0).tolist()
+        rep["smoothed_rms_rel"] = np.sqrt((pe ** 2).mean(axis=0)).tolist()
+        if got.smoothed_cov is not None:
+            ce = cov_rel_err(got.smoothed_cov[idx], ref.smoothed_cov[idx])
+            rep["smoothed_cov_max_rel"] = float(ce.max())
+            rep["smoothed_cov_rms_rel"] = float(np.sqrt((ce ** 2).mean()))
+    return rep
+
+
+def pulls(res, events) -> dict:
+    """Pull mean and width of the smoothed parameters at each track's first
+    measurement state, against the generator's truth there."""
+    so, mo = events.seq_offsets, events.meas_offsets
+    vals = []
+    for t in np.nonzero(res.status == 0)[0]:
+        for k in range(res.n_states[t]):
+            s = so[t] + k
+            if res.state_flags[s] & 1:  # HAS_MEASUREMENT
+                surf = res.state_surface[s]
+                ms = np.nonzero(events.meas_surfaces[mo[t]:mo[t + 1]] == surf)[0]
+                if not len(ms):
+                    break
+                truth = events.truth_meas[mo[t] + ms[0]]
+                cov = res.smoothed_cov[s].reshape(6, 6)
+                d = res.smoothed[s] - truth
+                d[2] = (d[2] + np.pi) % (2 * np.pi) - np.pi
+                sig = np.sqrt(np.diag(cov))
+                vals.append((d / np.where(sig > 0, sig, 1.0))[:5])
+                break
+    v = np.array(vals)
+    return {"n": int(len(v)), "mean": v.mean(axis=0).tolist(), "std": v.std(axis=0).tolist()}

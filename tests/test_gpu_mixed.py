@@ -1,0 +1,39 @@
+"""Mixed precision (FP32 stepping and FP32 field map, FP64 Kalman algebra)
+against the FP64 oracle on the same inputs.
+
+Stated accuracy (checked here): the same fit status for >= 99.5 % of the
+tracks; over tracks with the same state sequence, smoothed parameters within
+1e-4 relative (floors 1 mm, 1 mm, 1 rad, 1 rad, 0.01/GeV, 1 mm) and
+smoothed covariances within 1e-2 of sqrt(C_ii C_jj); pull means within 0.02
+and pull widths within 2 % of the FP64 fit's.
+"""
+import numpy as np
+import pytest
+
+import acts_amd as A
+from acts_amd import capi
+from acts_amd.fitter import ALL_STATE_OUTPUTS
+
+from parity import error_report, pulls
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mixed_precision_solenoid(oracle_mod):
+    det, field, ev = A.workload("solenoid", n_tracks=6000, seed=21)
+    fitter = A.KalmanFitter(det, field, A.KalmanFitterOptions(precision=capi.PRECISION_MIXED))
+    try:
+        got = fitter.fit(ev, outputs=ALL_STATE_OUTPUTS)
+    finally:
+        fitter.close()
+    ref, _ = oracle_mod.fit(det, field, A.KalmanFitterOptions(), ev, n_threads=0)
+    rep = error_report(got, ref, ev)
+    pg, pr = pulls(got, ev), pulls(ref, ev)
+    print(rep, flush=True)
+    print("pulls mixed", pg, flush=True)
+    print("pulls fp64 ", pr, flush=True)
+    assert rep["status_mismatch"] <= 0.005 * ev.n_tracks
+    assert max(rep["smoothed_max_rel"]) < 1e-4
+    assert rep["smoothed_cov_max_rel"] < 1e-2
+    assert np.all(np.abs(np.array(pg["mean"]) - pr["mean"]) < 0.02)
+    assert np.all(np.abs(np.array(pg["std"]) / pr["std"] - 1) < 0.02)
