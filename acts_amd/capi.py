@@ -110,6 +110,7 @@ class Options(C.Structure):
         ("length_sorted", C.c_int32),
         ("sched_inner_steps", C.c_int32),
         ("sched_wait_64ths", C.c_int32),
+        ("kernel_timing", C.c_int32),
     ]
 
 
@@ -192,6 +193,8 @@ SIGNATURES = {
     "acts_kf_fit_batch_device": (C.c_int, [C.c_void_p, C.POINTER(BatchIn),
                                            C.POINTER(BatchOut), C.c_void_p]),
     "acts_kf_sync": (C.c_int, [C.c_void_p]),
+    "acts_kf_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]),
     "acts_kf_destroy": (None, [C.c_void_p]),
     "acts_kf_status_string": (C.c_char_p, [C.c_int32]),
     "acts_kf_last_error": (C.c_char_p, []),

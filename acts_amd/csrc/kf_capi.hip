@@ -42,14 +42,17 @@ constexpr int64_t kMaxBatch = (int64_t{1} << 31) - (int64_t{1} << 24);
 #endif
 
 // Forward filter, persistent: each wavefront keeps its 64 lanes busy with
-// tracks taken from a global queue (`next`).  Per outer iteration: idle lanes
+// tracks taken from a global queue (`next`), in the order `order` when given
+// (longest surface sequence first, so that long tracks do not start last).
+// Per outer iteration: idle lanes
 // take new tracks; stepping lanes step until no lane steps any more, or at
 // least `waitFrac`/64 of the busy lanes wait at a surface, or `maxInner`
 // steps have been taken; then the waiting lanes run the surface update
 // together, and finished lanes write their summary and go idle.
 template <bool Mixed>
 __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKernel(
-    FwdArgs a, int32_t* __restrict__ next, int32_t maxInner, int32_t waitFrac) {
+    FwdArgs a, int32_t* __restrict__ next, int32_t maxInner, int32_t waitFrac,
+    const int32_t* __restrict__ order) {
   const int lane = static_cast<int>(threadIdx.x) & 63;
   const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t n = a.in.n;
@@ -64,8 +67,8 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
       if (lane == leader) base = atomicAdd(next, cnt);
       base = __shfl(base, leader);
       if (phase == kPhaseIdle) {
-        const int64_t t = static_cast<int64_t>(base) + __popcll(idle & lanesBelow);
-        phase = (t < n) ? laneInit<Mixed>(L, t, a) : kPhaseRetired;
+        const int64_t q = static_cast<int64_t>(base) + __popcll(idle & lanesBelow);
+        phase = (q < n) ? laneInit<Mixed>(L, order ? order[q] : q, a) : kPhaseRetired;
       }
     }
     if (phase == kPhaseFinished) {
@@ -184,6 +187,11 @@ struct acts_kf_ctx {
   double* fieldValues = nullptr;
   float* fieldValuesF = nullptr;
   bool mixed = false;
+  // kernel timing: events before the forward kernel, between the two
+  // kernels and after the smoother
+  bool timing = false;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  bool timed = false;
   DevOpts opts{};
   PhysConst pc{};
   DevBuf scratchD, scratchI, scratchT, lenBuf, permBuf, histBuf, queueBuf;
@@ -390,6 +398,13 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   o.absQ = options->abs_charge;
   c->pc = makePhysConst();
   c->lengthSorted = options->length_sorted != 0;
+  c->timing = options->kernel_timing != 0;
+  if (c->timing)
+    for (auto& e : c->ev)
+      if (hipEventCreate(&e) != hipSuccess) {
+        acts_kf_destroy(c);
+        return fail(ACTS_KF_ERR_DEVICE, "hipEventCreate failed");
+      }
   if (options->sched_inner_steps > 0) c->innerSteps = options->sched_inner_steps;
   if (options->sched_wait_64ths > 0) c->waitFrac = std::min(options->sched_wait_64ths, 64);
   {
@@ -472,7 +487,8 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
       return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "permutation allocation failed");
     perm = static_cast<int32_t*>(c->permBuf.p);
     int32_t* hist = static_cast<int32_t*>(c->histBuf.p);
-    KF_HIP(hipMemsetAsync(hist, 0, sizeof(int32_t) * (states + 1), stream));
+    hipLaunchKernelGGL(zeroKernel, dim3(static_cast<unsigned>((states + 1 + 255) / 256)),
+                       dim3(256), 0, stream, hist, states + 1);
     const unsigned nb = static_cast<unsigned>((n + 255) / 256);
     hipLaunchKernelGGL(lengthHistogramKernel, dim3(nb), dim3(256), 0, stream, in->seq_offsets,
                        n, states, hist);
@@ -494,16 +510,22 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   int32_t* queue = static_cast<int32_t*>(c->queueBuf.p);
   hipLaunchKernelGGL(zeroKernel, dim3(1), dim3(64), 0, stream, queue, 1);
   const int64_t waves = std::min(blocks, c->residentWaves);
+  if (c->timing) KF_HIP(hipEventRecord(c->ev[0], stream));
   if (c->mixed)
     hipLaunchKernelGGL(fitForwardKernel<true>, dim3(static_cast<unsigned>(waves)), dim3(kBlock),
-                       0, stream, fa, queue, c->innerSteps, c->waitFrac);
+                       0, stream, fa, queue, c->innerSteps, c->waitFrac, perm);
   else
     hipLaunchKernelGGL(fitForwardKernel<false>, dim3(static_cast<unsigned>(waves)), dim3(kBlock),
-                       0, stream, fa, queue, c->innerSteps, c->waitFrac);
+                       0, stream, fa, queue, c->innerSteps, c->waitFrac, perm);
   KF_HIP(hipGetLastError());
+  if (c->timing) KF_HIP(hipEventRecord(c->ev[1], stream));
   hipLaunchKernelGGL(smoothKernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream,
                      b, d, sc, perm);
   KF_HIP(hipGetLastError());
+  if (c->timing) {
+    KF_HIP(hipEventRecord(c->ev[2], stream));
+    c->timed = true;
+  }
   return ACTS_KF_OK;
 }
 
@@ -644,6 +666,20 @@ int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
   return ACTS_KF_OK;
 }
 
+int acts_kf_kernel_times(acts_kf_ctx* c, double* forwardMs, double* smootherMs) {
+  if (!c || !forwardMs || !smootherMs) return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "null argument");
+  if (!c->timing || !c->timed)
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "no timed fit (set options.kernel_timing)");
+  KF_HIP(hipSetDevice(c->device));
+  KF_HIP(hipEventSynchronize(c->ev[2]));
+  float a = 0, b = 0;
+  KF_HIP(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+  KF_HIP(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+  *forwardMs = a;
+  *smootherMs = b;
+  return ACTS_KF_OK;
+}
+
 int acts_kf_sync(acts_kf_ctx* c) {
   if (!c) return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "null context");
   KF_HIP(hipSetDevice(c->device));
@@ -668,6 +704,8 @@ void acts_kf_destroy(acts_kf_ctx* c) {
   if (c->slabs) (void)hipFree(c->slabs);
   if (c->fieldValues) (void)hipFree(c->fieldValues);
   if (c->fieldValuesF) (void)hipFree(c->fieldValuesF);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -46,6 +46,7 @@ class KalmanFitterOptions:
     length_sorted: bool = True
     sched_inner_steps: int = 0              # 0 = library default
     sched_wait_64ths: int = 0               # 0 = library default
+    kernel_timing: bool = False             # events around the kernels
 
     def struct(self) -> capi.Options:
         o = capi.Options()
@@ -139,6 +140,13 @@ class KalmanFitter:
 
     def sync(self) -> None:
         capi.check(self.lib.acts_kf_sync(self.ctx), "acts_kf_sync")
+
+    def kernel_times(self) -> tuple[float, float]:
+        """(forward, smoother) kernel milliseconds of the last fit."""
+        f, s = C.c_double(), C.c_double()
+        capi.check(self.lib.acts_kf_kernel_times(self.ctx, C.byref(f), C.byref(s)),
+                   "acts_kf_kernel_times")
+        return f.value, s.value
 
     def close(self) -> None:
         if getattr(self, "ctx", None):

@@ -56,3 +56,13 @@ def batch_flops(result) -> float:
     """Total algorithmic FLOP of one fit of the batch held in `result`."""
     return float(np.sum(flops_per_track(result.n_steps, result.n_step_trials,
                                         result.n_states, result.n_measurements)))
+
+
+def forward_flops(result) -> float:
+    """Algorithmic FLOP of the forward-filter kernel alone (no smoother)."""
+    t = np.asarray(result.n_step_trials, np.float64)
+    s = np.asarray(result.n_steps, np.float64)
+    n = np.asarray(result.n_states, np.float64)
+    m = np.asarray(result.n_measurements, np.float64)
+    return float(np.sum(t * F_TRIAL + s * F_STEP + n * (F_SURF + F_MAT + F_NAV)
+                        + m * (F_UPD + F_UPDATE_STATE)))

@@ -1,13 +1,19 @@
 #!/usr/bin/env python3
 """Throughput of the batched Kalman track fit on one MI355X.
 
-One step = one fit (forward filter + RTS smoother) of the whole synthetic
-batch, inputs already resident in HBM.  Prints one JSON line.
+One step = one fit (forward filter + RTS smoother + track summary) of the
+whole synthetic batch, inputs already resident in HBM.  Prints one JSON line.
 
-Workload (default ``endcap``): 100,000 muons, 10 silicon barrel layers
-(r = 32..1000 mm, half-length 3 m) plus 2x7 endcap discs (|z| = 600..2900 mm),
-0.3 mm Si per surface, uniform Bz = 2 T, pT 1-10 GeV, |eta| < 2.5,
-15/50 um measurements, plain FP64 fit.
+Workloads:
+  endcap (default): 100,000 muons, 10 silicon barrel cylinders (r = 32..1000
+      mm, half-length 3 m) plus 2x7 endcap discs (|z| = 600..2900 mm), 0.3 mm
+      Si per surface, uniform Bz = 2 T, pT 1-10 GeV, |eta| < 2.5, 15/50 um
+      measurements.
+  solenoid: 500,000 muons, pT log-uniform 0.5-100 GeV, same detector in a
+      201x201x401 field map of a 2 T solenoid.
+`value` is the plain FP64 fit (the drop-in); the mixed-precision fit (FP32
+stepping and field) of the same batch is timed beside it, with its error
+against the FP64 results.
 """
 from __future__ import annotations
 
@@ -25,6 +31,8 @@ sys.path.insert(0, REPO)
 METRIC = ("tracks/s fitted, mixed vs FP64 at stated pull accuracy; "
           "p50/p99 event latency")
 FP64_VECTOR_PEAK_TFLOPS = 78.6  # MI355X published FP64 vector peak
+OUTPUTS = ("state_surface", "state_flags", "state_path", "state_chi2", "smoothed",
+           "smoothed_cov")
 
 
 def parse():
@@ -32,12 +40,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="endcap", choices=["endcap", "barrel"])
+    ap.add_argument("--workload", default="endcap", choices=["endcap", "barrel", "solenoid"])
     ap.add_argument("--n-tracks", type=int, default=None)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--input-order", action="store_true",
-                    help="fit tracks in input order instead of length-sorted")
+                    help="queue tracks in input order instead of longest first")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (for the "
                          "HBM-counter calibration of rocprof runs)")
@@ -45,21 +54,44 @@ def parse():
 
 
 def cpu_baseline(det, field, opts, events, threads):
-    """Time the CPU oracle (restatement of the reference fitter) on the host."""
+    """Time the CPU oracle (restatement of the reference fitter) on the host,
+    on a bounded sample of the batch."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle  # noqa: E402  (test infrastructure: the timed CPU baseline)
     oracle.load()
     n = events.n_tracks
-    # warm-up on a small slice, then all tracks at `threads` threads
     oracle.fit(det, field, opts, events.subset(np.arange(min(n, 500))), n_threads=threads)
-    _, secs = oracle.fit(det, field, opts, events, n_threads=threads)
+    ns = min(n, 100_000)
+    _, secs = oracle.fit(det, field, opts, events.subset(np.arange(ns)), n_threads=threads)
     n1 = min(n, 5000)
     _, secs1 = oracle.fit(det, field, opts, events.subset(np.arange(n1)), n_threads=1)
-    return {"value": n / secs, "unit": "tracks/s", "cores": threads, "kind": "port",
-            "sample": f"all {n} tracks of the workload at {threads} threads "
-                      f"(restated reference CPU fitter, g++ -O2)",
+    return {"value": ns / secs, "unit": "tracks/s", "cores": threads, "kind": "port",
+            "sample": f"first {ns} tracks of the workload at {threads} threads "
+                      f"(restated reference CPU fitter, g++ -O2, FP64)",
             "single_thread_value": n1 / secs1,
             "single_thread_sample": f"first {n1} tracks, 1 thread"}
+
+
+def timed_fits(fitter, bin_, bout, stream, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fitter.fit_device(bin_, bout, stream.cuda_stream)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record(stream)
+    fwd = []
+    for _ in range(steps):
+        fitter.fit_device(bin_, bout, stream.cuda_stream)
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # kernel durations of the forward filter, measured on the library's
+    # stream with HIP events (one extra timed fit per sample)
+    for _ in range(min(steps, 5)):
+        fitter.fit_device(bin_, bout, stream.cuda_stream)
+        fwd.append(fitter.kernel_times())
+    return start.elapsed_time(end) / steps, wall, fwd
 
 
 def main() -> int:
@@ -72,35 +104,29 @@ def main() -> int:
 
     import torch
     import acts_amd as A
+    from acts_amd import capi
     from acts_amd.device import DeviceEvents, DeviceResult
-    from acts_amd.flops import batch_flops
+    from acts_amd.flops import batch_flops, forward_flops
 
     det, field, events = A.workload(args.workload, n_tracks=args.n_tracks, seed=args.seed)
-    opts = A.KalmanFitterOptions(length_sorted=not args.input_order)
     dev = torch.device("cuda:0")
     dev_in = DeviceEvents(events, dev)
-    outputs = ("state_surface", "state_flags", "state_path", "state_chi2", "smoothed",
-               "smoothed_cov")
-    dev_out = DeviceResult(events.n_tracks, dev_in.n_slots, outputs, dev)
-    fitter = A.KalmanFitter(det, field, opts)
-    bin_, bout = dev_in.batch_in(), dev_out.batch_out()
-    # a dedicated (default-priority) stream: its handle is non-null, so the
-    # library launches on it and the torch events below time exactly the fits
-    stream = torch.cuda.Stream(device=dev)
-    torch.cuda.synchronize()
+    bin_ = dev_in.batch_in()
+    stream = torch.cuda.Stream(device=dev)  # default priority
+    n = events.n_tracks
 
-    for _ in range(args.warmup):
-        fitter.fit_device(bin_, bout, stream.cuda_stream)
+    def run(precision):
+        opts = A.KalmanFitterOptions(length_sorted=not args.input_order, precision=precision,
+                                     kernel_timing=True)
+        out = DeviceResult(n, dev_in.n_slots, OUTPUTS, dev)
+        fitter = A.KalmanFitter(det, field, opts)
+        ms, wall, kt = timed_fits(fitter, bin_, out.batch_out(), stream, args.steps,
+                                  args.warmup)
+        fitter.close()
+        return ms, wall, kt, out
+
+    ms64, wall64, kt64, out64 = run(capi.PRECISION_FP64)
     torch.cuda.synchronize()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    start.record(stream)
-    for _ in range(args.steps):
-        fitter.fit_device(bin_, bout, stream.cuda_stream)
-    end.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = start.elapsed_time(end) / args.steps
 
     if args.copy_calibration_mib > 0:
         nbytes = args.copy_calibration_mib << 20
@@ -109,51 +135,73 @@ def main() -> int:
         dst.copy_(src)
         torch.cuda.synchronize()
 
-    # per-track counters of the last fit -> algorithmic FLOP per launch
     class R:
         pass
     r = R()
     for name in ("status", "n_steps", "n_step_trials", "n_states", "n_measurements"):
-        setattr(r, name, dev_out.host(name))
-    flop = batch_flops(r)
-    achieved = flop / (ms_per_step * 1e-3) / 1e12
+        setattr(r, name, out64.host(name))
+    flop_fwd = forward_flops(r)
+    flop_all = batch_flops(r)
+    fwd_ms = float(np.mean([k[0] for k in kt64]))
+    smooth_ms = float(np.mean([k[1] for k in kt64]))
+    achieved = flop_fwd / (fwd_ms * 1e-3) / 1e12
     traffic = None
     tpath = os.path.join(REPO, "profiles", f"traffic_{args.workload}.json")
     if os.path.exists(tpath) and args.n_tracks is None:
         with open(tpath) as f:
-            tr = json.load(f)
-        traffic = tr.get("bytes_per_launch")
+            traffic = json.load(f).get("bytes_per_launch")
 
-    n = events.n_tracks
     line = {
         "metric": METRIC,
-        "value": n / (ms_per_step * 1e-3),
+        "value": n / (ms64 * 1e-3),
         "unit": "tracks/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": ms64,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (project generator, seed %d)" % args.seed,
-        "config": {"workload": f"{args.workload}: {n} muons, "
-                               f"{len(det.surfaces)} surfaces, Bz=2T, FP64 KF+RTS",
-                   "n_tracks": n, "precision": "fp64",
+        "config": {"workload": f"{args.workload}: {n} muons, {len(det.surfaces)} surfaces, "
+                               f"{'field map' if args.workload == 'solenoid' else 'Bz=2T'}, "
+                               "FP64 KF+RTS",
+                   "n_tracks": n,
                    "fit_failures": int(np.count_nonzero(r.status)),
                    "mean_states": float(np.mean(r.n_states)),
                    "mean_steps": float(np.mean(r.n_steps))},
-        "roofline": {"bound": "valu", "achieved": achieved,
-                     "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "roofline": {"bound": "valu", "kernel": "fitForwardKernel (FP64)",
+                     "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": traffic,
-                     "flop_per_launch": flop},
-        "wall_s_timed": wall,
+                     "flop_per_launch": flop_fwd, "kernel_ms": fwd_ms,
+                     "smoother_ms": smooth_ms, "fit_flop_per_step": flop_all},
+        "wall_s_timed": wall64,
     }
+
+    if not args.no_mixed:
+        msm, _, ktm, outm = run(capi.PRECISION_MIXED)
+        s64 = out64.host("status")
+        sm = outm.host("status")
+        same = (s64 == sm) & (s64 == 0)
+        p64, pm = out64.host("smoothed"), outm.host("smoothed")
+        so = events.seq_offsets
+        ns = out64.host("n_states")
+        idx = np.concatenate([np.arange(so[t], so[t] + ns[t]) for t in np.nonzero(same)[0]])
+        floors = np.array([1.0, 1.0, 1.0, 1.0, 0.01, 1.0])
+        err = np.abs(pm[idx] - p64[idx]) / np.maximum(np.abs(p64[idx]), floors)
+        line["mixed"] = {
+            "value": n / (msm * 1e-3), "ms_per_step": msm, "dtype": "f32 stepping/field, f64 KF",
+            "forward_kernel_ms": float(np.mean([k[0] for k in ktm])),
+            "status_mismatch_vs_fp64": int(np.count_nonzero(s64 != sm)),
+            "smoothed_rel_err_vs_fp64": {"rms": float(np.sqrt((err ** 2).mean())),
+                                         "p999": float(np.quantile(err, 0.999)),
+                                         "max": float(err.max())}}
+
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
-        line["cpu_baseline"] = cpu_baseline(det, field, opts, events, threads)
-    fitter.close()
+        line["cpu_baseline"] = cpu_baseline(det, field, A.KalmanFitterOptions(), events,
+                                            threads)
     print(json.dumps(line), flush=True)
     return 0
 

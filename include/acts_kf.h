@@ -163,6 +163,9 @@ typedef struct acts_kf_options {
   int32_t length_sorted;
   int32_t sched_inner_steps;
   int32_t sched_wait_64ths;
+  /* 1 = record HIP events around the kernels of each fit, for
+   * acts_kf_kernel_times (no reference counterpart)                         */
+  int32_t kernel_timing;
 } acts_kf_options;
 
 /* ---- one batch of tracks ------------------------------------------------- */
@@ -256,6 +259,11 @@ int acts_kf_sync(acts_kf_ctx* ctx);
 /* Release the context and its device memory (the end of life of the
  * KalmanFitter / Propagator / field objects of the reference).             */
 void acts_kf_destroy(acts_kf_ctx* ctx);
+
+/* Durations in milliseconds of the forward-filter kernel and of the
+ * smoother kernel of the context's last fit (needs options.kernel_timing;
+ * waits for that fit).  No reference counterpart: a profiling aid.          */
+int acts_kf_kernel_times(acts_kf_ctx* ctx, double* forward_ms, double* smoother_ms);
 
 /* Message for a per-track status code: the reference's error messages
  * (std::error_category::message of KalmanFitterError.cpp:21,
