@@ -128,6 +128,26 @@ def barrel_endcap_detector(radii=BARREL_RADII, half_length=3000.0,
     return det
 
 
+def double_layer_detector(radii=BARREL_RADII, half_length=3000.0, disc_z=None,
+                          disc_rmin=30.0, disc_rmax=1000.0, gap=2.0,
+                          thickness=0.3) -> Detector:
+    """Barrel of double layers (a second sensitive cylinder `gap` mm outside
+    each layer) plus the 2x7 single endcap discs: up to 27 crossings."""
+    det = Detector(name="double-layer barrel+endcaps")
+    mi = det.add_material(silicon(), thickness)
+    for r in radii:
+        for dr in (0.0, gap):
+            det.surfaces.append(SurfaceDesc(capi.SURFACE_CYLINDER, np.eye(3), np.zeros(3),
+                                            (r + dr, half_length), mi, True))
+    zs = endcap_z_positions() if disc_z is None else list(disc_z)
+    for side in (-1.0, 1.0):
+        for z in zs:
+            det.surfaces.append(SurfaceDesc(capi.SURFACE_DISC, np.eye(3),
+                                            np.array([0.0, 0.0, side * z]),
+                                            (disc_rmin, disc_rmax), mi, True))
+    return det
+
+
 @dataclass
 class ConstantField:
     bx: float = 0.0

@@ -18,7 +18,7 @@ import numpy as np
 
 from . import capi
 from .detector import (ConstantField, Detector, FieldMap, MUON_MASS, NS, UM,
-                       barrel_detector, barrel_endcap_detector)
+                       barrel_detector, barrel_endcap_detector, double_layer_detector)
 
 
 @dataclass
@@ -167,6 +167,17 @@ WORKLOADS = {
     # and length 6 m (100 coils)
     "solenoid": dict(detector=barrel_endcap_detector, n_tracks=500000, eta=(-2.5, 2.5),
                      pt=(0.5, 100.0), pt_log=True, field="solenoid"),
+    # 2,000,000 muons with 14-27 measurements: barrel layers doubled (up to
+    # 27 crossings with the discs), 5 % hole probability per crossed layer,
+    # 10 % of the tracks with 1-3 outliers at 20 sigma (rejected by a chi2
+    # cut of 25), uniform Bz = 2 T.  A track with a rejected outlier never
+    # counts all its measurements, so with the direct navigator it only ends
+    # at the loop-protection path limit (half a helix turn); in a finite field
+    # map it would end with FIELD_OUT_OF_BOUNDS instead, as in the reference.
+    "ragged": dict(detector=double_layer_detector, n_tracks=2_000_000, eta=(-2.5, 2.5),
+                   pt=(0.5, 100.0), pt_log=True, hole_probability=0.05,
+                   outlier_fraction=0.1, min_measurements=6,
+                   options=dict(outlier_chi2_cut=25.0)),
 }
 
 FIELD_MAP_POINTS = (201, 201, 401)
@@ -183,5 +194,15 @@ def workload(name: str, n_tracks: int | None = None, seed: int = 42,
         field = ConstantField()
     n = w["n_tracks"] if n_tracks is None else n_tracks
     cfg = gen_config(n, seed, pt=w["pt"], pt_log=w.get("pt_log", False), eta=w["eta"],
-                     n_threads=n_threads)
+                     hole_probability=w.get("hole_probability", 0.0),
+                     outlier_fraction=w.get("outlier_fraction", 0.0),
+                     min_measurements=w.get("min_measurements", 1), n_threads=n_threads)
     return det, field, generate(det, field, cfg)
+
+
+def workload_options(name: str, **overrides):
+    """KalmanFitterOptions of a named workload (e.g. its outlier chi2 cut)."""
+    from .fitter import KalmanFitterOptions
+    kw = dict(WORKLOADS[name].get("options", {}))
+    kw.update(overrides)
+    return KalmanFitterOptions(**kw)
