@@ -161,6 +161,17 @@ struct DevBuf {
   }
 };
 
+// What a captured fit depends on: the caller's buffers, the scratch buffers
+// and sizes, the stream.
+struct GraphKey {
+  acts_kf_batch_in in;
+  acts_kf_batch_out out;
+  int32_t states;
+  int32_t pad;
+  void* stream;
+  void* scratch[5];
+};
+
 PhysConst makePhysConst() {
   PhysConst c;
   const long double MeV = 1e-3L;
@@ -192,6 +203,10 @@ struct acts_kf_ctx {
   bool timing = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   bool timed = false;
+  // captured launch sequence of the last fit, replayed while its key holds
+  bool useGraph = false;
+  hipGraphExec_t graphExec = nullptr;
+  GraphKey graphKey{};
   DevOpts opts{};
   PhysConst pc{};
   DevBuf scratchD, scratchI, scratchT, lenBuf, permBuf, histBuf, queueBuf;
@@ -399,6 +414,7 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   c->pc = makePhysConst();
   c->lengthSorted = options->length_sorted != 0;
   c->timing = options->kernel_timing != 0;
+  c->useGraph = options->use_graph != 0;
   if (c->timing)
     for (auto& e : c->ev)
       if (hipEventCreate(&e) != hipSuccess) {
@@ -441,6 +457,34 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   if (c->scratchD.ensure(needD) || c->scratchI.ensure(needI) || c->scratchT.ensure(needT) ||
       c->queueBuf.ensure(sizeof(int32_t)))
     return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "scratch allocation failed");
+  const bool sorted = c->lengthSorted && n > kBlock;
+  if (sorted && (c->permBuf.ensure(sizeof(int32_t) * n) ||
+                 c->histBuf.ensure(sizeof(int32_t) * (states + 1))))
+    return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "permutation allocation failed");
+  // Replay the captured launch sequence when nothing it depends on changed.
+  GraphKey key;
+  if (c->useGraph) {
+    std::memset(&key, 0, sizeof(key));
+    key.in = *in;
+    key.out = *out;
+    key.states = states;
+    key.stream = stream;
+    key.scratch[0] = c->scratchD.p;
+    key.scratch[1] = c->scratchI.p;
+    key.scratch[2] = c->scratchT.p;
+    key.scratch[3] = c->permBuf.p;
+    key.scratch[4] = c->histBuf.p;
+    if (c->graphExec && std::memcmp(&key, &c->graphKey, sizeof(key)) == 0) {
+      KF_HIP(hipGraphLaunch(c->graphExec, stream));
+      return ACTS_KF_OK;
+    }
+    if (c->graphExec) {
+      (void)hipGraphExecDestroy(c->graphExec);
+      c->graphExec = nullptr;
+    }
+    KF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  }
+  const bool timing = c->timing && !c->useGraph;
   DevBatch b;
   b.n = n;
   b.startParams = in->start_params;
@@ -481,10 +525,7 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   sc.maxStates = states;
   const int64_t blocks = (n + kBlock - 1) / kBlock;
   int32_t* perm = nullptr;
-  if (c->lengthSorted && n > kBlock) {
-    if (c->permBuf.ensure(sizeof(int32_t) * n) ||
-        c->histBuf.ensure(sizeof(int32_t) * (states + 1)))
-      return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "permutation allocation failed");
+  if (sorted) {
     perm = static_cast<int32_t*>(c->permBuf.p);
     int32_t* hist = static_cast<int32_t*>(c->histBuf.p);
     hipLaunchKernelGGL(zeroKernel, dim3(static_cast<unsigned>((states + 1 + 255) / 256)),
@@ -495,7 +536,6 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
     hipLaunchKernelGGL(lengthScanKernel, dim3(1), dim3(64), 0, stream, states, hist);
     hipLaunchKernelGGL(lengthScatterKernel, dim3(nb), dim3(256), 0, stream, in->seq_offsets,
                        n, states, hist, perm);
-    KF_HIP(hipGetLastError());
   }
   FwdArgs fa;
   fa.in = b;
@@ -510,19 +550,40 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   int32_t* queue = static_cast<int32_t*>(c->queueBuf.p);
   hipLaunchKernelGGL(zeroKernel, dim3(1), dim3(64), 0, stream, queue, 1);
   const int64_t waves = std::min(blocks, c->residentWaves);
-  if (c->timing) KF_HIP(hipEventRecord(c->ev[0], stream));
+  if (timing) KF_HIP(hipEventRecord(c->ev[0], stream));
   if (c->mixed)
     hipLaunchKernelGGL(fitForwardKernel<true>, dim3(static_cast<unsigned>(waves)), dim3(kBlock),
                        0, stream, fa, queue, c->innerSteps, c->waitFrac, perm);
   else
     hipLaunchKernelGGL(fitForwardKernel<false>, dim3(static_cast<unsigned>(waves)), dim3(kBlock),
                        0, stream, fa, queue, c->innerSteps, c->waitFrac, perm);
-  KF_HIP(hipGetLastError());
-  if (c->timing) KF_HIP(hipEventRecord(c->ev[1], stream));
+  if (timing) KF_HIP(hipEventRecord(c->ev[1], stream));
   hipLaunchKernelGGL(smoothKernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream,
                      b, d, sc, perm);
-  KF_HIP(hipGetLastError());
-  if (c->timing) {
+  const hipError_t launchErr = hipGetLastError();
+  if (c->useGraph) {
+    hipGraph_t graph = nullptr;
+    const hipError_t endErr = hipStreamEndCapture(stream, &graph);
+    if (launchErr != hipSuccess || endErr != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return fail(ACTS_KF_ERR_DEVICE, std::string("graph capture failed: ") +
+                                          hipGetErrorString(launchErr != hipSuccess ? launchErr
+                                                                                    : endErr));
+    }
+    const hipError_t instErr = hipGraphInstantiate(&c->graphExec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (instErr != hipSuccess) {
+      c->graphExec = nullptr;
+      return fail(ACTS_KF_ERR_DEVICE,
+                  std::string("hipGraphInstantiate: ") + hipGetErrorString(instErr));
+    }
+    c->graphKey = key;
+    KF_HIP(hipGraphLaunch(c->graphExec, stream));
+    return ACTS_KF_OK;
+  }
+  if (launchErr != hipSuccess)
+    return fail(ACTS_KF_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(launchErr));
+  if (timing) {
     KF_HIP(hipEventRecord(c->ev[2], stream));
     c->timed = true;
   }
@@ -704,6 +765,7 @@ void acts_kf_destroy(acts_kf_ctx* c) {
   if (c->slabs) (void)hipFree(c->slabs);
   if (c->fieldValues) (void)hipFree(c->fieldValues);
   if (c->fieldValuesF) (void)hipFree(c->fieldValuesF);
+  if (c->graphExec) (void)hipGraphExecDestroy(c->graphExec);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -166,6 +166,10 @@ typedef struct acts_kf_options {
   /* 1 = record HIP events around the kernels of each fit, for
    * acts_kf_kernel_times (no reference counterpart)                         */
   int32_t kernel_timing;
+  /* 1 = capture the launch sequence of a fit in a hipGraph and replay it for
+   * later fits with the same buffers and sizes (no reference counterpart;
+   * kernel_timing is ignored then)                                           */
+  int32_t use_graph;
 } acts_kf_options;
 
 /* ---- one batch of tracks ------------------------------------------------- */
