@@ -34,9 +34,15 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 constexpr int kBlock = 64;
+// geometry kept in LDS by the forward kernel (224 B per surface)
+constexpr int kLdsSurfaces = 128;
+constexpr int kLdsSlabs = 32;
 // the track queue counts in int32 and may overshoot n by one per lane
 constexpr int64_t kMaxBatch = (int64_t{1} << 31) - (int64_t{1} << 24);
 
+#ifndef KF_GEOMETRY_IN_LDS
+#define KF_GEOMETRY_IN_LDS 1
+#endif
 #ifndef KF_FORWARD_WAVES_PER_EU
 #define KF_FORWARD_WAVES_PER_EU 1
 #endif
@@ -56,6 +62,28 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
   const int lane = static_cast<int>(threadIdx.x) & 63;
   const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t n = a.in.n;
+  // Surfaces and material slabs are read at every step: keep a copy in LDS
+  // when the geometry is small enough (it always is for the benchmark
+  // detectors), else read them from global memory.
+  const DevSurface* geo = a.geo;
+  const DevSlab* slabs = a.slabs;
+#if KF_GEOMETRY_IN_LDS
+  __shared__ DevSurface sGeo[kLdsSurfaces];
+  __shared__ DevSlab sSlabs[kLdsSlabs];
+  if (a.nSurfaces <= kLdsSurfaces && a.nSlabs <= kLdsSlabs) {
+    const double* src = reinterpret_cast<const double*>(a.geo);
+    double* dst = reinterpret_cast<double*>(sGeo);
+    const int nd = a.nSurfaces * static_cast<int>(sizeof(DevSurface) / sizeof(double));
+    for (int i = static_cast<int>(threadIdx.x); i < nd; i += blockDim.x) dst[i] = src[i];
+    const int32_t* ss = reinterpret_cast<const int32_t*>(a.slabs);
+    int32_t* sd_ = reinterpret_cast<int32_t*>(sSlabs);
+    const int ns = a.nSlabs * static_cast<int>(sizeof(DevSlab) / sizeof(int32_t));
+    for (int i = static_cast<int>(threadIdx.x); i < ns; i += blockDim.x) sd_[i] = ss[i];
+    __syncthreads();
+    geo = sGeo;
+    slabs = sSlabs;
+  }
+#endif
   Lane L;
   int phase = kPhaseIdle;
   while (true) {
@@ -68,7 +96,7 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
       base = __shfl(base, leader);
       if (phase == kPhaseIdle) {
         const int64_t q = static_cast<int64_t>(base) + __popcll(idle & lanesBelow);
-        phase = (q < n) ? laneInit<Mixed>(L, order ? order[q] : q, a) : kPhaseRetired;
+        phase = (q < n) ? laneInit<Mixed>(L, order ? order[q] : q, a, geo) : kPhaseRetired;
       }
     }
     if (phase == kPhaseFinished) {
@@ -78,12 +106,12 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
     const uint64_t busy = __ballot(phase != kPhaseRetired);
     if (!busy) break;
     for (int it = 0; it < maxInner; ++it) {
-      if (phase == kPhaseStepping) phase = laneStep<Mixed>(L, a);
+      if (phase == kPhaseStepping) phase = laneStep<Mixed>(L, a, geo);
       const int nStep = __popcll(__ballot(phase == kPhaseStepping));
       const int nWait = __popcll(__ballot(phase == kPhaseSurface || phase == kPhaseFinished));
       if (nStep == 0 || 64 * nWait >= waitFrac * (nStep + nWait)) break;
     }
-    if (phase == kPhaseSurface) phase = laneFilter<Mixed>(L, a);
+    if (phase == kPhaseSurface) phase = laneFilter<Mixed>(L, a, geo, slabs);
     if (phase == kPhaseFinished) {
       laneFinish(L, a);
       phase = kPhaseIdle;
@@ -192,6 +220,7 @@ struct acts_kf_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int32_t nSurfaces = 0;
+  int32_t nSlabs = 0;
   DevSurface* geo = nullptr;
   DevSlab* slabs = nullptr;
   DevField field{};
@@ -356,6 +385,7 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   if (hipMalloc(&c->geo, sizeof(DevSurface) * nSurfaces) != hipSuccess ||
       hipMalloc(&c->slabs, sizeof(DevSlab) * hm.size()) != hipSuccess)
     return cleanup(ACTS_KF_ERR_OUT_OF_MEMORY, "hipMalloc geometry failed");
+  c->nSlabs = static_cast<int32_t>(hm.size());
   if (hipMemcpy(c->geo, hs.data(), sizeof(DevSurface) * nSurfaces, hipMemcpyHostToDevice) !=
           hipSuccess ||
       hipMemcpy(c->slabs, hm.data(), sizeof(DevSlab) * hm.size(), hipMemcpyHostToDevice) !=
@@ -547,6 +577,7 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   fa.o = c->opts;
   fa.pc = c->pc;
   fa.nSurfaces = c->nSurfaces;
+  fa.nSlabs = c->nSlabs;
   int32_t* queue = static_cast<int32_t*>(c->queueBuf.p);
   hipLaunchKernelGGL(zeroKernel, dim3(1), dim3(64), 0, stream, queue, 1);
   const int64_t waves = std::min(blocks, c->residentWaves);

@@ -68,13 +68,15 @@ struct FwdArgs {
   DevOpts o;
   PhysConst pc;
   int32_t nSurfaces;
+  int32_t nSlabs;
 };
 
 // EigenStepper::initialize (EigenStepper.ipp:37-70), DirectNavigator
 // initialisation and the pre-propagation act of the KalmanFitter actor.
 // Returns the lane's next phase.
 template <bool Mixed>
-__device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a) {
+__device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a,
+                               const DevSurface* geo) {
   const DevBatch& in = a.in;
   const DevOpts& o = a.o;
   L.t = t;
@@ -152,14 +154,14 @@ __device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a) {
   loopProtection<Mixed>(T, a.F, o, L.kfLimit, true);
   L.finished = (L.nMeasStates == L.nUnique) || pathAbort(T, L.kfLimit, o.surfaceTolerance);
   if (L.finished || pathAbort(T, L.propLimit, o.surfaceTolerance)) return kPhaseFinished;
-  L.status = getNextTarget(L.nv, T, a.geo, o, L.seq);
+  L.status = getNextTarget(L.nv, T, geo, o, L.seq);
   return L.status ? kPhaseFinished : kPhaseStepping;
 }
 
 // One iteration of the propagation loop (Propagator.ipp:25-210): step, target
 // status, the actor's finalisation checks, aborters, next target.
 template <bool Mixed>
-__device__ inline int laneStep(Lane& L, const FwdArgs& a) {
+__device__ inline int laneStep(Lane& L, const FwdArgs& a, const DevSurface* geo) {
   const DevOpts& o = a.o;
   Track& T = L.T;
   if (L.iter >= o.maxSteps) {
@@ -175,7 +177,7 @@ __device__ inline int laneStep(Lane& L, const FwdArgs& a) {
   T.cNav = kNotSet;
   T.cActor = kNotSet;
   if (L.nv.tSurf >= 0) {
-    const int ps = updateSurfaceStatus(T, a.geo[L.nv.tSurf], L.nv.tIdx, o.surfaceTolerance);
+    const int ps = updateSurfaceStatus(T, geo[L.nv.tSurf], L.nv.tIdx, o.surfaceTolerance);
     if (ps == kOnSurface && !L.nv.navBreak) L.reached = L.nv.tSurf;
     if (ps != kReachable) L.nv.tSurf = -1;
   }
@@ -188,7 +190,7 @@ __device__ inline int laneStep(Lane& L, const FwdArgs& a) {
   if (L.finished || pathAbort(T, L.propLimit, o.surfaceTolerance)) return kPhaseFinished;
   if (L.nv.tSurf < 0) {
     T.cNav = kNotSet;
-    L.status = getNextTarget(L.nv, T, a.geo, o, L.seq);
+    L.status = getNextTarget(L.nv, T, geo, o, L.seq);
     if (L.status) return kPhaseFinished;
   }
   return kPhaseStepping;
@@ -197,7 +199,8 @@ __device__ inline int laneStep(Lane& L, const FwdArgs& a) {
 // KalmanFitter::Actor::filter on the reached surface (KalmanFitter.hpp:
 // 339-671), then the rest of the propagation-loop iteration.
 template <bool Mixed>
-__device__ inline int laneFilter(Lane& L, const FwdArgs& a) {
+__device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* geo,
+                                 const DevSlab* slabs) {
   const DevBatch& in = a.in;
   const DevOpts& o = a.o;
   const DevScratch& sc = a.sc;
@@ -207,7 +210,7 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a) {
   L.reached = -1;
   if (!L.finished) {
     if (L.kfLimit == DBL_MAX) loopProtection<Mixed>(T, a.F, o, L.kfLimit, true);
-    const DevSurface& s = a.geo[reached];
+    const DevSurface& s = geo[reached];
     int mi = -1;
     for (int i = 0; i < L.nMeas; ++i)
       if (mi < 0 && in.measSurf[L.m0 + i] == reached) mi = i;
@@ -297,7 +300,7 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a) {
           si(sc, k, kIFlags, t) = static_cast<int32_t>(flags);
           L.nStates = k + 1;
           if (mi >= 0) L.tip = k;
-          if (hasMat) L.status = materialInteraction(T, s, a.slabs[s.mat], a.pc, o);
+          if (hasMat) L.status = materialInteraction(T, s, slabs[s.mat], a.pc, o);
         }
       }
     }
@@ -306,7 +309,7 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a) {
   }
   if (L.finished || pathAbort(T, L.propLimit, o.surfaceTolerance)) return kPhaseFinished;
   T.cNav = kNotSet;
-  L.status = getNextTarget(L.nv, T, a.geo, o, L.seq);
+  L.status = getNextTarget(L.nv, T, geo, o, L.seq);
   return L.status ? kPhaseFinished : kPhaseStepping;
 }
 
