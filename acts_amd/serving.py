@@ -7,8 +7,8 @@ Here events arrive as a Poisson stream; a dispatcher groups the events that
 have arrived into batches, admitted by an HBM budget per batch, and hands
 each batch to a free slot.  A slot is one fitter context (own stream,
 staging buffers and scratch, geometry and field resident); its worker thread
-copies the batch in, fits it, copies the results (smoothed parameters and
-covariances of every state, track summaries) back, and stamps every event
+copies the batch in, fits it, copies the results (surface, flags and
+smoothed parameters of every state, track summaries) back, and stamps every event
 of the batch with its completion time.  Latency of an event = completion -
 arrival.
 """
@@ -22,9 +22,12 @@ from dataclasses import dataclass
 import numpy as np
 
 from .events import Events
-from .fitter import FitResult, KalmanFitter, KalmanFitterOptions
+from .fitter import KalmanFitter, KalmanFitterOptions
 
-SERVING_OUTPUTS = ("state_surface", "state_flags", "state_chi2", "smoothed", "smoothed_cov")
+# per-state results returned to the host for every event (the smoothed
+# covariances stay on the device: they would triple the device-to-host
+# traffic)
+SERVING_OUTPUTS = ("state_surface", "state_flags", "smoothed")
 
 
 @dataclass
@@ -85,7 +88,7 @@ def batch_bytes(n_tracks: int, max_len: int) -> float:
     """Device memory of one batch: inputs, scratch (744 B per state slot),
     serving outputs and per-track scratch."""
     per_track = 48 + 288 + 96 + 16 + 9 * 4 + 8
-    per_slot = 744 + 4 + 4 + 4 + 48 + 288 + 4 + 52
+    per_slot = 744 + 4 + 4 + 48 + 4 + 52
     return n_tracks * (per_track + max_len * per_slot)
 
 

@@ -18,6 +18,7 @@ against the FP64 results.
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -40,13 +41,18 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="endcap", choices=["endcap", "barrel", "solenoid"])
+    ap.add_argument("--workload", default="endcap",
+                    choices=["endcap", "barrel", "solenoid", "ragged"])
     ap.add_argument("--n-tracks", type=int, default=None)
     ap.add_argument("--seed", type=int, default=42)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--input-order", action="store_true",
                     help="queue tracks in input order instead of longest first")
+    ap.add_argument("--serving-seconds", type=float, default=3.0,
+                    help="length of the event-serving run (0 = skip)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the scheduling knobs on the workload before timing")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (for the "
                          "HBM-counter calibration of rocprof runs)")
@@ -70,6 +76,27 @@ def cpu_baseline(det, field, opts, events, threads):
                       f"(restated reference CPU fitter, g++ -O2, FP64)",
             "single_thread_value": n1 / secs1,
             "single_thread_sample": f"first {n1} tracks, 1 thread"}
+
+
+def serving(args, fp64_tracks_per_s):
+    """Event serving (tracks as in the field-map workload): Poisson arrivals,
+    lognormal multiplicity (median 5000, clipped 500-20000), offered at half
+    the batch throughput measured above; FP64 and mixed precision."""
+    import acts_amd as A
+    from acts_amd import capi
+    from acts_amd.serving import ServingConfig, serve
+    det, field, pool = A.workload("solenoid", n_tracks=200_000, seed=args.seed + 1)
+    mean_tracks = 5000 * np.exp(0.7 ** 2 / 2)
+    rate = 0.5 * fp64_tracks_per_s / mean_tracks
+    cfg = ServingConfig(rate_hz=rate, duration_s=args.serving_seconds, seed=args.seed)
+    out = {"config": {"rate_hz": rate, "duration_s": cfg.duration_s,
+                      "multiplicity": "lognormal median 5000 sigma 0.7 clipped 500-20000",
+                      "slots": cfg.slots, "hbm_budget_bytes_per_batch": cfg.hbm_budget_bytes,
+                      "tracks": "pT 0.5-100 GeV log-uniform, |eta|<2.5, 201x201x401 "
+                                "solenoid map", "latency": "arrival -> results on host"}}
+    for name, prec in (("fp64", capi.PRECISION_FP64), ("mixed", capi.PRECISION_MIXED)):
+        out[name] = serve(det, field, pool, cfg, A.KalmanFitterOptions(precision=prec))
+    return out
 
 
 def timed_fits(fitter, bin_, bout, stream, steps, warmup):
@@ -115,9 +142,16 @@ def main() -> int:
     stream = torch.cuda.Stream(device=dev)  # default priority
     n = events.n_tracks
 
+    wl_opts = A.workload_options(args.workload)
+    tuned = {}
+    if args.autotune:
+        from acts_amd.autotune import autotune
+        tuned, _ = autotune(det, field, events, wl_opts, cache_path="")
+    elif args.input_order:
+        tuned = {"length_sorted": False}
+
     def run(precision):
-        opts = A.KalmanFitterOptions(length_sorted=not args.input_order, precision=precision,
-                                     kernel_timing=True)
+        opts = dataclasses.replace(wl_opts, precision=precision, kernel_timing=True, **tuned)
         out = DeviceResult(n, dev_in.n_slots, OUTPUTS, dev)
         fitter = A.KalmanFitter(det, field, opts)
         ms, wall, kt = timed_fits(fitter, bin_, out.batch_out(), stream, args.steps,
@@ -198,10 +232,15 @@ def main() -> int:
                                          "p999": float(np.quantile(err, 0.999)),
                                          "max": float(err.max())}}
 
+    if tuned:
+        line["config"]["scheduling"] = tuned
+
+    if args.serving_seconds > 0:
+        line["serving"] = serving(args, line["value"])
+
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
-        line["cpu_baseline"] = cpu_baseline(det, field, A.KalmanFitterOptions(), events,
-                                            threads)
+        line["cpu_baseline"] = cpu_baseline(det, field, wl_opts, events, threads)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -33,4 +33,4 @@ def test_serving_loop_and_batch_independence():
     so_m = batch_of(pool, [(5000, 700)]).seq_offsets[-1]
     n_slots = alone.smoothed.shape[0]
     assert np.array_equal(alone.smoothed, mixed.smoothed[so_m:so_m + n_slots])
-    assert np.array_equal(alone.smoothed_cov, mixed.smoothed_cov[so_m:so_m + n_slots])
+    assert np.array_equal(alone.state_surface, mixed.state_surface[so_m:so_m + n_slots])
