@@ -1,12 +1,14 @@
 """Autotuner for the scheduling knobs of the fit kernels.
 
-The persistent forward kernel has three knobs that change only the speed of
+The persistent forward kernel has four knobs that change only the speed of
 a fit, never its results (acts_kf_options):
   sched_inner_steps  RK steps taken before the lanes waiting at a surface are
                      updated;
   sched_wait_64ths   the fraction of busy lanes that must wait at a surface
                      before the surface update runs early;
-  length_sorted      whether tracks enter the queue longest-first.
+  length_sorted      whether tracks enter the queue longest-first;
+  sched_quarantine_steps  a wavefront holding a track older than this many
+                     steps takes no new tracks until it ends.
 `autotune` times candidate settings on a device-resident sample of the
 workload by coordinate descent and keeps the fastest; results are cached in
 a JSON file keyed by the workload's shape.
@@ -24,8 +26,10 @@ SPACE = {
     "sched_inner_steps": [4, 8, 16, 32, 64],
     "sched_wait_64ths": [16, 32, 48, 64],
     "length_sorted": [True, False],
+    "sched_quarantine_steps": [0, 100, 200, 400],
 }
-DEFAULTS = {"sched_inner_steps": 16, "sched_wait_64ths": 32, "length_sorted": True}
+DEFAULTS = {"sched_inner_steps": 16, "sched_wait_64ths": 32, "length_sorted": True,
+            "sched_quarantine_steps": 0}
 
 
 def coordinate_descent(evaluate: Callable[[dict], float], space=SPACE, start=None,

@@ -58,7 +58,7 @@ constexpr int64_t kMaxBatch = (int64_t{1} << 31) - (int64_t{1} << 24);
 template <bool Mixed>
 __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKernel(
     FwdArgs a, int32_t* __restrict__ next, int32_t maxInner, int32_t waitFrac,
-    const int32_t* __restrict__ order) {
+    int32_t quarantine, const int32_t* __restrict__ order) {
   const int lane = static_cast<int>(threadIdx.x) & 63;
   const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t n = a.in.n;
@@ -87,7 +87,12 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
   Lane L;
   int phase = kPhaseIdle;
   while (true) {
-    const uint64_t idle = __ballot(phase == kPhaseIdle);
+    // a wavefront holding a track that has run longer than `quarantine`
+    // steps takes no new tracks until it ends, so that the long track is not
+    // slowed down by the surface updates of fresh ones
+    const bool hold = quarantine > 0 &&
+                      __ballot(phase == kPhaseStepping && L.iter > static_cast<uint32_t>(quarantine)) != 0;
+    const uint64_t idle = hold ? 0ull : __ballot(phase == kPhaseIdle);
     if (idle) {
       const int cnt = __popcll(idle);
       const int leader = __ffsll(static_cast<long long>(idle)) - 1;
@@ -244,6 +249,7 @@ struct acts_kf_ctx {
   int64_t residentWaves = 0;
   int32_t innerSteps = 16;
   int32_t waitFrac = 32;
+  int32_t quarantine = 0;
   // staging for the host-buffer entry point
   DevBuf inBuf, outBuf;
 };
@@ -453,6 +459,7 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
       }
   if (options->sched_inner_steps > 0) c->innerSteps = options->sched_inner_steps;
   if (options->sched_wait_64ths > 0) c->waitFrac = std::min(options->sched_wait_64ths, 64);
+  c->quarantine = std::max(options->sched_quarantine_steps, 0);
   {
     int blocksPerCu = 0, cus = 0;
     const hipError_t oe =
@@ -584,10 +591,10 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   if (timing) KF_HIP(hipEventRecord(c->ev[0], stream));
   if (c->mixed)
     hipLaunchKernelGGL(fitForwardKernel<true>, dim3(static_cast<unsigned>(waves)), dim3(kBlock),
-                       0, stream, fa, queue, c->innerSteps, c->waitFrac, perm);
+                       0, stream, fa, queue, c->innerSteps, c->waitFrac, c->quarantine, perm);
   else
     hipLaunchKernelGGL(fitForwardKernel<false>, dim3(static_cast<unsigned>(waves)), dim3(kBlock),
-                       0, stream, fa, queue, c->innerSteps, c->waitFrac, perm);
+                       0, stream, fa, queue, c->innerSteps, c->waitFrac, c->quarantine, perm);
   if (timing) KF_HIP(hipEventRecord(c->ev[1], stream));
   hipLaunchKernelGGL(smoothKernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream,
                      b, d, sc, perm);

@@ -46,6 +46,7 @@ class KalmanFitterOptions:
     length_sorted: bool = True
     sched_inner_steps: int = 0              # 0 = library default
     sched_wait_64ths: int = 0               # 0 = library default
+    sched_quarantine_steps: int = 0         # 0 = off
     kernel_timing: bool = False             # events around the kernels
     use_graph: bool = False                 # replay fits as a captured hipGraph
 

@@ -5,7 +5,8 @@ from acts_amd.autotune import DEFAULTS, SPACE, coordinate_descent, signature
 
 
 def test_coordinate_descent_finds_separable_minimum():
-    target = {"sched_inner_steps": 32, "sched_wait_64ths": 48, "length_sorted": False}
+    target = {"sched_inner_steps": 32, "sched_wait_64ths": 48, "length_sorted": False,
+              "sched_quarantine_steps": 200}
     calls = []
 
     def cost(cfg):
@@ -13,6 +14,7 @@ def test_coordinate_descent_finds_separable_minimum():
         c = abs(SPACE["sched_inner_steps"].index(cfg["sched_inner_steps"]) - 3)
         c += abs(SPACE["sched_wait_64ths"].index(cfg["sched_wait_64ths"]) - 2)
         c += 0 if cfg["length_sorted"] is False else 1
+        c += abs(SPACE["sched_quarantine_steps"].index(cfg["sched_quarantine_steps"]) - 2)
         return 1.0 + c
 
     best, seen = coordinate_descent(cost)
