@@ -140,28 +140,52 @@ __global__ void zeroKernel(int32_t* p, int32_t count) {
 }
 
 // Counting sort of the tracks by sequence length, longest first.
-__global__ void lengthHistogramKernel(const int64_t* off, int64_t n, int32_t maxLen,
-                                      int32_t* hist) {
+// Queue key of a track: its sequence length (longest tracks first), or
+// maxLen + 1 for a track with a measurement within kSeamRisk rad of the +-pi
+// seam of a cylinder.  There the measured loc0 = R phi and the prediction can
+// lie on opposite sides of the seam; the residual is not wrapped (as in the
+// reference), the update throws the track far off and it may run into the
+// step limit, so such tracks are started first.  The key only orders the
+// queue: results do not depend on it.
+constexpr double kSeamRisk = 0.05;
+struct QueueKeyArgs {
+  const int64_t* seqOff;
+  const int64_t* measOff;
+  const int32_t* measSurf;
+  const double* measVal;
+  const DevSurface* geo;
+  int32_t nSurfaces;
+  int32_t maxLen;
+};
+__device__ inline int32_t queueKey(const QueueKeyArgs& k, int64_t t) {
+  const int32_t len = min(static_cast<int32_t>(k.seqOff[t + 1] - k.seqOff[t]), k.maxLen);
+  for (int64_t m = k.measOff[t]; m < k.measOff[t + 1]; ++m) {
+    const int32_t s = k.measSurf[m];
+    if (s < 0 || s >= k.nSurfaces || k.geo[s].type != ACTS_KF_SURFACE_CYLINDER) continue;
+    const double phi = k.measVal[2 * m] / k.geo[s].b0;
+    if (fabs(phi) > kPi - kSeamRisk) return k.maxLen + 1;
+  }
+  return len;
+}
+__global__ void lengthHistogramKernel(QueueKeyArgs k, int64_t n, int32_t* hist) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int32_t len = min(static_cast<int32_t>(off[t + 1] - off[t]), maxLen);
-  atomicAdd(&hist[len], 1);
+  atomicAdd(&hist[queueKey(k, t)], 1);
 }
-__global__ void lengthScanKernel(int32_t maxLen, int32_t* hist) {
+// exclusive scan from the largest key down: bucket starts, largest key first
+__global__ void lengthScanKernel(int32_t maxKey, int32_t* hist) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   int32_t acc = 0;
-  for (int32_t len = maxLen; len >= 0; --len) {
-    const int32_t c = hist[len];
-    hist[len] = acc;
+  for (int32_t key = maxKey; key >= 0; --key) {
+    const int32_t c = hist[key];
+    hist[key] = acc;
     acc += c;
   }
 }
-__global__ void lengthScatterKernel(const int64_t* off, int64_t n, int32_t maxLen,
-                                    int32_t* cursor, int32_t* perm) {
+__global__ void lengthScatterKernel(QueueKeyArgs k, int64_t n, int32_t* cursor, int32_t* perm) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int32_t len = min(static_cast<int32_t>(off[t + 1] - off[t]), maxLen);
-  perm[atomicAdd(&cursor[len], 1)] = static_cast<int32_t>(t);
+  perm[atomicAdd(&cursor[queueKey(k, t)], 1)] = static_cast<int32_t>(t);
 }
 
 __global__ void maxLengthKernel(const int64_t* off, int64_t n, int32_t* result) {
@@ -496,7 +520,7 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
     return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "scratch allocation failed");
   const bool sorted = c->lengthSorted && n > kBlock;
   if (sorted && (c->permBuf.ensure(sizeof(int32_t) * n) ||
-                 c->histBuf.ensure(sizeof(int32_t) * (states + 1))))
+                 c->histBuf.ensure(sizeof(int32_t) * (states + 2))))
     return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "permutation allocation failed");
   // Replay the captured launch sequence when nothing it depends on changed.
   GraphKey key;
@@ -565,14 +589,20 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   if (sorted) {
     perm = static_cast<int32_t*>(c->permBuf.p);
     int32_t* hist = static_cast<int32_t*>(c->histBuf.p);
-    hipLaunchKernelGGL(zeroKernel, dim3(static_cast<unsigned>((states + 1 + 255) / 256)),
-                       dim3(256), 0, stream, hist, states + 1);
+    hipLaunchKernelGGL(zeroKernel, dim3(static_cast<unsigned>((states + 2 + 255) / 256)),
+                       dim3(256), 0, stream, hist, states + 2);
     const unsigned nb = static_cast<unsigned>((n + 255) / 256);
-    hipLaunchKernelGGL(lengthHistogramKernel, dim3(nb), dim3(256), 0, stream, in->seq_offsets,
-                       n, states, hist);
-    hipLaunchKernelGGL(lengthScanKernel, dim3(1), dim3(64), 0, stream, states, hist);
-    hipLaunchKernelGGL(lengthScatterKernel, dim3(nb), dim3(256), 0, stream, in->seq_offsets,
-                       n, states, hist, perm);
+    QueueKeyArgs qk;
+    qk.seqOff = in->seq_offsets;
+    qk.measOff = in->meas_offsets;
+    qk.measSurf = in->meas_surfaces;
+    qk.measVal = in->meas_values;
+    qk.geo = c->geo;
+    qk.nSurfaces = c->nSurfaces;
+    qk.maxLen = states;
+    hipLaunchKernelGGL(lengthHistogramKernel, dim3(nb), dim3(256), 0, stream, qk, n, hist);
+    hipLaunchKernelGGL(lengthScanKernel, dim3(1), dim3(64), 0, stream, states + 1, hist);
+    hipLaunchKernelGGL(lengthScatterKernel, dim3(nb), dim3(256), 0, stream, qk, n, hist, perm);
   }
   FwdArgs fa;
   fa.in = b;
