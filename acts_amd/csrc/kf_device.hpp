@@ -341,21 +341,31 @@ __device__ inline double stepScaling(double tol, double err) {
   return fmin(fmax(x, 0.25), 4.0);
 }
 
-// EigenStepper::step (EigenStepper.ipp:169-388) + transportMatrix
-// (EigenStepperDefaultExtension.hpp:137-228), propagation direction +1.
-__device__ inline int rkStep(Track& T, const DevField& F, const DevOpts& o) {
-  const V3 pos = T.pos;
-  const V3 dir = T.dir;
-  const double qop = T.qop;
-  V3 B1;
-  if (const int fs = fieldAt(F, pos, B1)) return fs;
-  const V3 c1 = cross(dir, B1);
-  const V3 k1 = mk(qop * c1.x, qop * c1.y, qop * c1.z);
+// Everything one accepted RK step leaves for the transport-matrix update
+// and the state update.
+struct RkStage {
+  V3 pos, dir, B1, B2, B3, k1, k2, k3, k4;
+  double qop, h, h2, half_h, err, initialH;
+  int nTrials;
+};
+
+// EigenStepper::step (EigenStepper.ipp:169-388): the adaptive RKN4 trials
+// until the error estimate is accepted.  Returns 0 or a status.
+__device__ inline int rkTrials(const Track& T, const DevField& F, const DevOpts& o, RkStage& S) {
+  S.pos = T.pos;
+  S.dir = T.dir;
+  S.qop = T.qop;
+  const V3& pos = S.pos;
+  const V3& dir = S.dir;
+  const double qop = S.qop;
+  if (const int fs = fieldAt(F, pos, S.B1)) return fs;
+  const V3 c1 = cross(dir, S.B1);
+  S.k1 = mk(qop * c1.x, qop * c1.y, qop * c1.z);
+  const V3& k1 = S.k1;
 
   double h = stepValue(T);
-  const double initialH = h;
+  S.initialH = h;
   double h2, half_h, err;
-  V3 B2, B3, k2, k3, k4;
   int nTrials = 0;
   while (true) {
     ++nTrials;
@@ -365,18 +375,18 @@ __device__ inline int rkStep(Track& T, const DevField& F, const DevOpts& o) {
     const V3 p1 = mk((pos.x + half_h * dir.x) + h28 * k1.x,
                      (pos.y + half_h * dir.y) + h28 * k1.y,
                      (pos.z + half_h * dir.z) + h28 * k1.z);
-    if (const int fs = fieldAt(F, p1, B2)) return fs;
-    k2 = kEval(qop, dir, half_h, k1, B2);
-    k3 = kEval(qop, dir, half_h, k2, B2);
+    if (const int fs = fieldAt(F, p1, S.B2)) return fs;
+    S.k2 = kEval(qop, dir, half_h, k1, S.B2);
+    S.k3 = kEval(qop, dir, half_h, S.k2, S.B2);
     const double h22 = h2 * 0.5;
-    const V3 p2 = mk((pos.x + h * dir.x) + h22 * k3.x,
-                     (pos.y + h * dir.y) + h22 * k3.y,
-                     (pos.z + h * dir.z) + h22 * k3.z);
-    if (const int fs = fieldAt(F, p2, B3)) return fs;
-    k4 = kEval(qop, dir, h, k3, B3);
-    const double e0 = ((k1.x - k2.x) - k3.x) + k4.x;
-    const double e1 = ((k1.y - k2.y) - k3.y) + k4.y;
-    const double e2 = ((k1.z - k2.z) - k3.z) + k4.z;
+    const V3 p2 = mk((pos.x + h * dir.x) + h22 * S.k3.x,
+                     (pos.y + h * dir.y) + h22 * S.k3.y,
+                     (pos.z + h * dir.z) + h22 * S.k3.z);
+    if (const int fs = fieldAt(F, p2, S.B3)) return fs;
+    S.k4 = kEval(qop, dir, h, S.k3, S.B3);
+    const double e0 = ((k1.x - S.k2.x) - S.k3.x) + S.k4.x;
+    const double e1 = ((k1.y - S.k2.y) - S.k3.y) + S.k4.y;
+    const double e2 = ((k1.z - S.k2.z) - S.k3.z) + S.k4.z;
     err = h2 * ((fabs(e0) + fabs(e1)) + fabs(e2));
     err = fmax(1e-20, err);
     if (err <= 4.0 * o.stepTolerance) break;
@@ -385,132 +395,194 @@ __device__ inline int rkStep(Track& T, const DevField& F, const DevOpts& o) {
     if (static_cast<uint32_t>(nTrials) > o.maxTrials)
       return ACTS_KF_STATUS_STEP_ADJUSTMENT_FAILED;
   }
+  S.h = h;
+  S.h2 = h2;
+  S.half_h = half_h;
+  S.err = err;
+  S.nTrials = nTrials;
+  return 0;
+}
 
-  // propagateTime
+// propagateTime (EigenStepperDefaultExtension.hpp:112-125); returns dt/ds
+// and the momentum p used by the transport matrix.
+__device__ inline double rkTime(Track& T, const DevOpts& o, const RkStage& S, double& p) {
   const float m = o.mass;
   // ParticleHypothesis::extractMomentum: float(copysign(|q|, q/p)) / (q/p)
-  const double p = copysign(static_cast<double>(o.absQ), qop) / qop;
+  p = copysign(static_cast<double>(o.absQ), S.qop) / S.qop;
   const double mp = m / p;
   const double dtds = sqrt(1 + mp * mp);
-  T.time += h * dtds;
+  T.time += S.h * dtds;
   T.der[3] = dtds;
+  return dtds;
+}
 
-  // transport matrix D (only the non-trivial blocks)
-  const double h6 = h / 6.;
-  const double hh6 = (h * h) / 6.;
-  const double qh2 = qop * half_h;
-  const double qh = qop * h;
-  V3 dk1dL = cross(dir, B1);
-  V3 dk2dL, dk3dL, dk4dL;
-  {
-    const V3 a = cross(mk(dir.x + half_h * k1.x, dir.y + half_h * k1.y,
-                          dir.z + half_h * k1.z), B2);
-    const V3 b = cross(dk1dL, B2);
-    dk2dL = mk(a.x + qh2 * b.x, a.y + qh2 * b.y, a.z + qh2 * b.z);
-  }
-  {
-    const V3 a = cross(mk(dir.x + half_h * k2.x, dir.y + half_h * k2.y,
-                          dir.z + half_h * k2.z), B2);
-    const V3 b = cross(dk2dL, B2);
-    dk3dL = mk(a.x + qh2 * b.x, a.y + qh2 * b.y, a.z + qh2 * b.z);
-  }
-  {
-    const V3 a = cross(mk(dir.x + h * k3.x, dir.y + h * k3.y, dir.z + h * k3.z), B3);
-    const V3 b = cross(dk3dL, B3);
-    dk4dL = mk(a.x + qh * b.x, a.y + qh * b.y, a.z + qh * b.z);
-  }
-  double dk1dT[9], dk2dT[9], dk3dT[9], dk4dT[9];
-  dk1dT[0] = qop * 0.0;     dk1dT[1] = qop * B1.z;  dk1dT[2] = qop * -B1.y;
-  dk1dT[3] = qop * -B1.z;   dk1dT[4] = qop * 0.0;   dk1dT[5] = qop * B1.x;
-  dk1dT[6] = qop * B1.y;    dk1dT[7] = qop * -B1.x; dk1dT[8] = qop * 0.0;
+// d(k_i)/dT of the four stages, column j (j = 0..2) only.
+__device__ inline void dkdTColumn(const RkStage& S, int j, double c1[3], double c2[3],
+                                  double c3[3], double c4[3]) {
+  const double qop = S.qop;
+  const V3& B1 = S.B1;
+  // column j of dk1dT = qop * (e_j x B1)
+  const double d1[9] = {qop * 0.0, qop * B1.z, qop * -B1.y, qop * -B1.z, qop * 0.0,
+                        qop * B1.x, qop * B1.y, qop * -B1.x, qop * 0.0};
+  c1[0] = d1[j];
+  c1[1] = d1[3 + j];
+  c1[2] = d1[6 + j];
   auto stage = [&](const double* prev, double hh, const V3& B, double* out) {
-    double m3[9];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) m3[3 * i + j] = (i == j ? 1.0 : 0.0) + hh * prev[3 * i + j];
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const V3 c = cross(mk(m3[j], m3[3 + j], m3[6 + j]), B);
-      out[j] = qop * c.x;
-      out[3 + j] = qop * c.y;
-      out[6 + j] = qop * c.z;
-    }
+    const double m0 = (j == 0 ? 1.0 : 0.0) + hh * prev[0];
+    const double m1 = (j == 1 ? 1.0 : 0.0) + hh * prev[1];
+    const double m2 = (j == 2 ? 1.0 : 0.0) + hh * prev[2];
+    const V3 c = cross(mk(m0, m1, m2), B);
+    out[0] = qop * c.x;
+    out[1] = qop * c.y;
+    out[2] = qop * c.z;
   };
-  stage(dk1dT, half_h, B2, dk2dT);
-  stage(dk2dT, half_h, B2, dk3dT);
-  stage(dk3dT, h, B3, dk4dT);
+  stage(c1, S.half_h, S.B2, c2);
+  stage(c2, S.half_h, S.B2, c3);
+  stage(c3, S.h, S.B3, c4);
+}
 
-  double dF[12], dG[12];  // rows 0..2 of D_TR and D_BR (cols 0..3)
+// d(k_i)/d(q/p) of the four stages.
+__device__ inline void dkdL(const RkStage& S, V3& d1, V3& d2, V3& d3, V3& d4) {
+  const V3& dir = S.dir;
+  const double qh2 = S.qop * S.half_h;
+  const double qh = S.qop * S.h;
+  d1 = cross(dir, S.B1);
+  {
+    const V3 a = cross(mk(dir.x + S.half_h * S.k1.x, dir.y + S.half_h * S.k1.y,
+                          dir.z + S.half_h * S.k1.z), S.B2);
+    const V3 b = cross(d1, S.B2);
+    d2 = mk(a.x + qh2 * b.x, a.y + qh2 * b.y, a.z + qh2 * b.z);
+  }
+  {
+    const V3 a = cross(mk(dir.x + S.half_h * S.k2.x, dir.y + S.half_h * S.k2.y,
+                          dir.z + S.half_h * S.k2.z), S.B2);
+    const V3 b = cross(d2, S.B2);
+    d3 = mk(a.x + qh2 * b.x, a.y + qh2 * b.y, a.z + qh2 * b.z);
+  }
+  {
+    const V3 a = cross(mk(dir.x + S.h * S.k3.x, dir.y + S.h * S.k3.y, dir.z + S.h * S.k3.z),
+                       S.B3);
+    const V3 b = cross(d3, S.B3);
+    d4 = mk(a.x + qh * b.x, a.y + qh * b.y, a.z + qh * b.z);
+  }
+}
+
+// Rows 0..2 of the non-trivial transport blocks D_TR (dF) and D_BR (dG),
+// column j (0..2 from dk/dT, 3 from dk/dL): out[i] = dF[4i+j], out[3+i] = dG[4i+j].
+__device__ inline void dFdGColumn(const RkStage& S, int j, const double* c1, const double* c2,
+                                  const double* c3, const double* c4, double out[6]) {
+  const double h = S.h;
+  const double h6 = h / 6.;
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const int e = 3 * i + j;
-      const double id = (i == j ? 1.0 : 0.0);
-      dF[4 * i + j] = (id + h6 * ((dk1dT[e] + dk2dT[e]) + dk3dT[e])) * h;
-      dG[4 * i + j] = id + h6 * ((dk1dT[e] + 2. * (dk2dT[e] + dk3dT[e])) + dk4dT[e]);
-    }
+    const double id = (i == j ? 1.0 : 0.0);
+    out[i] = (id + h6 * ((c1[i] + c2[i]) + c3[i])) * h;
+    out[3 + i] = id + h6 * ((c1[i] + 2. * (c2[i] + c3[i])) + c4[i]);
   }
-  dF[3] = hh6 * ((dk1dL.x + dk2dL.x) + dk3dL.x);
-  dF[7] = hh6 * ((dk1dL.y + dk2dL.y) + dk3dL.y);
-  dF[11] = hh6 * ((dk1dL.z + dk2dL.z) + dk3dL.z);
-  dG[3] = h6 * ((dk1dL.x + 2. * (dk2dL.x + dk3dL.x)) + dk4dL.x);
-  dG[7] = h6 * ((dk1dL.y + 2. * (dk2dL.y + dk3dL.y)) + dk4dL.y);
-  dG[11] = h6 * ((dk1dL.z + 2. * (dk2dL.z + dk3dL.z)) + dk4dL.z);
-  const double D37 = h * m * m / (p * p * qop * dtds);
+}
+__device__ inline void dFdGLambda(const RkStage& S, const V3& d1, const V3& d2, const V3& d3,
+                                  const V3& d4, double out[6]) {
+  const double h = S.h;
+  const double h6 = h / 6.;
+  const double hh6 = (h * h) / 6.;
+  out[0] = hh6 * ((d1.x + d2.x) + d3.x);
+  out[1] = hh6 * ((d1.y + d2.y) + d3.y);
+  out[2] = hh6 * ((d1.z + d2.z) + d3.z);
+  out[3] = h6 * ((d1.x + 2. * (d2.x + d3.x)) + d4.x);
+  out[4] = h6 * ((d1.y + 2. * (d2.y + d3.y)) + d4.y);
+  out[5] = h6 * ((d1.z + 2. * (d2.z + d3.z)) + d4.z);
+}
+__device__ inline double transportD37(const DevOpts& o, const RkStage& S, double p, double dtds) {
+  const float m = o.mass;
+  return S.h * m * m / (p * p * S.qop * dtds);
+}
 
-  // EigenStepper.ipp:341-347: TR += D_TR * BR ; BR = D_BR * BR
-  // (BR row 3 is (0,0,0,1), D_TR row 3 is (0,0,0,D37))
+// EigenStepper.ipp:341-347: TR += D_TR * BR ; BR = D_BR * BR, entry (i, j)
+// (BR row 3 is (0,0,0,1), D_TR row 3 is (0,0,0,D37)).  dF/dG rows i hold the
+// columns 0..3 of row i.
+__device__ inline void blockEntry(int i, int j, const double* dFrow, const double* dGrow,
+                                  const double* br, double tr, double& ntr, double& nbr) {
+  double pt = (dFrow[0] * br[j] + dFrow[1] * br[4 + j]) + dFrow[2] * br[8 + j];
+  double pb = (dGrow[0] * br[j] + dGrow[1] * br[4 + j]) + dGrow[2] * br[8 + j];
+  if (j == 3) {
+    pt = pt + dFrow[3];
+    pb = pb + dGrow[3];
+  }
+  ntr = tr + pt;
+  nbr = pb;
+  (void)i;
+}
+
+// The transport-matrix update of one accepted step, in one lane.
+__device__ inline void rkTransport(Track& T, const DevOpts& o, const RkStage& S, double p,
+                                   double dtds) {
+  double col[4][6];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    double c1[3], c2[3], c3[3], c4[3];
+    dkdTColumn(S, j, c1, c2, c3, c4);
+    dFdGColumn(S, j, c1, c2, c3, c4, col[j]);
+  }
+  V3 d1, d2, d3, d4;
+  dkdL(S, d1, d2, d3, d4);
+  dFdGLambda(S, d1, d2, d3, d4, col[3]);
   double ntr[12], nbr[12];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
+    const double dFrow[4] = {col[0][i], col[1][i], col[2][i], col[3][i]};
+    const double dGrow[4] = {col[0][3 + i], col[1][3 + i], col[2][3 + i], col[3][3 + i]};
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      double pt = (dF[4 * i + 0] * T.br[j] + dF[4 * i + 1] * T.br[4 + j]) +
-                  dF[4 * i + 2] * T.br[8 + j];
-      double pb = (dG[4 * i + 0] * T.br[j] + dG[4 * i + 1] * T.br[4 + j]) +
-                  dG[4 * i + 2] * T.br[8 + j];
-      if (j == 3) {
-        pt = pt + dF[4 * i + 3];
-        pb = pb + dG[4 * i + 3];
-      }
-      ntr[4 * i + j] = T.tr[4 * i + j] + pt;
-      nbr[4 * i + j] = pb;
-    }
+    for (int j = 0; j < 4; ++j)
+      blockEntry(i, j, dFrow, dGrow, T.br, T.tr[4 * i + j], ntr[4 * i + j], nbr[4 * i + j]);
   }
 #pragma unroll
   for (int i = 0; i < 12; ++i) {
     T.tr[i] = ntr[i];
     T.br[i] = nbr[i];
   }
-  T.tr37 = T.tr37 + D37;
+  T.tr37 = T.tr37 + transportD37(o, S, p, dtds);
+}
 
-  // EigenStepper.ipp:355-359
-  const double h26 = h2 / 6.;
-  T.pos = mk(pos.x + (h * dir.x + h26 * ((k1.x + k2.x) + k3.x)),
-             pos.y + (h * dir.y + h26 * ((k1.y + k2.y) + k3.y)),
-             pos.z + (h * dir.z + h26 * ((k1.z + k2.z) + k3.z)));
-  V3 nd = mk(dir.x + h6 * ((k1.x + 2. * (k2.x + k3.x)) + k4.x),
-             dir.y + h6 * ((k1.y + 2. * (k2.y + k3.y)) + k4.y),
-             dir.z + h6 * ((k1.z + 2. * (k2.z + k3.z)) + k4.z));
+// EigenStepper.ipp:355-385: position, direction, derivative, path,
+// statistics, accuracy.
+__device__ inline void rkApply(Track& T, const DevOpts& o, const RkStage& S) {
+  const V3& pos = S.pos;
+  const V3& dir = S.dir;
+  const double h = S.h;
+  const double h6 = h / 6.;
+  const double h26 = S.h2 / 6.;
+  T.pos = mk(pos.x + (h * dir.x + h26 * ((S.k1.x + S.k2.x) + S.k3.x)),
+             pos.y + (h * dir.y + h26 * ((S.k1.y + S.k2.y) + S.k3.y)),
+             pos.z + (h * dir.z + h26 * ((S.k1.z + S.k2.z) + S.k3.z)));
+  V3 nd = mk(dir.x + h6 * ((S.k1.x + 2. * (S.k2.x + S.k3.x)) + S.k4.x),
+             dir.y + h6 * ((S.k1.y + 2. * (S.k2.y + S.k3.y)) + S.k4.y),
+             dir.z + h6 * ((S.k1.z + 2. * (S.k2.z + S.k3.z)) + S.k4.z));
   T.dir = normalized(nd);
   T.der[0] = T.dir.x;
   T.der[1] = T.dir.y;
   T.der[2] = T.dir.z;
-  T.der[4] = k4.x;
-  T.der[5] = k4.y;
-  T.der[6] = k4.z;
+  T.der[4] = S.k4.x;
+  T.der[5] = S.k4.y;
+  T.der[6] = S.k4.z;
   T.pathAcc += h;
   ++T.nSteps;
-  T.nTrials += nTrials;
-
-  // EigenStepper.ipp:379-385
-  const double nextAccuracy = fabs(h * stepScaling(o.stepTolerance, err));
+  T.nTrials += S.nTrials;
+  const double nextAccuracy = fabs(h * stepScaling(o.stepTolerance, S.err));
   const double previousAccuracy = fabs(T.cAcc);
-  const double initialStepLength = fabs(initialH);
+  const double initialStepLength = fabs(S.initialH);
   if (nextAccuracy < initialStepLength || nextAccuracy > previousAccuracy)
     T.cAcc = nextAccuracy;
+}
+
+// EigenStepper::step (EigenStepper.ipp:169-388) + transportMatrix
+// (EigenStepperDefaultExtension.hpp:137-228), propagation direction +1.
+__device__ inline int rkStep(Track& T, const DevField& F, const DevOpts& o) {
+  RkStage S;
+  if (const int st = rkTrials(T, F, o, S)) return st;
+  double p;
+  const double dtds = rkTime(T, o, S, p);
+  rkTransport(T, o, S, p, dtds);
+  rkApply(T, o, S);
   return 0;
 }
 
