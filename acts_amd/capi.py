@@ -111,7 +111,6 @@ class Options(C.Structure):
         ("sched_inner_steps", C.c_int32),
         ("sched_wait_64ths", C.c_int32),
         ("sched_quarantine_steps", C.c_int32),
-        ("sched_no_coop", C.c_int32),
         ("kernel_timing", C.c_int32),
         ("use_graph", C.c_int32),
     ]

@@ -58,7 +58,7 @@ constexpr int64_t kMaxBatch = (int64_t{1} << 31) - (int64_t{1} << 24);
 template <bool Mixed>
 __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKernel(
     FwdArgs a, int32_t* __restrict__ next, int32_t maxInner, int32_t waitFrac,
-    int32_t quarantine, int32_t coop, const int32_t* __restrict__ order) {
+    int32_t quarantine, const int32_t* __restrict__ order) {
   const int lane = static_cast<int>(threadIdx.x) & 63;
   const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t n = a.in.n;
@@ -67,7 +67,6 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
   // detectors), else read them from global memory.
   const DevSurface* geo = a.geo;
   const DevSlab* slabs = a.slabs;
-  __shared__ CoopBuf coopBuf;
 #if KF_GEOMETRY_IN_LDS
   __shared__ DevSurface sGeo[kLdsSurfaces];
   __shared__ DevSlab sSlabs[kLdsSlabs];
@@ -112,16 +111,7 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
     const uint64_t busy = __ballot(phase != kPhaseRetired);
     if (!busy) break;
     for (int it = 0; it < maxInner; ++it) {
-      // a single stepping lane gets the whole wavefront's help with its
-      // transport-matrix update (FP64 path)
-      const uint64_t stepping = __ballot(phase == kPhaseStepping);
-      if (!Mixed && coop && __popcll(stepping) == 1) {
-        const int owner = __ffsll(static_cast<long long>(stepping)) - 1;
-        const int r = laneStepCoop(L, a, geo, owner, coopBuf);
-        if (lane == owner) phase = r;
-      } else if (phase == kPhaseStepping) {
-        phase = laneStep<Mixed>(L, a, geo);
-      }
+      if (phase == kPhaseStepping) phase = laneStep<Mixed>(L, a, geo);
       const int nStep = __popcll(__ballot(phase == kPhaseStepping));
       const int nWait = __popcll(__ballot(phase == kPhaseSurface || phase == kPhaseFinished));
       if (nStep == 0 || 64 * nWait >= waitFrac * (nStep + nWait)) break;
@@ -284,7 +274,6 @@ struct acts_kf_ctx {
   int32_t innerSteps = 16;
   int32_t waitFrac = 32;
   int32_t quarantine = 0;
-  int32_t coop = 1;
   // staging for the host-buffer entry point
   DevBuf inBuf, outBuf;
 };
@@ -495,7 +484,6 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   if (options->sched_inner_steps > 0) c->innerSteps = options->sched_inner_steps;
   if (options->sched_wait_64ths > 0) c->waitFrac = std::min(options->sched_wait_64ths, 64);
   c->quarantine = std::max(options->sched_quarantine_steps, 0);
-  c->coop = options->sched_no_coop ? 0 : 1;
   {
     int blocksPerCu = 0, cus = 0;
     const hipError_t oe =
@@ -633,10 +621,10 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   if (timing) KF_HIP(hipEventRecord(c->ev[0], stream));
   if (c->mixed)
     hipLaunchKernelGGL(fitForwardKernel<true>, dim3(static_cast<unsigned>(waves)), dim3(kBlock),
-                       0, stream, fa, queue, c->innerSteps, c->waitFrac, c->quarantine, c->coop, perm);
+                       0, stream, fa, queue, c->innerSteps, c->waitFrac, c->quarantine, perm);
   else
     hipLaunchKernelGGL(fitForwardKernel<false>, dim3(static_cast<unsigned>(waves)), dim3(kBlock),
-                       0, stream, fa, queue, c->innerSteps, c->waitFrac, c->quarantine, c->coop, perm);
+                       0, stream, fa, queue, c->innerSteps, c->waitFrac, c->quarantine, perm);
   if (timing) KF_HIP(hipEventRecord(c->ev[1], stream));
   hipLaunchKernelGGL(smoothKernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream,
                      b, d, sc, perm);

@@ -208,98 +208,6 @@ __device__ inline int laneAfterStep(Lane& L, const FwdArgs& a, const DevSurface*
   return kPhaseStepping;
 }
 
-// LDS exchange area of a wavefront for the cooperative step.
-struct CoopBuf {
-  RkStage S;
-  double p, dtds;
-  double tr[12], br[12];
-  double col[4][6];
-  double ntr[12], nbr[12];
-  int status;
-  int pad;
-};
-
-// One propagation-loop iteration of the single stepping lane `owner`, with
-// the transport-matrix update spread over the wavefront: lanes 0-2 compute
-// the dk/dT columns and lane 3 the dk/dL chain with their D-block columns,
-// then lanes 0-11 one (TR, BR) block entry each.  Every value is computed by
-// the same expression as in rkTransport, so the result is identical.  All
-// lanes of the wavefront call this together.
-__device__ inline int laneStepCoop(Lane& L, const FwdArgs& a, const DevSurface* geo, int owner,
-                                   CoopBuf& cb) {
-  const DevOpts& o = a.o;
-  const int lane = static_cast<int>(threadIdx.x) & 63;
-  const bool me = lane == owner;
-  int ret = kPhaseStepping;
-  if (me) {
-    int st = 0;
-    if (L.iter >= o.maxSteps) {
-      L.status = ACTS_KF_STATUS_STEP_COUNT_LIMIT;
-      st = -1;
-    } else {
-      ++L.iter;
-      st = rkTrials(L.T, a.F, o, cb.S);
-      if (st) {
-        L.status = st;
-      } else {
-        cb.dtds = rkTime(L.T, o, cb.S, cb.p);
-#pragma unroll
-        for (int i = 0; i < 12; ++i) {
-          cb.tr[i] = L.T.tr[i];
-          cb.br[i] = L.T.br[i];
-        }
-      }
-    }
-    cb.status = st;
-  }
-  __syncthreads();
-  const int st = cb.status;
-  if (st == 0) {
-    if (lane < 3) {
-      double c1[3], c2[3], c3[3], c4[3], col[6];
-      dkdTColumn(cb.S, lane, c1, c2, c3, c4);
-      dFdGColumn(cb.S, lane, c1, c2, c3, c4, col);
-#pragma unroll
-      for (int i = 0; i < 6; ++i) cb.col[lane][i] = col[i];
-    } else if (lane == 3) {
-      V3 d1, d2, d3, d4;
-      double col[6];
-      dkdL(cb.S, d1, d2, d3, d4);
-      dFdGLambda(cb.S, d1, d2, d3, d4, col);
-#pragma unroll
-      for (int i = 0; i < 6; ++i) cb.col[3][i] = col[i];
-    }
-    __syncthreads();
-    if (lane < 12) {
-      const int i = lane >> 2, j = lane & 3;
-      const double dFrow[4] = {cb.col[0][i], cb.col[1][i], cb.col[2][i], cb.col[3][i]};
-      const double dGrow[4] = {cb.col[0][3 + i], cb.col[1][3 + i], cb.col[2][3 + i],
-                               cb.col[3][3 + i]};
-      double ntr, nbr;
-      blockEntry(i, j, dFrow, dGrow, cb.br, cb.tr[lane], ntr, nbr);
-      cb.ntr[lane] = ntr;
-      cb.nbr[lane] = nbr;
-    }
-    __syncthreads();
-    if (me) {
-      Track& T = L.T;
-#pragma unroll
-      for (int i = 0; i < 12; ++i) {
-        T.tr[i] = cb.ntr[i];
-        T.br[i] = cb.nbr[i];
-      }
-      T.tr37 = T.tr37 + transportD37(o, cb.S, cb.p, cb.dtds);
-      rkApply(T, o, cb.S);
-      ret = laneAfterStep<false>(L, a, geo);
-    }
-  } else if (me) {
-    ret = kPhaseFinished;
-  }
-  // the buffer is reused by the next call
-  __syncthreads();
-  return ret;
-}
-
 // KalmanFitter::Actor::filter on the reached surface (KalmanFitter.hpp:
 // 339-671), then the rest of the propagation-loop iteration.
 template <bool Mixed>

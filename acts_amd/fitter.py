@@ -47,7 +47,6 @@ class KalmanFitterOptions:
     sched_inner_steps: int = 0              # 0 = library default
     sched_wait_64ths: int = 0               # 0 = library default
     sched_quarantine_steps: int = 0         # 0 = off
-    sched_no_coop: bool = False             # no wavefront help for lone tracks
     kernel_timing: bool = False             # events around the kernels
     use_graph: bool = False                 # replay fits as a captured hipGraph
 

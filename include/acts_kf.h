@@ -161,14 +161,11 @@ typedef struct acts_kf_options {
    * sched_wait_64ths  the surface update runs early once this many 64ths of
    *                   the busy lanes of a wavefront wait (0 = 32);
    * sched_quarantine_steps  a wavefront takes no new tracks while one of its
-   *                   tracks has run more than this many steps (0 = off);
-   * sched_no_coop     1 = never spread a lone track's transport-matrix update
-   *                   over its wavefront (0 = do, the default).             */
+   *                   tracks has run more than this many steps (0 = off).   */
   int32_t length_sorted;
   int32_t sched_inner_steps;
   int32_t sched_wait_64ths;
   int32_t sched_quarantine_steps;
-  int32_t sched_no_coop;
   /* 1 = record HIP events around the kernels of each fit, for
    * acts_kf_kernel_times (no reference counterpart)                         */
   int32_t kernel_timing;

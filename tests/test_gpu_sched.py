@@ -1,6 +1,6 @@
 """Scheduling knobs change only the speed of a fit: results are bit for bit
-the same with the cooperative step on or off, with quarantine, other inner
-step counts and wait fractions, and in input order."""
+the same with quarantine, other inner step counts and wait fractions, and in
+input order."""
 import numpy as np
 import pytest
 
@@ -10,7 +10,7 @@ from acts_amd.fitter import ALL_STATE_OUTPUTS
 pytestmark = pytest.mark.gpu
 
 VARIANTS = [
-    dict(sched_no_coop=True),
+    dict(sched_quarantine_steps=100),
     dict(sched_quarantine_steps=20, sched_inner_steps=64, sched_wait_64ths=64),
     dict(sched_inner_steps=4, sched_wait_64ths=16),
     dict(length_sorted=False),
