@@ -167,25 +167,79 @@ __device__ inline int32_t queueKey(const QueueKeyArgs& k, int64_t t) {
   }
   return len;
 }
-__global__ void lengthHistogramKernel(QueueKeyArgs k, int64_t n, int32_t* hist) {
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  atomicAdd(&hist[queueKey(k, t)], 1);
+// Counting sort of the tracks by queue key, largest key first, in three
+// passes without global atomics: per-block histograms (LDS atomics) stored
+// key-major with the largest key first, so that one exclusive scan over the
+// whole array gives every (key, block) its output offset; then a per-block
+// scatter (LDS atomics for the rank inside a block and key; the order inside
+// a bucket is not fixed, which only changes the queue order).
+constexpr int kSortBlock = 256;
+constexpr int kScanBlock = 1024;
+constexpr int kMaxKeys = 1026;  // lengths above 1023 share a bucket
+
+__device__ inline int32_t sortKey(const QueueKeyArgs& k, int64_t t) {
+  return min(queueKey(k, t), kMaxKeys - 1);
 }
-// exclusive scan from the largest key down: bucket starts, largest key first
-__global__ void lengthScanKernel(int32_t maxKey, int32_t* hist) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int32_t acc = 0;
-  for (int32_t key = maxKey; key >= 0; --key) {
-    const int32_t c = hist[key];
-    hist[key] = acc;
+__device__ inline int64_t histIndex(int32_t key, int32_t nKeys, int64_t block, int64_t nBlocks) {
+  return static_cast<int64_t>(nKeys - 1 - key) * nBlocks + block;
+}
+
+__global__ __launch_bounds__(kSortBlock) void keyHistogramKernel(QueueKeyArgs k, int64_t n,
+                                                                 int32_t nKeys,
+                                                                 int32_t* blockHist) {
+  __shared__ int32_t h[kMaxKeys];
+  for (int i = threadIdx.x; i < nKeys; i += kSortBlock) h[i] = 0;
+  __syncthreads();
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kSortBlock + threadIdx.x;
+  if (t < n) atomicAdd(&h[sortKey(k, t)], 1);
+  __syncthreads();
+  for (int i = threadIdx.x; i < nKeys; i += kSortBlock)
+    blockHist[histIndex(i, nKeys, blockIdx.x, gridDim.x)] = h[i];
+}
+
+// Exclusive prefix sum of `count` ints in place, one workgroup: each thread
+// sums a contiguous chunk, the chunk sums are scanned in LDS, each thread
+// rewrites its chunk.
+__global__ __launch_bounds__(kScanBlock) void exclusiveScanKernel(int32_t* a, int64_t count) {
+  __shared__ int32_t part[kScanBlock];
+  const int64_t chunk = (count + kScanBlock - 1) / kScanBlock;
+  const int64_t lo = threadIdx.x * chunk;
+  const int64_t hi = min(lo + chunk, count);
+  int32_t sum = 0;
+  for (int64_t i = lo; i < hi; ++i) sum += a[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t acc = 0;
+    for (int i = 0; i < kScanBlock; ++i) {
+      const int32_t c = part[i];
+      part[i] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+  int32_t acc = part[threadIdx.x];
+  for (int64_t i = lo; i < hi; ++i) {
+    const int32_t c = a[i];
+    a[i] = acc;
     acc += c;
   }
 }
-__global__ void lengthScatterKernel(QueueKeyArgs k, int64_t n, int32_t* cursor, int32_t* perm) {
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  perm[atomicAdd(&cursor[queueKey(k, t)], 1)] = static_cast<int32_t>(t);
+
+__global__ __launch_bounds__(kSortBlock) void keyScatterKernel(QueueKeyArgs k, int64_t n,
+                                                               int32_t nKeys,
+                                                               const int32_t* blockOffset,
+                                                               int32_t* perm) {
+  __shared__ int32_t h[kMaxKeys];
+  for (int i = threadIdx.x; i < nKeys; i += kSortBlock) h[i] = 0;
+  __syncthreads();
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kSortBlock + threadIdx.x;
+  if (t < n) {
+    const int32_t key = sortKey(k, t);
+    const int32_t rank = atomicAdd(&h[key], 1);
+    perm[blockOffset[histIndex(key, nKeys, blockIdx.x, gridDim.x)] + rank] =
+        static_cast<int32_t>(t);
+  }
 }
 
 __global__ void maxLengthKernel(const int64_t* off, int64_t n, int32_t* result) {
@@ -519,8 +573,10 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
       c->queueBuf.ensure(sizeof(int32_t)))
     return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "scratch allocation failed");
   const bool sorted = c->lengthSorted && n > kBlock;
+  const int32_t nKeys = std::min(states + 2, kMaxKeys);
+  const int64_t sortBlocks = (n + kSortBlock - 1) / kSortBlock;
   if (sorted && (c->permBuf.ensure(sizeof(int32_t) * n) ||
-                 c->histBuf.ensure(sizeof(int32_t) * (states + 2))))
+                 c->histBuf.ensure(sizeof(int32_t) * nKeys * sortBlocks)))
     return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "permutation allocation failed");
   // Replay the captured launch sequence when nothing it depends on changed.
   GraphKey key;
@@ -589,9 +645,6 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   if (sorted) {
     perm = static_cast<int32_t*>(c->permBuf.p);
     int32_t* hist = static_cast<int32_t*>(c->histBuf.p);
-    hipLaunchKernelGGL(zeroKernel, dim3(static_cast<unsigned>((states + 2 + 255) / 256)),
-                       dim3(256), 0, stream, hist, states + 2);
-    const unsigned nb = static_cast<unsigned>((n + 255) / 256);
     QueueKeyArgs qk;
     qk.seqOff = in->seq_offsets;
     qk.measOff = in->meas_offsets;
@@ -600,9 +653,13 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
     qk.geo = c->geo;
     qk.nSurfaces = c->nSurfaces;
     qk.maxLen = states;
-    hipLaunchKernelGGL(lengthHistogramKernel, dim3(nb), dim3(256), 0, stream, qk, n, hist);
-    hipLaunchKernelGGL(lengthScanKernel, dim3(1), dim3(64), 0, stream, states + 1, hist);
-    hipLaunchKernelGGL(lengthScatterKernel, dim3(nb), dim3(256), 0, stream, qk, n, hist, perm);
+    const unsigned nb = static_cast<unsigned>(sortBlocks);
+    hipLaunchKernelGGL(keyHistogramKernel, dim3(nb), dim3(kSortBlock), 0, stream, qk, n, nKeys,
+                       hist);
+    hipLaunchKernelGGL(exclusiveScanKernel, dim3(1), dim3(kScanBlock), 0, stream, hist,
+                       static_cast<int64_t>(nKeys) * sortBlocks);
+    hipLaunchKernelGGL(keyScatterKernel, dim3(nb), dim3(kSortBlock), 0, stream, qk, n, nKeys,
+                       hist, perm);
   }
   FwdArgs fa;
   fa.in = b;
