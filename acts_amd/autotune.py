@@ -23,12 +23,12 @@ from typing import Callable
 import numpy as np
 
 SPACE = {
-    "sched_inner_steps": [4, 8, 16, 32, 64],
+    "sched_inner_steps": [8, 16, 32, 64, 128],
     "sched_wait_64ths": [16, 32, 48, 64],
     "length_sorted": [True, False],
     "sched_quarantine_steps": [0, 100, 200, 400],
 }
-DEFAULTS = {"sched_inner_steps": 16, "sched_wait_64ths": 32, "length_sorted": True,
+DEFAULTS = {"sched_inner_steps": 64, "sched_wait_64ths": 64, "length_sorted": True,
             "sched_quarantine_steps": 0}
 
 

@@ -325,8 +325,8 @@ struct acts_kf_ctx {
   bool lengthSorted = true;
   // persistent forward kernel: resident wavefronts and scheduling knobs
   int64_t residentWaves = 0;
-  int32_t innerSteps = 16;
-  int32_t waitFrac = 32;
+  int32_t innerSteps = 64;
+  int32_t waitFrac = 64;
   int32_t quarantine = 0;
   // staging for the host-buffer entry point
   DevBuf inBuf, outBuf;

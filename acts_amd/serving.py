@@ -6,14 +6,17 @@ one (Examples/Algorithms/TrackFitting/src/TrackFittingAlgorithm.cpp:117-173).
 Here events arrive as a Poisson stream; a dispatcher groups the events that
 have arrived into batches, admitted by an HBM budget per batch, and hands
 each batch to a free slot.  A slot is one fitter context (own stream,
-staging buffers and scratch, geometry and field resident); its worker thread
-copies the batch in, fits it, copies the results (surface, flags and
-smoothed parameters of every state, track summaries) back, and stamps every event
+staging buffers and scratch, geometry and field resident) with pinned host
+buffers; its worker thread copies the batch's tracks into them, fits the
+batch (host-to-device copies, kernels, device-to-host copies on the slot's
+stream), leaving the results (surface, flags and smoothed parameters of
+every state, track summaries) in pinned memory, and stamps every event
 of the batch with its completion time.  Latency of an event = completion -
 arrival.
 """
 from __future__ import annotations
 
+import ctypes as C
 import queue
 import threading
 import time
@@ -21,6 +24,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from . import capi
 from .events import Events
 from .fitter import KalmanFitter, KalmanFitterOptions
 
@@ -40,6 +44,7 @@ class ServingConfig:
     max_tracks: int = 20000
     slots: int = 3                   # batches in flight
     hbm_budget_bytes: float = 4e9    # device memory one batch may use
+    max_batch_tracks: int = 60000    # latency bound: tracks per batch
     seed: int = 1
 
 
@@ -92,6 +97,75 @@ def batch_bytes(n_tracks: int, max_len: int) -> float:
     return n_tracks * (per_track + max_len * per_slot)
 
 
+class _Slot:
+    """One in-flight batch: a fitter context plus pinned host buffers for the
+    batch's inputs and outputs, reused from batch to batch."""
+
+    IN = (("start_params", 6, np.float64, "t"), ("start_cov", 36, np.float64, "t"),
+          ("start_plane", 12, np.float64, "t"), ("seq_surfaces", 1, np.int32, "s"),
+          ("meas_surfaces", 1, np.int32, "m"), ("meas_values", 2, np.float64, "m"),
+          ("meas_cov", 4, np.float64, "m"))
+
+    def __init__(self, fitter: KalmanFitter, cap_tracks: int, cap_seq: int, cap_meas: int):
+        import torch
+
+        def pinned(n, w, dt):
+            tdt = {np.float64: torch.float64, np.int32: torch.int32,
+                   np.int64: torch.int64, np.float32: torch.float32}[dt]
+            shape = (n,) if w == 1 else (n, w)
+            return torch.empty(shape, dtype=tdt, pin_memory=True).numpy()
+
+        self.fitter = fitter
+        caps = {"t": cap_tracks, "s": cap_seq, "m": cap_meas}
+        self.arr = {name: pinned(caps[k], w, dt) for name, w, dt, k in self.IN}
+        self.arr["seq_offsets"] = pinned(cap_tracks + 1, 1, np.int64)
+        self.arr["meas_offsets"] = pinned(cap_tracks + 1, 1, np.int64)
+        self.out = {name: pinned(cap_tracks, 1, np.int32)
+                    for name in ("status", "n_states", "ndf", "n_measurements", "n_holes",
+                                 "n_outliers", "n_steps", "n_step_trials")}
+        self.out["chi2"] = pinned(cap_tracks, 1, np.float32)
+        self.out["state_surface"] = pinned(cap_seq, 1, np.int32)
+        self.out["state_flags"] = pinned(cap_seq, 1, np.int32)
+        self.out["smoothed"] = pinned(cap_seq, 6, np.float64)
+
+    def fit(self, pool: Events, ranges) -> int:
+        """Copy the tracks pool[off:off+m] of `ranges` into the pinned inputs,
+        fit them, and leave the results in the pinned outputs."""
+        so, mo = pool.seq_offsets, pool.meas_offsets
+        a = self.arr
+        t = s = m = 0
+        a["seq_offsets"][0] = 0
+        a["meas_offsets"][0] = 0
+        for off, cnt in ranges:
+            lo, hi = off, off + cnt
+            ns, nm = int(so[hi] - so[lo]), int(mo[hi] - mo[lo])
+            a["start_params"][t:t + cnt] = pool.start_params[lo:hi]
+            a["start_cov"][t:t + cnt] = pool.start_cov[lo:hi]
+            a["start_plane"][t:t + cnt] = pool.start_plane[lo:hi]
+            a["seq_surfaces"][s:s + ns] = pool.seq_surfaces[so[lo]:so[hi]]
+            a["meas_surfaces"][m:m + nm] = pool.meas_surfaces[mo[lo]:mo[hi]]
+            a["meas_values"][m:m + nm] = pool.meas_values[mo[lo]:mo[hi]]
+            a["meas_cov"][m:m + nm] = pool.meas_cov[mo[lo]:mo[hi]]
+            a["seq_offsets"][t + 1:t + cnt + 1] = so[lo + 1:hi + 1] - so[lo] + s
+            a["meas_offsets"][t + 1:t + cnt + 1] = mo[lo + 1:hi + 1] - mo[lo] + m
+            t, s, m = t + cnt, s + ns, m + nm
+        bin_ = capi.BatchIn()
+        bin_.n_tracks = t
+        for name in ("start_params", "start_cov", "start_plane", "seq_offsets", "seq_surfaces",
+                     "meas_offsets", "meas_surfaces", "meas_values", "meas_cov"):
+            setattr(bin_, name, a[name].ctypes.data)
+        bin_.max_sequence_length = int(np.max(np.diff(a["seq_offsets"][:t + 1]))) if t else 0
+        bout = capi.BatchOut()
+        for name in capi.BATCH_OUT_FIELDS:
+            arr = self.out.get(name)
+            setattr(bout, name, arr.ctypes.data if arr is not None else None)
+        lib, ctx = self.fitter.lib, self.fitter.ctx
+        capi.check(lib.acts_kf_fit_batch(ctx, C.byref(bin_), C.byref(bout), None),
+                   "acts_kf_fit_batch")
+        capi.check(lib.acts_kf_sync(ctx), "acts_kf_sync")
+        return t
+
+
 def serve(detector, field, pool: Events, cfg: ServingConfig,
           options: KalmanFitterOptions | None = None, device: int = 0) -> dict:
     """Run the serving loop; returns throughput and latency figures."""
@@ -99,14 +173,16 @@ def serve(detector, field, pool: Events, cfg: ServingConfig,
     arrivals, mult, offs = event_stream(cfg, pool.n_tracks)
     n_ev = len(arrivals)
     done_at = np.full(n_ev, np.nan)
-    max_len = pool.max_sequence_length
-    cap_tracks = max(cfg.max_tracks, int(cfg.hbm_budget_bytes // batch_bytes(1, max_len)))
+    max_len = max(pool.max_sequence_length, 1)
+    max_meas = int(np.max(np.diff(pool.meas_offsets))) if pool.n_tracks else 1
+    cap_tracks = max(cfg.max_tracks,
+                     min(cfg.max_batch_tracks, int(cfg.hbm_budget_bytes // batch_bytes(1, max_len))))
 
-    fitters = [KalmanFitter(detector, field, options, device=device) for _ in range(cfg.slots)]
+    slots = [_Slot(KalmanFitter(detector, field, options, device=device), cap_tracks,
+                   cap_tracks * max_len, cap_tracks * max_meas) for _ in range(cfg.slots)]
     # warm each context up to the largest batch it may get (scratch sizes)
-    warm = batch_of(pool, [(0, min(cap_tracks, pool.n_tracks))])
-    for f in fitters:
-        f.fit(warm, outputs=SERVING_OUTPUTS)
+    for sl in slots:
+        sl.fit(pool, [(0, min(cap_tracks, pool.n_tracks))])
 
     free = queue.Queue()
     for i in range(cfg.slots):
@@ -116,19 +192,15 @@ def serve(detector, field, pool: Events, cfg: ServingConfig,
     t0 = [0.0]
 
     def worker(i):
-        f = fitters[i]
+        sl = slots[i]
         while True:
-            item = work[i].get()
-            if item is None:
+            ids = work[i].get()
+            if ids is None:
                 return
-            ids = item
-            ev = batch_of(pool, [(int(offs[e]), int(mult[e])) for e in ids])
-            res = f.fit(ev, outputs=SERVING_OUTPUTS)
-            now = time.perf_counter() - t0[0]
-            done_at[ids] = now
+            n = sl.fit(pool, [(int(offs[e]), int(mult[e])) for e in ids])
+            done_at[ids] = time.perf_counter() - t0[0]
             stats["batches"] += 1
-            stats["batch_tracks"].append(ev.n_tracks)
-            del res
+            stats["batch_tracks"].append(n)
             free.put(i)
 
     threads = [threading.Thread(target=worker, args=(i,), daemon=True)
@@ -140,7 +212,7 @@ def serve(detector, field, pool: Events, cfg: ServingConfig,
     while nxt < n_ev:
         now = time.perf_counter() - t0[0]
         if arrivals[nxt] > now:
-            time.sleep(min(arrivals[nxt] - now, 0.002))
+            time.sleep(min(arrivals[nxt] - now, 0.001))
             continue
         slot = free.get()  # blocks while every slot is busy
         now = time.perf_counter() - t0[0]
@@ -156,8 +228,8 @@ def serve(detector, field, pool: Events, cfg: ServingConfig,
     for th in threads:
         th.join()
     total = time.perf_counter() - t0[0]
-    for f in fitters:
-        f.close()
+    for sl in slots:
+        sl.fitter.close()
     lat = (done_at - arrivals) * 1e3
     return {"events": int(n_ev), "tracks": int(mult.sum()),
             "throughput_tracks_per_s": float(mult.sum() / total),

@@ -157,9 +157,9 @@ typedef struct acts_kf_options {
    * length_sorted     1 = the smoother visits tracks in order of decreasing
    *                   sequence length (default), 0 = input order;
    * sched_inner_steps most RK steps the forward kernel takes before the
-   *                   lanes waiting at a surface are updated (0 = 16);
+   *                   lanes waiting at a surface are updated (0 = 64);
    * sched_wait_64ths  the surface update runs early once this many 64ths of
-   *                   the busy lanes of a wavefront wait (0 = 32);
+   *                   the busy lanes of a wavefront wait (0 = 64: all);
    * sched_quarantine_steps  a wavefront takes no new tracks while one of its
    *                   tracks has run more than this many steps (0 = off).   */
   int32_t length_sorted;

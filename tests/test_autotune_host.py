@@ -5,8 +5,9 @@ from acts_amd.autotune import DEFAULTS, SPACE, coordinate_descent, signature
 
 
 def test_coordinate_descent_finds_separable_minimum():
-    target = {"sched_inner_steps": 32, "sched_wait_64ths": 48, "length_sorted": False,
-              "sched_quarantine_steps": 200}
+    target = {"sched_inner_steps": SPACE["sched_inner_steps"][3],
+              "sched_wait_64ths": SPACE["sched_wait_64ths"][2], "length_sorted": False,
+              "sched_quarantine_steps": SPACE["sched_quarantine_steps"][2]}
     calls = []
 
     def cost(cfg):
