@@ -78,18 +78,23 @@ def cpu_baseline(det, field, opts, events, threads):
             "single_thread_sample": f"first {n1} tracks, 1 thread"}
 
 
-def serving(args, fp64_tracks_per_s):
+def serving(args):
     """Event serving (tracks as in the field-map workload): Poisson arrivals,
-    lognormal multiplicity (median 5000, clipped 500-20000), offered at half
-    the batch throughput measured above; FP64 and mixed precision."""
+    lognormal multiplicity (median 5000, clipped 500-20000).  A short
+    saturated run measures the loop's capacity; events are then offered at
+    half of it, for FP64 and for mixed precision."""
     import acts_amd as A
     from acts_amd import capi
     from acts_amd.serving import ServingConfig, serve
     det, field, pool = A.workload("solenoid", n_tracks=200_000, seed=args.seed + 1)
     mean_tracks = 5000 * np.exp(0.7 ** 2 / 2)
-    rate = 0.5 * fp64_tracks_per_s / mean_tracks
+    sat = serve(det, field, pool, ServingConfig(rate_hz=1e5, duration_s=0.05, seed=args.seed),
+                A.KalmanFitterOptions())
+    capacity = sat["throughput_tracks_per_s"]
+    rate = 0.5 * capacity / mean_tracks
     cfg = ServingConfig(rate_hz=rate, duration_s=args.serving_seconds, seed=args.seed)
-    out = {"config": {"rate_hz": rate, "duration_s": cfg.duration_s,
+    out = {"config": {"capacity_tracks_per_s": capacity, "load": 0.5,
+                      "rate_hz": rate, "duration_s": cfg.duration_s,
                       "multiplicity": "lognormal median 5000 sigma 0.7 clipped 500-20000",
                       "slots": cfg.slots, "hbm_budget_bytes_per_batch": cfg.hbm_budget_bytes,
                       "tracks": "pT 0.5-100 GeV log-uniform, |eta|<2.5, 201x201x401 "
@@ -236,7 +241,7 @@ def main() -> int:
         line["config"]["scheduling"] = tuned
 
     if args.serving_seconds > 0:
-        line["serving"] = serving(args, line["value"])
+        line["serving"] = serving(args)
 
     if not args.no_cpu_baseline:
         threads = max(1, min(16, os.cpu_count() or 1))
