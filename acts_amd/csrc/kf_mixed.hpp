@@ -7,8 +7,12 @@
 // update computed in float.  The track state (position, direction, q/p,
 // time, path length, jacobian blocks) stays in double: each step's float
 // increments are added to it, so rounding does not accumulate at float
-// precision over a track.  Everything per surface (covariance transport,
-// gain-matrix update, material, smoother) stays FP64.
+// precision over a track.  The step length, the straight-line part h*dir of
+// the position update and the direction's renormalisation are double too:
+// they set where a track lands on a surface, and their float rounding would
+// show up as ~1e-5 relative q/p error at 100 GeV.  So are the time, dt/ds
+// and d(t)/d(q/p).  Everything per surface (covariance transport, gain-matrix
+// update, material, smoother) stays FP64.
 #pragma once
 
 #include "kf_device.hpp"
@@ -88,6 +92,9 @@ __device__ inline int rkStepMixed(Track& T, const DevField& F, const DevOpts& o)
   const F3 k1 = mkf(qop * c1.x, qop * c1.y, qop * c1.z);
 
   const double initialH = stepValue(T);
+  // the step length actually taken, kept in double: a step the navigator
+  // sized to reach a surface must land on it to double precision
+  double hD = initialH;
   float h = static_cast<float>(initialH);
   float h2, half_h, err;
   F3 B2, B3, k2, k3, k4;
@@ -117,16 +124,20 @@ __device__ inline int rkStepMixed(Track& T, const DevField& F, const DevOpts& o)
     err = fmaxf(1e-20f, err);
     if (err <= 4.0f * tol) break;
     h *= stepScalingF(tol, err);
-    if (fabs(static_cast<double>(h)) < fabs(o.stepSizeCutOff))
+    hD = static_cast<double>(h);
+    if (fabs(hD) < fabs(o.stepSizeCutOff))
       return ACTS_KF_STATUS_STEP_SIZE_STALLED;
     if (static_cast<uint32_t>(nTrials) > o.maxTrials) return ACTS_KF_STATUS_STEP_ADJUSTMENT_FAILED;
   }
 
+  // time and its derivatives in double (as rkTime): the time is not
+  // measured, so its smoothed value is set by these jacobian entries through
+  // the ill-conditioned RTS gain, which magnifies float rounding to ~1e-5
   const float m = o.mass;
-  const float p = copysignf(o.absQ, qop) / qop;
-  const float mp = m / p;
-  const float dtds = sqrtf(1.f + mp * mp);
-  T.time += static_cast<double>(h * dtds);
+  const double pD = copysign(static_cast<double>(o.absQ), T.qop) / T.qop;
+  const double mpD = m / pD;
+  const double dtds = sqrt(1 + mpD * mpD);
+  T.time += hD * dtds;
   T.der[3] = dtds;
 
   // transport matrix D (only the non-trivial blocks)
@@ -190,7 +201,7 @@ __device__ inline int rkStepMixed(Track& T, const DevField& F, const DevOpts& o)
   dG[3] = h6 * ((dk1dL.x + 2.f * (dk2dL.x + dk3dL.x)) + dk4dL.x);
   dG[7] = h6 * ((dk1dL.y + 2.f * (dk2dL.y + dk3dL.y)) + dk4dL.y);
   dG[11] = h6 * ((dk1dL.z + 2.f * (dk2dL.z + dk3dL.z)) + dk4dL.z);
-  const float D37 = h * m * m / (p * p * qop * dtds);
+  const double D37 = hD * m * m / (pD * pD * T.qop * dtds);
 
   float br[12];
 #pragma unroll
@@ -209,17 +220,22 @@ __device__ inline int rkStepMixed(Track& T, const DevField& F, const DevOpts& o)
       T.br[4 * i + j] = static_cast<double>(pb);
     }
   }
-  T.tr37 += static_cast<double>(D37);
+  T.tr37 += D37;
 
+  // the straight-line part h*dir in double (in float it would carry an
+  // absolute error of |h| * 6e-8, i.e. 6e-5 mm for a 1 m step, a 1e-5
+  // relative error of q/p at 100 GeV); the bending part in float
   const float h26 = h2 / 6.f;
-  T.pos.x += static_cast<double>(h * dir.x + h26 * ((k1.x + k2.x) + k3.x));
-  T.pos.y += static_cast<double>(h * dir.y + h26 * ((k1.y + k2.y) + k3.y));
-  T.pos.z += static_cast<double>(h * dir.z + h26 * ((k1.z + k2.z) + k3.z));
+  T.pos.x += hD * T.dir.x + static_cast<double>(h26 * ((k1.x + k2.x) + k3.x));
+  T.pos.y += hD * T.dir.y + static_cast<double>(h26 * ((k1.y + k2.y) + k3.y));
+  T.pos.z += hD * T.dir.z + static_cast<double>(h26 * ((k1.z + k2.z) + k3.z));
   const V3 nd = mk(T.dir.x + static_cast<double>(h6 * ((k1.x + 2.f * (k2.x + k3.x)) + k4.x)),
                    T.dir.y + static_cast<double>(h6 * ((k1.y + 2.f * (k2.y + k3.y)) + k4.y)),
                    T.dir.z + static_cast<double>(h6 * ((k1.z + 2.f * (k2.z + k3.z)) + k4.z)));
-  // renormalise with a float reciprocal norm: the correction is O(h^2 k^2)
-  const float inv = rsqrtf(static_cast<float>(dot(nd, nd)));
+  // renormalise: float reciprocal square root, one Newton step in double
+  const double nn = dot(nd, nd);
+  double inv = static_cast<double>(rsqrtf(static_cast<float>(nn)));
+  inv = inv * (1.5 - (0.5 * nn) * (inv * inv));
   T.dir = mk(nd.x * inv, nd.y * inv, nd.z * inv);
   T.der[0] = T.dir.x;
   T.der[1] = T.dir.y;
@@ -227,11 +243,11 @@ __device__ inline int rkStepMixed(Track& T, const DevField& F, const DevOpts& o)
   T.der[4] = k4.x;
   T.der[5] = k4.y;
   T.der[6] = k4.z;
-  T.pathAcc += static_cast<double>(h);
+  T.pathAcc += hD;
   ++T.nSteps;
   T.nTrials += nTrials;
 
-  const double nextAccuracy = fabs(static_cast<double>(h * stepScalingF(tol, err)));
+  const double nextAccuracy = fabs(hD * static_cast<double>(stepScalingF(tol, err)));
   const double previousAccuracy = fabs(T.cAcc);
   const double initialStepLength = fabs(initialH);
   if (nextAccuracy < initialStepLength || nextAccuracy > previousAccuracy) T.cAcc = nextAccuracy;

@@ -65,3 +65,9 @@ class DeviceResult:
 
     def host(self, name: str) -> np.ndarray:
         return self.t[name].cpu().numpy()
+
+    def host_rows(self, name: str, rows: np.ndarray) -> np.ndarray:
+        """Rows `rows` of an output array, gathered on the device first."""
+        t = self.t[name]
+        idx = torch.from_numpy(np.ascontiguousarray(rows, dtype=np.int64)).to(t.device)
+        return t.index_select(0, idx).cpu().numpy()

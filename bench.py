@@ -138,6 +138,7 @@ def main() -> int:
     import acts_amd as A
     from acts_amd import capi
     from acts_amd.device import DeviceEvents, DeviceResult
+    from acts_amd import quality
     from acts_amd.flops import batch_flops, forward_flops
 
     det, field, events = A.workload(args.workload, n_tracks=args.n_tracks, seed=args.seed)
@@ -218,32 +219,16 @@ def main() -> int:
         "wall_s_timed": wall64,
     }
 
+    first64 = quality.first_states(out64, events)
+    line["pulls"] = quality.pulls(first64, events)
     if not args.no_mixed:
         msm, _, ktm, outm = run(capi.PRECISION_MIXED)
-        s64 = out64.host("status")
-        sm = outm.host("status")
-        ns, nsm = out64.host("n_states"), outm.host("n_states")
-        p64, pm = out64.host("smoothed"), outm.host("smoothed")
-        surf64, surfm = out64.host("state_surface"), outm.host("state_surface")
-        so = events.seq_offsets
-        # compare tracks that end with the same status and state sequence
-        same = (s64 == sm) & (s64 == 0) & (ns == nsm)
-        owner = np.repeat(np.arange(n), np.diff(so))
-        slot_in = np.arange(len(owner)) - so[owner] < ns[owner]
-        bad_slot = slot_in & (surf64 != surfm)
-        same[np.unique(owner[bad_slot])] = False
-        idx = np.concatenate([np.arange(so[t], so[t] + ns[t]) for t in np.nonzero(same)[0]])
-        floors = np.array([1.0, 1.0, 1.0, 1.0, 0.01, 1.0])
-        err = np.abs(pm[idx] - p64[idx]) / np.maximum(np.abs(p64[idx]), floors)
+        firstm = quality.first_states(outm, events)
         line["mixed"] = {
             "value": n / (msm * 1e-3), "ms_per_step": msm, "dtype": "f32 stepping/field, f64 KF",
             "forward_kernel_ms": float(np.mean([k[0] for k in ktm])),
-            "status_mismatch_vs_fp64": int(np.count_nonzero(s64 != sm)),
-            "state_sequence_mismatch_vs_fp64": int(np.count_nonzero(
-                (s64 == sm) & (s64 == 0) & ~same)),
-            "smoothed_rel_err_vs_fp64": {"rms": float(np.sqrt((err ** 2).mean())),
-                                         "p999": float(np.quantile(err, 0.999)),
-                                         "max": float(err.max())}}
+            "pulls": quality.pulls(firstm, events),
+            "first_state_smoothed_rel_err_vs_fp64": quality.relative_error(firstm, first64)}
 
     if tuned:
         line["config"]["scheduling"] = tuned

@@ -100,14 +100,20 @@ def compare(got, ref, events, rel: float = 1e-10, rel_smoothed_cov: float = 1e-8
     return rep
 
 
-def error_report(got, ref, events) -> dict:
+def error_report(got, ref, events, max_chi2_ndf: float | None = None) -> dict:
     """Errors of a reduced-precision fit against the FP64 reference fit.
 
     Over the tracks that both fit with the same state sequence: max and RMS
     of the relative parameter error (floors as in `compare`) of the smoothed
     parameters, and of the smoothed covariance (relative to sqrt(C_ii C_jj)).
+    With `max_chi2_ndf`, only tracks whose reference fit has chi2/ndf at most
+    that are compared (a fit that did not converge, e.g. after a residual
+    across the +-pi wrap of loc0 = R phi, has no meaningful parameters).
     """
     same = (got.status == ref.status) & (ref.status == 0) & (got.n_states == ref.n_states)
+    n_same = int(np.count_nonzero(same))
+    if max_chi2_ndf is not None:
+        same &= ref.chi2 / np.maximum(ref.ndf, 1) <= max_chi2_ndf
     so = events.seq_offsets
     idx = np.concatenate([np.arange(so[t], so[t] + ref.n_states[t])
                           for t in np.nonzero(same)[0]]) if same.any() else np.zeros(0, int)
@@ -117,7 +123,8 @@ def error_report(got, ref, events) -> dict:
         idx = idx[seq_ok]
     rep = {"n_tracks": int(events.n_tracks),
            "status_mismatch": int(np.count_nonzero(got.status != ref.status)),
-           "n_compared_states": int(len(idx))}
+           "n_compared_states": int(len(idx)),
+           "n_not_converged": n_same - int(np.count_nonzero(same))}
     if len(idx):
         pe = param_rel_err(got.smoothed[idx], ref.smoothed[idx])
         rep["smoothed_max_rel"] = pe.max(axis=0).tolist()
