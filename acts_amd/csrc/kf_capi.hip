@@ -40,9 +40,6 @@ constexpr int kLdsSlabs = 32;
 // the track queue counts in int32 and may overshoot n by one per lane
 constexpr int64_t kMaxBatch = (int64_t{1} << 31) - (int64_t{1} << 24);
 
-#ifndef KF_GEOMETRY_IN_LDS
-#define KF_GEOMETRY_IN_LDS 1
-#endif
 #ifndef KF_FORWARD_WAVES_PER_EU
 #define KF_FORWARD_WAVES_PER_EU 1
 #endif
@@ -55,22 +52,22 @@ constexpr int64_t kMaxBatch = (int64_t{1} << 31) - (int64_t{1} << 24);
 // least `waitFrac`/64 of the busy lanes wait at a surface, or `maxInner`
 // steps have been taken; then the waiting lanes run the surface update
 // together, and finished lanes write their summary and go idle.
-template <bool Mixed>
+template <bool Mixed, bool GeoInLds>
 __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKernel(
     FwdArgs a, int32_t* __restrict__ next, int32_t maxInner, int32_t waitFrac,
     int32_t quarantine, const int32_t* __restrict__ order) {
   const int lane = static_cast<int>(threadIdx.x) & 63;
   const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t n = a.in.n;
-  // Surfaces and material slabs are read at every step: keep a copy in LDS
-  // when the geometry is small enough (it always is for the benchmark
-  // detectors), else read them from global memory.
-  const DevSurface* geo = a.geo;
-  const DevSlab* slabs = a.slabs;
-#if KF_GEOMETRY_IN_LDS
-  __shared__ DevSurface sGeo[kLdsSurfaces];
-  __shared__ DevSlab sSlabs[kLdsSlabs];
-  if (a.nSurfaces <= kLdsSurfaces && a.nSlabs <= kLdsSlabs) {
+  // Surfaces and material slabs are read at every step: with GeoInLds (the
+  // geometry fits, as it does for the benchmark detectors) the workgroup
+  // keeps a copy in LDS.  The choice is a template parameter so that every
+  // geometry access compiles to an LDS read: a pointer that may point to
+  // either memory needs flat loads, which wait for all outstanding global
+  // loads and stores of the wavefront before their data can be used.
+  __shared__ DevSurface sGeo[GeoInLds ? kLdsSurfaces : 1];
+  __shared__ DevSlab sSlabs[GeoInLds ? kLdsSlabs : 1];
+  if constexpr (GeoInLds) {
     const double* src = reinterpret_cast<const double*>(a.geo);
     double* dst = reinterpret_cast<double*>(sGeo);
     const int nd = a.nSurfaces * static_cast<int>(sizeof(DevSurface) / sizeof(double));
@@ -80,10 +77,9 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
     const int ns = a.nSlabs * static_cast<int>(sizeof(DevSlab) / sizeof(int32_t));
     for (int i = static_cast<int>(threadIdx.x); i < ns; i += blockDim.x) sd_[i] = ss[i];
     __syncthreads();
-    geo = sGeo;
-    slabs = sSlabs;
   }
-#endif
+  const DevSurface* geo = GeoInLds ? sGeo : a.geo;
+  const DevSlab* slabs = GeoInLds ? sSlabs : a.slabs;
   Lane L;
   int phase = kPhaseIdle;
   while (true) {
@@ -332,6 +328,21 @@ struct acts_kf_ctx {
   DevBuf inBuf, outBuf;
 };
 
+namespace {
+
+using ForwardFn = void (*)(FwdArgs, int32_t*, int32_t, int32_t, int32_t, const int32_t*);
+
+bool geometryFitsLds(const acts_kf_ctx* c) {
+  return c->nSurfaces <= kLdsSurfaces && c->nSlabs <= kLdsSlabs;
+}
+
+ForwardFn forwardKernel(bool mixed, bool geoInLds) {
+  if (mixed) return geoInLds ? fitForwardKernel<true, true> : fitForwardKernel<true, false>;
+  return geoInLds ? fitForwardKernel<false, true> : fitForwardKernel<false, false>;
+}
+
+}  // namespace
+
 extern "C" {
 
 void acts_kf_default_options(acts_kf_options* o) {
@@ -540,11 +551,8 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   c->quarantine = std::max(options->sched_quarantine_steps, 0);
   {
     int blocksPerCu = 0, cus = 0;
-    const hipError_t oe =
-        c->mixed ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocksPerCu,
-                                                                fitForwardKernel<true>, kBlock, 0)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocksPerCu,
-                                                                fitForwardKernel<false>, kBlock, 0);
+    const hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &blocksPerCu, forwardKernel(c->mixed, geometryFitsLds(c)), kBlock, 0);
     if (oe != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) !=
             hipSuccess ||
@@ -676,12 +684,9 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   hipLaunchKernelGGL(zeroKernel, dim3(1), dim3(64), 0, stream, queue, 1);
   const int64_t waves = std::min(blocks, c->residentWaves);
   if (timing) KF_HIP(hipEventRecord(c->ev[0], stream));
-  if (c->mixed)
-    hipLaunchKernelGGL(fitForwardKernel<true>, dim3(static_cast<unsigned>(waves)), dim3(kBlock),
-                       0, stream, fa, queue, c->innerSteps, c->waitFrac, c->quarantine, perm);
-  else
-    hipLaunchKernelGGL(fitForwardKernel<false>, dim3(static_cast<unsigned>(waves)), dim3(kBlock),
-                       0, stream, fa, queue, c->innerSteps, c->waitFrac, c->quarantine, perm);
+  hipLaunchKernelGGL(forwardKernel(c->mixed, geometryFitsLds(c)),
+                     dim3(static_cast<unsigned>(waves)), dim3(kBlock), 0, stream, fa, queue,
+                     c->innerSteps, c->waitFrac, c->quarantine, perm);
   if (timing) KF_HIP(hipEventRecord(c->ev[1], stream));
   hipLaunchKernelGGL(smoothKernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream,
                      b, d, sc, perm);

@@ -47,6 +47,7 @@ struct Lane {
   int64_t t;
   int64_t m0;
   int nMeas, nUnique;
+  int mNext;  // measurement after the last one matched (all surfaces distinct)
   SeqAt seq;
   Track T;
   Nav nv;
@@ -106,15 +107,39 @@ __device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a,
     L.status = ACTS_KF_STATUS_PROPAGATION_FAILURE;
     return kPhaseFinished;
   }
-  // unique measurement surfaces (KalmanFitter.hpp:750-755, first one wins)
+  // unique measurement surfaces (KalmanFitter.hpp:750-755, first one wins):
+  // a set of the surface indices below 256 in four 64-bit words; pairwise
+  // comparison only for a track with a measurement on another index
   int nUnique = 0;
+  bool wide = false;
+  uint64_t seen[4] = {0, 0, 0, 0};
   for (int i = 0; i < L.nMeas; ++i) {
-    const int si_ = in.measSurf[L.m0 + i];
+    const int sm = in.measSurf[L.m0 + i];
+    wide = wide || sm < 0 || sm >= 256;
+    const uint64_t bit = 1ull << (sm & 63);
+    const int w = (sm >> 6) & 3;
     bool dup = false;
-    for (int j = 0; j < i; ++j) dup = dup || (in.measSurf[L.m0 + j] == si_);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      dup = dup || (w == k && (seen[k] & bit) != 0);
+      seen[k] |= (w == k) ? bit : 0ull;
+    }
     nUnique += dup ? 0 : 1;
   }
+  if (wide) {
+    nUnique = 0;
+    for (int i = 0; i < L.nMeas; ++i) {
+      const int si_ = in.measSurf[L.m0 + i];
+      bool dup = false;
+      for (int j = 0; j < i; ++j) {
+        const int sj = in.measSurf[L.m0 + j];
+        dup = dup || sj == si_;
+      }
+      nUnique += dup ? 0 : 1;
+    }
+  }
   L.nUnique = nUnique;
+  L.mNext = 0;
 
   Track& T = L.T;
   {
@@ -223,9 +248,20 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
   if (!L.finished) {
     if (L.kfLimit == DBL_MAX) loopProtection<Mixed>(T, a.F, o, L.kfLimit, true);
     const DevSurface& s = geo[reached];
+    // the measurement on this surface: the first one in input order; when
+    // all of the track's measurement surfaces differ and the measurements
+    // come in the order of the surface sequence, it is the one after the
+    // last match
     int mi = -1;
-    for (int i = 0; i < L.nMeas; ++i)
-      if (mi < 0 && in.measSurf[L.m0 + i] == reached) mi = i;
+    if (L.nUnique == L.nMeas && L.mNext < L.nMeas && in.measSurf[L.m0 + L.mNext] == reached) {
+      mi = L.mNext;
+    } else {
+      for (int i = 0; i < L.nMeas; ++i) {
+        const int sm = in.measSurf[L.m0 + i];
+        if (mi < 0 && sm == reached) mi = i;
+      }
+    }
+    if (mi >= 0) L.mNext = mi + 1;
     const bool hasMat = s.mat >= 0;
     const bool sensitive = (s.flags & ACTS_KF_SURFACE_SENSITIVE) != 0;
     const bool preceding = L.nMeasStates > 0;
