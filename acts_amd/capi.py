@@ -21,8 +21,8 @@ ERR_DEVICE = -2
 ERR_OUT_OF_MEMORY = -3
 ERR_UNSUPPORTED = -4
 
-CAT_KALMAN, CAT_PROPAGATOR, CAT_STEPPER, CAT_SURFACE, CAT_FIELD = (
-    0x100, 0x200, 0x300, 0x400, 0x500)
+CAT_KALMAN, CAT_PROPAGATOR, CAT_STEPPER, CAT_SURFACE, CAT_FIELD, CAT_EXTRAPOLATION = (
+    0x100, 0x200, 0x300, 0x400, 0x500, 0x600)
 STATUS_OK = 0
 STATUS_UPDATE_FAILED = CAT_KALMAN | 1
 STATUS_SMOOTH_FAILED = CAT_KALMAN | 2
@@ -35,6 +35,8 @@ STATUS_STEP_INVALID = CAT_STEPPER | 2
 STATUS_STEP_ADJUSTMENT_FAILED = CAT_STEPPER | 3
 STATUS_NOT_ON_SURFACE = CAT_SURFACE | 1
 STATUS_FIELD_OUT_OF_BOUNDS = CAT_FIELD | 1
+STATUS_NO_COMPATIBLE_STATE = CAT_EXTRAPOLATION | 1
+STATUS_REFERENCE_UNREACHABLE = CAT_EXTRAPOLATION | 2
 
 STATE_HAS_MEASUREMENT = 1 << 0
 STATE_HAS_PARAMETERS = 1 << 1
@@ -42,7 +44,9 @@ STATE_IS_OUTLIER = 1 << 2
 STATE_IS_HOLE = 1 << 3
 STATE_HAS_MATERIAL = 1 << 4
 
-SURFACE_PLANE, SURFACE_CYLINDER, SURFACE_DISC = 0, 1, 2
+SURFACE_PLANE, SURFACE_CYLINDER, SURFACE_DISC, SURFACE_LINE = 0, 1, 2, 3
+REFERENCE_NONE, REFERENCE_PERIGEE = 0, 1
+EXTRAPOLATE_FIRST, EXTRAPOLATE_LAST, EXTRAPOLATE_FIRST_OR_LAST = 0, 1, 2
 SURFACE_SENSITIVE = 1
 
 FIELD_CONSTANT, FIELD_GRID_XYZ = 0, 1
@@ -107,6 +111,9 @@ class Options(C.Structure):
         ("abs_pdg", C.c_int32),
         ("mass", C.c_float),
         ("abs_charge", C.c_float),
+        ("reference_surface", C.c_int32),
+        ("reference_strategy", C.c_int32),
+        ("reference_position", C.c_double * 3),
         ("length_sorted", C.c_int32),
         ("sched_inner_steps", C.c_int32),
         ("sched_wait_64ths", C.c_int32),
@@ -140,7 +147,8 @@ BATCH_OUT_FIELDS = [
     "status", "n_states", "chi2", "ndf", "n_measurements", "n_holes",
     "n_outliers", "n_steps", "n_step_trials", "state_surface", "state_flags",
     "state_path", "state_chi2", "predicted", "predicted_cov", "filtered",
-    "filtered_cov", "smoothed", "smoothed_cov", "jacobian",
+    "filtered_cov", "smoothed", "smoothed_cov", "jacobian", "reference_params",
+    "reference_cov",
 ]
 
 

@@ -12,7 +12,7 @@
 #include <vector>
 
 #include "kf_field_host.hpp"
-#include "kf_kernel.hpp"
+#include "kf_reference.hpp"
 
 using namespace kf;
 
@@ -128,6 +128,35 @@ __global__ __launch_bounds__(kBlock) void smoothKernel(DevBatch in, DevOut out,
   const bool valid = g < in.n;
   const int64_t t = !valid ? 0 : (perm ? perm[g] : g);
   smoothTrack(t, valid, in, out, sc);
+}
+
+// Extrapolation of every successfully smoothed track to the reference
+// surface, one lane per track (KalmanFitter.hpp:959-971).  A failure fails
+// the fit: the status becomes the extrapolation's error and the summary is
+// cleared, as for any failed fit.
+template <bool Mixed>
+__global__ __launch_bounds__(kBlock) void referenceKernel(DevOut out, DevScratch sc,
+                                                          const DevSurface* __restrict__ geo,
+                                                          DevField F, DevOpts o, LineRef L,
+                                                          int32_t strategy, int64_t n) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= n || sc.track[t] != 0) return;
+  double p[6], C[36];
+  const int st = extrapolateTrack<Mixed>(t, sc, sc.ref, geo, F, o, L, strategy, p, C);
+  if (st) {
+    if (out.status) out.status[t] = st;
+    if (out.nStates) out.nStates[t] = 0;
+    if (out.chi2) out.chi2[t] = 0.f;
+    if (out.ndf) out.ndf[t] = 0;
+    if (out.nMeas) out.nMeas[t] = 0;
+    if (out.nHoles) out.nHoles[t] = 0;
+    if (out.nOutliers) out.nOutliers[t] = 0;
+    return;
+  }
+  if (out.referenceParams)
+    for (int i = 0; i < 6; ++i) out.referenceParams[6 * t + i] = p[i];
+  if (out.referenceCov)
+    for (int i = 0; i < 36; ++i) out.referenceCov[36 * t + i] = C[i];
 }
 
 __global__ void zeroKernel(int32_t* p, int32_t count) {
@@ -277,6 +306,7 @@ struct GraphKey {
   int32_t pad;
   void* stream;
   void* scratch[5];
+  void* ref;
 };
 
 PhysConst makePhysConst() {
@@ -324,6 +354,13 @@ struct acts_kf_ctx {
   int32_t innerSteps = 64;
   int32_t waitFrac = 64;
   int32_t quarantine = 0;
+  // reference surface of the fitted parameters, and the default propagator
+  // and stepper options its extrapolation runs with
+  int32_t refSurface = ACTS_KF_REFERENCE_NONE;
+  int32_t refStrategy = ACTS_KF_EXTRAPOLATE_FIRST_OR_LAST;
+  LineRef refLine{};
+  DevOpts extOpts{};
+  DevBuf refBuf;
   // staging for the host-buffer entry point
   DevBuf inBuf, outBuf;
 };
@@ -422,6 +459,12 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   if (options->precision != ACTS_KF_PRECISION_FP64 &&
       options->precision != ACTS_KF_PRECISION_MIXED)
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: precision mode not supported");
+  if (options->reference_surface != ACTS_KF_REFERENCE_NONE &&
+      options->reference_surface != ACTS_KF_REFERENCE_PERIGEE)
+    return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: reference surface not supported");
+  if (options->reference_strategy < ACTS_KF_EXTRAPOLATE_FIRST ||
+      options->reference_strategy > ACTS_KF_EXTRAPOLATE_FIRST_OR_LAST)
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: bad extrapolation strategy");
   for (int32_t i = 0; i < nSurfaces; ++i) {
     const acts_kf_surface& s = surfaces[i];
     if (s.type < 0 || s.type > 2)
@@ -537,6 +580,31 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   o.mass = options->mass;
   o.absQ = options->abs_charge;
   c->pc = makePhysConst();
+  c->refSurface = options->reference_surface;
+  c->refStrategy = options->reference_strategy;
+  for (int i = 0; i < 3; ++i) c->refLine.c[i] = options->reference_position[i];
+  // extrapolateTrackToReferenceSurface runs with default-constructed
+  // propagator options (KalmanFitter.hpp:960-961): the defaults of
+  // PropagatorOptions.hpp:26-52 and StepperOptions.hpp:36-49, the fit's
+  // particle hypothesis
+  {
+    acts_kf_options d;
+    acts_kf_default_options(&d);
+    DevOpts& e = c->extOpts;
+    e = o;
+    e.stepTolerance = d.step_tolerance;
+    e.stepSizeCutOff = d.step_size_cutoff;
+    e.initialStepSize = d.initial_step_size;
+    e.maxStepSize = d.max_step_size;
+    e.pathLimit = d.path_limit;
+    e.loopFraction = d.loop_fraction;
+    e.surfaceTolerance = d.surface_tolerance;
+    e.nearLimit = d.near_limit;
+    e.maxTrials = d.max_runge_kutta_step_trials;
+    e.maxSteps = d.max_steps;
+    e.maxTargetSkipping = d.max_target_skipping;
+    e.loopProtection = d.loop_protection;
+  }
   c->lengthSorted = options->length_sorted != 0;
   c->timing = options->kernel_timing != 0;
   c->useGraph = options->use_graph != 0;
@@ -576,7 +644,10 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   const int32_t states = std::max(maxLen, 1);
   const size_t needD = sizeof(double) * static_cast<size_t>(states) * kScratchComps * n;
   const size_t needI = sizeof(int32_t) * static_cast<size_t>(states) * kIScratchComps * n;
-  const size_t needT = sizeof(int32_t) * 2 * n;
+  const size_t needT = sizeof(int32_t) * 4 * n;
+  const bool withRef = c->refSurface != ACTS_KF_REFERENCE_NONE;
+  if (withRef && c->refBuf.ensure(sizeof(double) * kRefComps * static_cast<size_t>(n)))
+    return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "reference scratch allocation failed");
   if (c->scratchD.ensure(needD) || c->scratchI.ensure(needI) || c->scratchT.ensure(needT) ||
       c->queueBuf.ensure(sizeof(int32_t)))
     return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "scratch allocation failed");
@@ -599,6 +670,7 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
     key.scratch[2] = c->scratchT.p;
     key.scratch[3] = c->permBuf.p;
     key.scratch[4] = c->histBuf.p;
+    key.ref = c->refBuf.p;
     if (c->graphExec && std::memcmp(&key, &c->graphKey, sizeof(key)) == 0) {
       KF_HIP(hipGraphLaunch(c->graphExec, stream));
       return ACTS_KF_OK;
@@ -642,7 +714,10 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   d.smoothed = out->smoothed;
   d.smoothedCov = out->smoothed_cov;
   d.jacobian = out->jacobian;
+  d.referenceParams = out->reference_params;
+  d.referenceCov = out->reference_cov;
   DevScratch sc;
+  sc.ref = withRef ? static_cast<double*>(c->refBuf.p) : nullptr;
   sc.d = static_cast<double*>(c->scratchD.p);
   sc.i = static_cast<int32_t*>(c->scratchI.p);
   sc.track = static_cast<int32_t*>(c->scratchT.p);
@@ -690,6 +765,16 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   if (timing) KF_HIP(hipEventRecord(c->ev[1], stream));
   hipLaunchKernelGGL(smoothKernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream,
                      b, d, sc, perm);
+  if (withRef) {
+    if (c->mixed)
+      hipLaunchKernelGGL(referenceKernel<true>, dim3(static_cast<unsigned>(blocks)), dim3(kBlock),
+                         0, stream, d, sc, c->geo, c->field, c->extOpts, c->refLine,
+                         c->refStrategy, n);
+    else
+      hipLaunchKernelGGL(referenceKernel<false>, dim3(static_cast<unsigned>(blocks)),
+                         dim3(kBlock), 0, stream, d, sc, c->geo, c->field, c->extOpts,
+                         c->refLine, c->refStrategy, n);
+  }
   const hipError_t launchErr = hipGetLastError();
   if (c->useGraph) {
     hipGraph_t graph = nullptr;
@@ -815,7 +900,9 @@ int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
       {out->predicted_cov, sizeof(double) * 36 * nSeq},
       {out->filtered, sizeof(double) * 6 * nSeq},  {out->filtered_cov, sizeof(double) * 36 * nSeq},
       {out->smoothed, sizeof(double) * 6 * nSeq},  {out->smoothed_cov, sizeof(double) * 36 * nSeq},
-      {out->jacobian, sizeof(double) * 36 * nSeq}};
+      {out->jacobian, sizeof(double) * 36 * nSeq},
+      {out->reference_params, sizeof(double) * 6 * n},
+      {out->reference_cov, sizeof(double) * 36 * n}};
   constexpr int nOut = sizeof(outs) / sizeof(outs[0]);
   size_t totalO = 0;
   for (const OSec& q : outs)
@@ -849,6 +936,8 @@ int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
   dout.smoothed = static_cast<double*>(optr[17]);
   dout.smoothed_cov = static_cast<double*>(optr[18]);
   dout.jacobian = static_cast<double*>(optr[19]);
+  dout.reference_params = static_cast<double*>(optr[20]);
+  dout.reference_cov = static_cast<double*>(optr[21]);
   int r = launchFit(c, &din, &dout, s, maxLen);
   if (r) return r;
   for (int i = 0; i < nOut; ++i)
@@ -885,6 +974,7 @@ void acts_kf_destroy(acts_kf_ctx* c) {
   c->scratchD.release();
   c->scratchI.release();
   c->scratchT.release();
+  c->refBuf.release();
   c->lenBuf.release();
   c->permBuf.release();
   c->histBuf.release();

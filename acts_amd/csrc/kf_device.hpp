@@ -41,11 +41,17 @@ struct DevBatch {
 constexpr int kPred = 0, kPCov = 6, kFilt = 27, kFCov = 33, kJac = 54,
               kPath = 90, kChi2 = 91, kScratchComps = 92;
 constexpr int kISurf = 0, kIFlags = 1, kIScratchComps = 2;
+// per track with a reference surface: smoothed parameters (6) and packed
+// covariance (21) of the first, then of the last measurement state
+constexpr int kRefComps = 54;
 
 struct DevScratch {
   double* d;
   int32_t* i;
-  int32_t* track;  // per track: [0] status, [1] tip state index
+  int32_t* track;  // per track: [0] status, [1] tip state index, [2] [3]
+                   // first and last measurement (not outlier) state index
+  double* ref;     // per track with a reference surface: the smoothed
+                   // first and last measurement states (null: none)
   int64_t stride;  // >= n, multiple of 64
   int32_t maxStates;
 };
@@ -350,8 +356,12 @@ struct RkStage {
 };
 
 // EigenStepper::step (EigenStepper.ipp:169-388): the adaptive RKN4 trials
-// until the error estimate is accepted.  Returns 0 or a status.
-__device__ inline int rkTrials(const Track& T, const DevField& F, const DevOpts& o, RkStage& S) {
+// until the error estimate is accepted.  Returns 0 or a status.  `propDir`
+// is the propagation direction (+1 or -1): the step constraints are path
+// lengths along it, and the step is their value times the direction
+// (EigenStepper.ipp:270).
+__device__ inline int rkTrials(const Track& T, const DevField& F, const DevOpts& o, RkStage& S,
+                               double propDir = 1.0) {
   S.pos = T.pos;
   S.dir = T.dir;
   S.qop = T.qop;
@@ -363,7 +373,7 @@ __device__ inline int rkTrials(const Track& T, const DevField& F, const DevOpts&
   S.k1 = mk(qop * c1.x, qop * c1.y, qop * c1.z);
   const V3& k1 = S.k1;
 
-  double h = stepValue(T);
+  double h = stepValue(T) * propDir;
   S.initialH = h;
   double h2, half_h, err;
   int nTrials = 0;
@@ -576,9 +586,10 @@ __device__ inline void rkApply(Track& T, const DevOpts& o, const RkStage& S) {
 
 // EigenStepper::step (EigenStepper.ipp:169-388) + transportMatrix
 // (EigenStepperDefaultExtension.hpp:137-228), propagation direction +1.
-__device__ inline int rkStep(Track& T, const DevField& F, const DevOpts& o) {
+__device__ inline int rkStep(Track& T, const DevField& F, const DevOpts& o,
+                             double propDir = 1.0) {
   RkStage S;
-  if (const int st = rkTrials(T, F, o, S)) return st;
+  if (const int st = rkTrials(T, F, o, S, propDir)) return st;
   double p;
   const double dtds = rkTime(T, o, S, p);
   rkTransport(T, o, S, p, dtds);

@@ -26,6 +26,8 @@ struct DevOut {
   double* smoothed;
   double* smoothedCov;
   double* jacobian;
+  double* referenceParams;
+  double* referenceCov;
 };
 
 // ---------------------------------------------------------------------------

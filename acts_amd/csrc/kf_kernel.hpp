@@ -53,6 +53,7 @@ struct Lane {
   Nav nv;
   double propLimit, kfLimit;
   int nMeasStates, nStates, tip;
+  int firstMeas, lastMeas;  // states updated with a measurement (not outliers)
   bool finished;
   int status;
   uint32_t iter;
@@ -91,6 +92,8 @@ __device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a,
   L.nMeasStates = 0;
   L.nStates = 0;
   L.tip = -1;
+  L.firstMeas = -1;
+  L.lastMeas = -1;
   L.iter = 0;
   L.reached = -1;
   L.finished = false;
@@ -329,6 +332,8 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
               for (int i = 0; i < 6; ++i) T.jp[i] = b2f[i];
               directionB2F(T.dir, T.jd);
               ++L.nMeasStates;
+              if (L.firstMeas < 0) L.firstMeas = k;
+              L.lastMeas = k;
             }
           } else {
             flags |= ACTS_KF_STATE_IS_OUTLIER;
@@ -370,6 +375,8 @@ __device__ inline void laneFinish(Lane& L, const FwdArgs& a) {
   if (!status && L.nMeasStates == 0) status = ACTS_KF_STATUS_NO_MEASUREMENT_FOUND;
   a.sc.track[t] = status;
   a.sc.track[a.sc.stride + t] = L.tip;
+  a.sc.track[2 * a.sc.stride + t] = L.firstMeas;
+  a.sc.track[3 * a.sc.stride + t] = L.lastMeas;
 }
 
 // Expand a packed symmetric matrix into a row-major 6x6 output.
@@ -388,6 +395,10 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
                                    const DevOut& out, const DevScratch& sc) {
   int status = valid ? sc.track[t] : 0;
   const int tip = (!valid || status) ? -1 : sc.track[sc.stride + t];
+  // with a reference surface: keep the smoothed first and last measurement
+  // states for the extrapolation
+  const int firstMeas = (valid && sc.ref) ? sc.track[2 * sc.stride + t] : -2;
+  const int lastMeas = (valid && sc.ref) ? sc.track[3 * sc.stride + t] : -2;
   int nHoles = 0, nOut = 0, nMeasT = 0, ndf = 0;
   float chi2sum = 0.f;
   // All lanes walk the same state index k (from the wavefront's largest tip
@@ -423,6 +434,17 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
         status = r;
         active = false;
         continue;
+      }
+    }
+    if (k == firstMeas || k == lastMeas) {
+      double* r = sc.ref + kRefComps * t + (k == firstMeas ? 0 : 27);
+#pragma unroll
+      for (int i = 0; i < 6; ++i) r[i] = xs[i];
+#pragma unroll
+      for (int i = 0; i < 21; ++i) r[6 + i] = Cs[i];
+      if (k == firstMeas && k == lastMeas) {
+#pragma unroll
+        for (int i = 0; i < 27; ++i) r[27 + i] = r[i];
       }
     }
     const uint32_t flags = static_cast<uint32_t>(si(sc, k, kIFlags, t));
@@ -462,6 +484,8 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
     for (int i = 0; i < 21; ++i) CsN[i] = Cs[i];
   }
   if (!valid) return;
+  // the status after smoothing, for the extrapolation kernel
+  sc.track[t] = status;
   if (out.status) out.status[t] = status;
   if (out.nStates) out.nStates[t] = status ? 0 : tip + 1;
   if (out.chi2) out.chi2[t] = status ? 0.f : chi2sum;

@@ -81,7 +81,8 @@ __device__ inline float stepScalingF(float tol, float err) {
   return fminf(fmaxf(x, 0.25f), 4.0f);
 }
 
-__device__ inline int rkStepMixed(Track& T, const DevField& F, const DevOpts& o) {
+__device__ inline int rkStepMixed(Track& T, const DevField& F, const DevOpts& o,
+                                  double propDir = 1.0) {
   const F3 pos = tof(T.pos);
   const F3 dir = tof(T.dir);
   const float qop = static_cast<float>(T.qop);
@@ -91,7 +92,7 @@ __device__ inline int rkStepMixed(Track& T, const DevField& F, const DevOpts& o)
   const F3 c1 = crossf(dir, B1);
   const F3 k1 = mkf(qop * c1.x, qop * c1.y, qop * c1.z);
 
-  const double initialH = stepValue(T);
+  const double initialH = stepValue(T) * propDir;
   // the step length actually taken, kept in double: a step the navigator
   // sized to reach a surface must land on it to double precision
   double hD = initialH;

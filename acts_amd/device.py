@@ -10,7 +10,7 @@ import torch
 
 from . import capi
 from .events import Events
-from .fitter import STATE_ARRAYS, TRACK_ARRAYS
+from .fitter import STATE_ARRAYS, TRACK_ARRAYS, TRACK_VECTOR_ARRAYS
 
 _TORCH_DT = {np.float64: torch.float64, np.float32: torch.float32,
              np.int32: torch.int32, np.int64: torch.int64, np.uint32: torch.int32}
@@ -52,6 +52,10 @@ class DeviceResult:
         for name, dt in TRACK_ARRAYS.items():
             self.t[name] = torch.zeros(n_tracks, dtype=_TORCH_DT[dt], device=device)
         for name in outputs:
+            if name in TRACK_VECTOR_ARRAYS:
+                self.t[name] = torch.zeros((n_tracks, TRACK_VECTOR_ARRAYS[name]),
+                                           dtype=torch.float64, device=device)
+                continue
             dt, w = STATE_ARRAYS[name]
             shape = (n_slots,) if w == 1 else (n_slots, w)
             self.t[name] = torch.zeros(shape, dtype=_TORCH_DT[dt], device=device)

@@ -42,6 +42,11 @@ class KalmanFitterOptions:
     abs_pdg: int = 13
     mass: float = MUON_MASS
     abs_charge: float = 1.0
+    # fitted parameters on a reference surface (KalmanFitterOptions::
+    # referenceSurface): capi.REFERENCE_*, capi.EXTRAPOLATE_*, perigee point
+    reference_surface: int = capi.REFERENCE_NONE
+    reference_strategy: int = capi.EXTRAPOLATE_FIRST_OR_LAST
+    reference_position: tuple = (0.0, 0.0, 0.0)
     # scheduling only; results do not depend on these
     length_sorted: bool = True
     sched_inner_steps: int = 0              # 0 = library default
@@ -54,6 +59,8 @@ class KalmanFitterOptions:
         o = capi.Options()
         for f in fields(self):
             v = getattr(self, f.name)
+            if isinstance(v, (tuple, list)):
+                v = (C.c_double * len(v))(*v)
             setattr(o, f.name, int(v) if isinstance(v, bool) else v)
         return o
 
@@ -77,6 +84,9 @@ TRACK_ARRAYS = {
     "n_measurements": np.int32, "n_holes": np.int32, "n_outliers": np.int32,
     "n_steps": np.int32, "n_step_trials": np.int32,
 }
+# per-track parameter arrays, filled when the options ask for a reference
+# surface: name -> width
+TRACK_VECTOR_ARRAYS = {"reference_params": 6, "reference_cov": 36}
 DEFAULT_STATE_OUTPUTS = ("state_surface", "state_flags", "state_path", "state_chi2",
                          "smoothed", "smoothed_cov")
 ALL_STATE_OUTPUTS = tuple(STATE_ARRAYS)
@@ -90,6 +100,8 @@ class FitResult:
         for name, dt in TRACK_ARRAYS.items():
             setattr(self, name, np.zeros(n_tracks, dt))
         self.outputs = tuple(outputs)
+        for name, w in TRACK_VECTOR_ARRAYS.items():
+            setattr(self, name, np.zeros((n_tracks, w)) if name in self.outputs else None)
         for name in STATE_ARRAYS:
             dt, w = STATE_ARRAYS[name]
             if name in self.outputs:

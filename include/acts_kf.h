@@ -50,6 +50,7 @@ extern "C" {
 #define ACTS_KF_CAT_STEPPER 0x300    /* EigenStepperError.hpp:18-26 */
 #define ACTS_KF_CAT_SURFACE 0x400    /* SurfaceError.hpp:18-22 */
 #define ACTS_KF_CAT_FIELD 0x500      /* MagneticFieldError.hpp:18-22 */
+#define ACTS_KF_CAT_EXTRAPOLATION 0x600 /* TrackExtrapolationError, TrackHelpers.hpp:48-53 */
 
 #define ACTS_KF_STATUS_UPDATE_FAILED (ACTS_KF_CAT_KALMAN | 1)
 #define ACTS_KF_STATUS_SMOOTH_FAILED (ACTS_KF_CAT_KALMAN | 2)
@@ -62,6 +63,8 @@ extern "C" {
 #define ACTS_KF_STATUS_STEP_ADJUSTMENT_FAILED (ACTS_KF_CAT_STEPPER | 3)
 #define ACTS_KF_STATUS_NOT_ON_SURFACE (ACTS_KF_CAT_SURFACE | 1)
 #define ACTS_KF_STATUS_FIELD_OUT_OF_BOUNDS (ACTS_KF_CAT_FIELD | 1)
+#define ACTS_KF_STATUS_NO_COMPATIBLE_STATE (ACTS_KF_CAT_EXTRAPOLATION | 1)
+#define ACTS_KF_STATUS_REFERENCE_UNREACHABLE (ACTS_KF_CAT_EXTRAPOLATION | 2)
 
 /* ---- track-state type flags (TrackStateType.hpp:30-49, bit positions) -- */
 #define ACTS_KF_STATE_HAS_MEASUREMENT (1u << 0)
@@ -74,6 +77,21 @@ extern "C" {
 #define ACTS_KF_SURFACE_PLANE 0    /* PlaneSurface.cpp */
 #define ACTS_KF_SURFACE_CYLINDER 1 /* CylinderSurface.cpp */
 #define ACTS_KF_SURFACE_DISC 2     /* DiscSurface.cpp */
+/* LineSurface.cpp; only as the reference surface (PerigeeSurface), not in
+ * the geometry table */
+#define ACTS_KF_SURFACE_LINE 3
+
+/* Reference surface of the fitted track parameters
+ * (KalmanFitterOptions::referenceSurface, KalmanFitter.hpp:159-163,959-971):
+ * none, or a PerigeeSurface (a line along the global z axis) through
+ * acts_kf_options.reference_position, as the Examples set it up
+ * (TrackFittingAlgorithm.cpp:89-90).                                        */
+#define ACTS_KF_REFERENCE_NONE 0
+#define ACTS_KF_REFERENCE_PERIGEE 1
+/* TrackExtrapolationStrategy (TrackHelpers.hpp:36-44) */
+#define ACTS_KF_EXTRAPOLATE_FIRST 0
+#define ACTS_KF_EXTRAPOLATE_LAST 1
+#define ACTS_KF_EXTRAPOLATE_FIRST_OR_LAST 2
 
 #define ACTS_KF_SURFACE_SENSITIVE 1u /* Surface::isSensitive() */
 
@@ -153,6 +171,15 @@ typedef struct acts_kf_options {
   int32_t abs_pdg;             /* 13 = muon */
   float mass;                  /* GeV */
   float abs_charge;            /* e */
+  /* extrapolation of the smoothed track to a reference surface after the
+   * fit (extrapolateTrackToReferenceSurface, TrackHelpers.hpp:339-383):
+   * ACTS_KF_REFERENCE_*, ACTS_KF_EXTRAPOLATE_* (default first-or-last, as
+   * KalmanFitterOptions; the Examples use first), and the perigee point.
+   * The propagation uses the reference's default propagator and stepper
+   * options, not the ones above (KalmanFitter.hpp:960-961).                 */
+  int32_t reference_surface;
+  int32_t reference_strategy;
+  double reference_position[3];
   /* scheduling (no reference counterpart; results do not depend on it):
    * length_sorted     1 = the smoother visits tracks in order of decreasing
    *                   sequence length (default), 0 = input order;
@@ -224,6 +251,10 @@ typedef struct acts_kf_batch_out {
   double* smoothed;         /* [slots][6] */
   double* smoothed_cov;     /* [slots][36] */
   double* jacobian;         /* [slots][36] transport jacobian from the previous state */
+  /* per track, with a reference surface: the track parameters and
+   * covariance on it (TrackProxy::parameters()/covariance())              */
+  double* reference_params; /* [n][6] */
+  double* reference_cov;    /* [n][36] */
 } acts_kf_batch_out;
 
 typedef struct acts_kf_ctx acts_kf_ctx;

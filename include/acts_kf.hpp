@@ -56,6 +56,11 @@ struct FitResult {
   float chi2 = 0;
   int32_t ndf = 0, nMeasurements = 0, nHoles = 0, nOutliers = 0;
   std::vector<TrackState> states;
+  // with a reference surface in the options: the track parameters and
+  // covariance on it (TrackProxy::parameters()/covariance())
+  bool hasReference = false;
+  std::array<double, 6> referenceParameters{};
+  std::array<double, 36> referenceCovariance{};
   bool ok() const { return status == ACTS_KF_STATUS_OK; }
   std::string message() const { return acts_kf_status_string(status); }
 };
@@ -64,7 +69,8 @@ class KalmanFitter {
  public:
   KalmanFitter(const std::vector<acts_kf_surface>& surfaces,
                const std::vector<acts_kf_material_slab>& materials,
-               const acts_kf_field& field, const acts_kf_options& options, int device = 0) {
+               const acts_kf_field& field, const acts_kf_options& options, int device = 0)
+      : withReference_(options.reference_surface != ACTS_KF_REFERENCE_NONE) {
     const int rc = acts_kf_create(surfaces.data(), static_cast<int32_t>(surfaces.size()),
                                   materials.data(), static_cast<int32_t>(materials.size()),
                                   &field, &options, device, &ctx_);
@@ -119,6 +125,8 @@ class KalmanFitter {
     std::vector<int32_t> stateSurface(slots);
     std::vector<uint32_t> stateFlags(slots);
     std::vector<double> statePath(slots), smoothed(6 * slots), smoothedCov(36 * slots);
+    std::vector<double> refParams(withReference_ ? 6 * n : 0),
+        refCov(withReference_ ? 36 * n : 0);
     acts_kf_batch_in in{};
     in.n_tracks = n;
     in.start_params = params_.data();
@@ -145,6 +153,10 @@ class KalmanFitter {
     out.state_chi2 = stateChi2.data();
     out.smoothed = smoothed.data();
     out.smoothed_cov = smoothedCov.data();
+    if (withReference_) {
+      out.reference_params = refParams.data();
+      out.reference_cov = refCov.data();
+    }
     check(acts_kf_fit_batch(ctx_, &in, &out, nullptr), "acts_kf_fit_batch");
     check(acts_kf_sync(ctx_), "acts_kf_sync");
     for (int64_t t = 0; t < n; ++t) {
@@ -156,6 +168,11 @@ class KalmanFitter {
       r.nHoles = nHoles[t];
       r.nOutliers = nOut[t];
       if (!r.ok()) continue;
+      if (withReference_) {
+        r.hasReference = true;
+        for (int i = 0; i < 6; ++i) r.referenceParameters[i] = refParams[6 * t + i];
+        for (int i = 0; i < 36; ++i) r.referenceCovariance[i] = refCov[36 * t + i];
+      }
       r.states.resize(static_cast<size_t>(nStates[t]));
       for (int32_t k = 0; k < nStates[t]; ++k) {
         const int64_t s = seqOffsets_[t] + k;
@@ -202,6 +219,7 @@ class KalmanFitter {
   }
 
   acts_kf_ctx* ctx_ = nullptr;
+  bool withReference_ = false;
   std::vector<double> params_, cov_, plane_;
   std::vector<int64_t> seqOffsets_{0};
   std::vector<int32_t> seqSurfaces_;

@@ -203,6 +203,21 @@ struct Quadratic {
 MultiIntersection Surface::intersect(const V3& pos, const V3& dir,
                                      double tol) const {
   MultiIntersection mi;
+  if (type == ACTS_KF_SURFACE_LINE) {
+    // LineSurface.cpp:136-186: closest approach of the straight line
+    // (pos, dir) to the surface's axis; no bounds (PerigeeSurface)
+    mi.size = 1;
+    const V3 mb = v3(pl.t[0], pl.t[1], pl.t[2]);
+    const V3 eb = pl.col(2);
+    const V3 mab = v3(mb[0] - pos[0], mb[1] - pos[1], mb[2] - pos[2]);
+    const double eaTeb = dot(dir, eb);
+    const double denom = 1 - eaTeb * eaTeb;
+    if (std::abs(denom) < std::abs(tol)) return mi;  // invalid
+    const double u = (dot(mab, dir) - dot(mab, eb) * eaTeb) / denom;
+    mi.is[0].path = u;
+    mi.is[0].status = std::abs(u) > std::abs(tol) ? IStatus::reachable : IStatus::onSurface;
+    return mi;
+  }
   if (type == ACTS_KF_SURFACE_CYLINDER) {
     // CylinderSurface.cpp:198-283 (boundary tolerance infinite)
     const double Rr = b0;
@@ -319,6 +334,47 @@ Mat<3, 3> Surface::referenceFrame(const V3& pos) const {
   return f;
 }
 
+// LineSurface.cpp:101-114; other types: referenceFrame(pos)
+Mat<3, 3> Surface::referenceFrame(const V3& pos, const V3& dir) const {
+  if (type != ACTS_KF_SURFACE_LINE) return referenceFrame(pos);
+  const V3 unitZ0 = pl.col(2);
+  const V3 unitD0 = normalized(cross(unitZ0, dir));
+  const V3 unitDistance = cross(unitD0, unitZ0);
+  Mat<3, 3> f;
+  for (int i = 0; i < 3; ++i) {
+    f.a[i][0] = unitD0[i];
+    f.a[i][1] = unitZ0[i];
+    f.a[i][2] = unitDistance[i];
+  }
+  return f;
+}
+
+// LineSurface.cpp:68-96: the frame's inverse applied to pos - center.  The
+// frame is orthonormal, so its inverse is its transpose (Eigen inverts the
+// 3x3 by cofactors; the two agree up to the last bits).
+bool Surface::globalToLocal(const V3& pos, const V3& dir, double& l0, double& l1,
+                            double tol) const {
+  if (type != ACTS_KF_SURFACE_LINE) return globalToLocal(pos, l0, l1, tol);
+  const Mat<3, 3> f = referenceFrame(pos, dir);
+  const V3 d = v3(pos[0] - pl.t[0], pos[1] - pl.t[1], pos[2] - pl.t[2]);
+  double loc[3];
+  for (int j = 0; j < 3; ++j) loc[j] = (f.a[0][j] * d[0] + f.a[1][j] * d[1]) + f.a[2][j] * d[2];
+  if (std::abs(loc[2]) > std::abs(tol)) return false;
+  l0 = loc[0];
+  l1 = loc[1];
+  return true;
+}
+
+// LineSurface.cpp:57-66
+V3 Surface::localToGlobal(double l0, double l1, const V3& dir) const {
+  if (type != ACTS_KF_SURFACE_LINE) return localToGlobal(l0, l1);
+  const V3 unitZ0 = pl.col(2);
+  const V3 radiusAxis = normalized(cross(unitZ0, dir));
+  const V3 locZ = pl.toGlobal(v3(0., 0., l1));
+  return v3(locZ[0] + l0 * radiusAxis[0], locZ[1] + l0 * radiusAxis[1],
+            locZ[2] + l0 * radiusAxis[2]);
+}
+
 double Surface::pathCorrection(const V3& pos, const V3& dir) const {
   if (type == ACTS_KF_SURFACE_CYLINDER) {
     // CylinderSurface.cpp:169-175
@@ -347,7 +403,7 @@ Mat<8, 6> Surface::boundToFreeJacobian(const V3& pos, const V3& dir) const {
     }
   } else {
     // Surface.cpp:249-268
-    Mat<3, 3> f = referenceFrame(pos);
+    Mat<3, 3> f = referenceFrame(pos, dir);
     for (int i = 0; i < 3; ++i) {
       J.a[i][0] = f.a[i][0];
       J.a[i][1] = f.a[i][1];
@@ -361,6 +417,27 @@ Mat<8, 6> Surface::boundToFreeJacobian(const V3& pos, const V3& dir) const {
   J.a[6][2] = 0;
   J.a[6][3] = -sinTheta;
   J.a[7][4] = 1;
+  if (type == ACTS_KF_SURFACE_LINE) {
+    // LineSurface.cpp:188-225: the position moves with the direction
+    const Mat<3, 3> f = referenceFrame(pos, dir);
+    double l0 = 0, l1 = 0;
+    globalToLocal(pos, dir, l0, l1, std::numeric_limits<double>::max());
+    const V3 fx = v3(f.a[0][0], f.a[1][0], f.a[2][0]);
+    const V3 fy = v3(f.a[0][1], f.a[1][1], f.a[2][1]);
+    const V3 fz = v3(f.a[0][2], f.a[1][2], f.a[2][2]);
+    const double ipdn = 1. / dot(dir, fz);
+    V3 dDPhiY = cross(fy, v3(J.a[4][2], J.a[5][2], J.a[6][2]));
+    V3 dDThetaY = cross(fy, v3(J.a[4][3], J.a[5][3], J.a[6][3]));
+    const double sp = dot(fx, dDPhiY), st = dot(fx, dDThetaY);
+    for (int i = 0; i < 3; ++i) {
+      dDPhiY[i] = dDPhiY[i] - fx[i] * sp;
+      dDThetaY[i] = dDThetaY[i] - fx[i] * st;
+    }
+    for (int i = 0; i < 3; ++i) {
+      J.a[i][2] = dDPhiY[i] * l0 * ipdn;
+      J.a[i][3] = dDThetaY[i] * l0 * ipdn;
+    }
+  }
   return J;
 }
 
@@ -382,7 +459,7 @@ Mat<6, 8> Surface::freeToBoundJacobian(const V3& pos, const V3& dir) const {
     }
   } else {
     // Surface.cpp:270-290
-    Mat<3, 3> f = referenceFrame(pos);
+    Mat<3, 3> f = referenceFrame(pos, dir);
     for (int k = 0; k < 3; ++k) {
       J.a[0][k] = f.a[k][0];
       J.a[1][k] = f.a[k][1];
@@ -399,8 +476,21 @@ Mat<6, 8> Surface::freeToBoundJacobian(const V3& pos, const V3& dir) const {
   return J;
 }
 
-// Surface.cpp:292-307
+// Surface.cpp:292-307; LineSurface.cpp:227-250
 Mat<1, 8> Surface::freeToPathDerivative(const V3& pos, const V3& dir) const {
+  if (type == ACTS_KF_SURFACE_LINE) {
+    const V3 pc = v3(pos[0] - pl.t[0], pos[1] - pl.t[1], pos[2] - pl.t[2]);
+    const V3 z = pl.col(2);
+    const double pz = dot(pc, z);
+    const double dz = dot(z, dir);
+    const double norm = 1 / (1 - dz * dz);
+    Mat<1, 8> s = Mat<1, 8>::zero();
+    for (int i = 0; i < 3; ++i) {
+      s.a[0][i] = norm * (dz * z[i] - dir[i]);
+      s.a[0][4 + i] = norm * (pz * z[i] - pc[i]);
+    }
+    return s;
+  }
   Mat<3, 3> f = referenceFrame(pos);
   V3 z = v3(f.a[0][2], f.a[1][2], f.a[2][2]);
   const double dz = dot(z, dir);
@@ -414,7 +504,7 @@ Vec8 boundToFree(const Surface& s, const Vec6& b) {
   const double sinTheta = std::sin(b.a[3][0]);
   V3 dir = v3(std::cos(b.a[2][0]) * sinTheta, std::sin(b.a[2][0]) * sinTheta,
               std::cos(b.a[3][0]));
-  V3 pos = s.localToGlobal(b.a[0][0], b.a[1][0]);
+  V3 pos = s.localToGlobal(b.a[0][0], b.a[1][0], dir);
   Vec8 f;
   f.a[0][0] = pos[0];
   f.a[1][0] = pos[1];
@@ -432,7 +522,7 @@ int freeToBound(const Vec8& f, const Surface& s, Vec6& out) {
   V3 pos = v3(f.a[0][0], f.a[1][0], f.a[2][0]);
   V3 dir = v3(f.a[4][0], f.a[5][0], f.a[6][0]);
   double l0, l1;
-  if (!s.globalToLocal(pos, l0, l1)) return ACTS_KF_STATUS_NOT_ON_SURFACE;
+  if (!s.globalToLocal(pos, dir, l0, l1)) return ACTS_KF_STATUS_NOT_ON_SURFACE;
   out.a[0][0] = l0;
   out.a[1][0] = l1;
   out.a[5][0] = f.a[3][0];
@@ -858,7 +948,7 @@ void reinitializeJacobians(const Surface& s, Stepper& st) {
   st.derivative = Vec8::zero();
   V3 pos = st.position(), dir = st.direction();
   double l0, l1;
-  if (!s.globalToLocal(pos, l0, l1)) return;
+  if (!s.globalToLocal(pos, dir, l0, l1)) return;
   st.jacToGlobal = s.boundToFreeJacobian(pos, dir);
 }
 
@@ -1205,9 +1295,9 @@ Target nextTarget(const Geometry& geo, DirectNav& nav, const V3& pos,
 }
 
 // LoopProtection.hpp:19-67
-void setupLoopProtection(const Geometry& geo, Stepper& st, double& limit,
-                         bool releaseLimit, double propDir) {
-  if (!geo.opt.loop_protection) return;
+void setupLoopProtection(const Geometry& geo, const acts_kf_options& o, Stepper& st,
+                         double& limit, bool releaseLimit, double propDir) {
+  if (!o.loop_protection) return;
   if (releaseLimit) limit = propDir * std::numeric_limits<double>::max();
   V3 B;
   if (geo.field.get(st.position(), B)) return;
@@ -1215,8 +1305,12 @@ void setupLoopProtection(const Geometry& geo, Stepper& st, double& limit,
   if (Bn == 0) return;
   const double p = geo.particle.extractMomentum(st.qop());
   const double helixPath = propDir * 2 * kPi * p / Bn;
-  const double loopLimit = geo.opt.loop_fraction * helixPath;
+  const double loopLimit = o.loop_fraction * helixPath;
   if (std::abs(loopLimit) < std::abs(limit)) limit = loopLimit;
+}
+void setupLoopProtection(const Geometry& geo, Stepper& st, double& limit,
+                         bool releaseLimit, double propDir) {
+  setupLoopProtection(geo, geo.opt, st, limit, releaseLimit, propDir);
 }
 
 // StandardAborters.hpp:41-61
@@ -1500,6 +1594,82 @@ void fitTrack(const Geometry& geo, const TrackIn& in, TrackOut& out) {
   out.states.reserve(tip + 1);
   for (int k = 0; k <= tip; ++k) out.states.push_back(S[k].s);
   out.status = 0;
+
+  // KalmanFitter.hpp:959-971: with a reference surface the fit ends with
+  // the extrapolation of the smoothed track, and fails if that fails
+  if (o.reference_surface == ACTS_KF_REFERENCE_PERIGEE) {
+    const int r = extrapolateToReference(geo, out);
+    if (r) {
+      const int nSteps = out.nSteps, nTrials = out.nTrials;
+      out = TrackOut();
+      out.status = r;
+      out.nSteps = nSteps;
+      out.nTrials = nTrials;
+    }
+  }
+}
+
+// findTrackStateForExtrapolation (TrackHelpers.hpp:180-320) and
+// extrapolateTrackToReferenceSurface (TrackHelpers.hpp:339-383)
+int extrapolateToReference(const Geometry& geo, TrackOut& out) {
+  const std::vector<StateOut>& S = out.states;
+  // findFirstMeasurementState / findLastMeasurementState: states whose type
+  // is a measurement (not an outlier)
+  int first = -1, last = -1;
+  for (int k = 0; k < static_cast<int>(S.size()); ++k) {
+    const uint32_t f = S[k].flags;
+    if ((f & ACTS_KF_STATE_HAS_MEASUREMENT) && !(f & ACTS_KF_STATE_IS_OUTLIER)) {
+      if (first < 0) first = k;
+      last = k;
+    }
+  }
+  if (first < 0) return ACTS_KF_STATUS_NO_COMPATIBLE_STATE;
+  // the straight-line intersection of the reference surface from a state's
+  // (smoothed) parameters
+  auto intersect = [&](int k) {
+    const Surface& s = geo.surfaces[S[k].surface];
+    const Vec8 f = boundToFree(s, S[k].smoothed);
+    const V3 pos = v3(f.a[0][0], f.a[1][0], f.a[2][0]);
+    const V3 dir = v3(f.a[4][0], f.a[5][0], f.a[6][0]);
+    return geo.reference.intersect(pos, dir, kOnSurfaceTolerance).is[0];
+  };
+  int k = -1;
+  double distance = 0;
+  const int strategy = geo.opt.reference_strategy;
+  if (strategy == ACTS_KF_EXTRAPOLATE_FIRST || strategy == ACTS_KF_EXTRAPOLATE_LAST) {
+    k = strategy == ACTS_KF_EXTRAPOLATE_FIRST ? first : last;
+    const Intersection x = intersect(k);
+    if (!x.valid()) return ACTS_KF_STATUS_REFERENCE_UNREACHABLE;
+    distance = x.path;
+  } else {
+    const Intersection xf = intersect(first);
+    const Intersection xl = intersect(last);
+    const double absFirst = std::abs(xf.path), absLast = std::abs(xl.path);
+    if (xf.valid() && absFirst <= absLast) {
+      k = first;
+      distance = xf.path;
+    } else if (xl.valid() && absLast <= absFirst) {
+      k = last;
+      distance = xl.path;
+    } else {
+      return ACTS_KF_STATUS_REFERENCE_UNREACHABLE;
+    }
+  }
+  // Direction::fromScalarZeroAsPositive
+  const double direction = distance >= 0 ? 1.0 : -1.0;
+  Vec6 endParams;
+  M6 endCov, jacobian;
+  double path = 0;
+  int nSteps = 0;
+  const int r = propagateToSurface(geo, S[k].smoothed, &S[k].smoothedCov,
+                                   geo.surfaces[S[k].surface], geo.reference, direction,
+                                   endParams, endCov, jacobian, path, nSteps,
+                                   &geo.extrapolationOpt);
+  if (r) return r;
+  out.hasReference = true;
+  out.referenceParams = endParams;
+  out.referenceCov = endCov;
+  return 0;
 }
 
 // ===========================================================================
@@ -1511,12 +1681,12 @@ int propagateToSurface(const Geometry& geo, const Vec6& startParams,
                        const M6* startCov, const Surface& startSurface,
                        const Surface& target, double direction, Vec6& endParams,
                        M6& endCov, M6& jacobian, double& pathLength,
-                       int& nSteps) {
-  const auto& o = geo.opt;
+                       int& nSteps, const acts_kf_options* options) {
+  const auto& o = options ? *options : geo.opt;
   Stepper st;
   initializeStepper(st, startSurface, startParams, startCov, o);
   double pathLimit = o.path_limit;
-  setupLoopProtection(geo, st, pathLimit, false, direction);
+  setupLoopProtection(geo, o, st, pathLimit, false, direction);
   const double nearLimit = std::numeric_limits<double>::lowest();
 
   // SurfaceReached::checkAbort

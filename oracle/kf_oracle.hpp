@@ -256,6 +256,12 @@ struct Surface {
                      double tol = kOnSurfaceTolerance) const;
   V3 localToGlobal(double l0, double l1) const;
   Mat<3, 3> referenceFrame(const V3& pos) const;
+  // the same with the direction, which a LineSurface's frame depends on
+  // (LineSurface.cpp:57-114); the other surface types ignore it
+  bool globalToLocal(const V3& pos, const V3& dir, double& l0, double& l1,
+                     double tol = kOnSurfaceTolerance) const;
+  V3 localToGlobal(double l0, double l1, const V3& dir) const;
+  Mat<3, 3> referenceFrame(const V3& pos, const V3& dir) const;
   double pathCorrection(const V3& pos, const V3& dir) const;
   Mat<8, 6> boundToFreeJacobian(const V3& pos, const V3& dir) const;
   Mat<6, 8> freeToBoundJacobian(const V3& pos, const V3& dir) const;
@@ -324,6 +330,10 @@ struct Geometry {
   Field field;
   acts_kf_options opt;
   Particle particle;
+  // the default propagator and stepper options, which the extrapolation to
+  // the reference surface uses (KalmanFitter.hpp:960-961)
+  acts_kf_options extrapolationOpt;
+  Surface reference;  // the reference surface (ACTS_KF_SURFACE_LINE)
 };
 
 struct StateOut {
@@ -338,6 +348,9 @@ struct StateOut {
 
 struct TrackOut {
   int status = 0;
+  bool hasReference = false;
+  Vec6 referenceParams;
+  M6 referenceCov;
   std::vector<StateOut> states;  // up to and including the tip
   float chi2 = 0.f;
   int ndf = 0, nMeas = 0, nHoles = 0, nOutliers = 0;
@@ -380,6 +393,11 @@ int propagateToSurface(const Geometry& geo, const Vec6& startParams,
                        const M6* startCov, const Surface& startSurface,
                        const Surface& target, double direction, Vec6& endParams,
                        M6& endCov, M6& jacobian, double& pathLength,
-                       int& nSteps);
+                       int& nSteps, const acts_kf_options* options = nullptr);
+
+// extrapolateTrackToReferenceSurface (TrackHelpers.hpp:180-383) for a fitted
+// track: the state chosen by the strategy, its smoothed parameters, to the
+// reference surface; returns a status
+int extrapolateToReference(const Geometry& geo, TrackOut& out);
 
 }  // namespace acts_oracle

@@ -59,6 +59,8 @@ def load():
                                                   C.c_double, dp, C.POINTER(C.c_int32)]
     lib.acts_oracle_surface_local.restype = C.c_int
     lib.acts_oracle_surface_local.argtypes = [C.POINTER(capi.Surface), C.c_int32, dp, dp]
+    lib.acts_oracle_line_local.restype = C.c_int
+    lib.acts_oracle_line_local.argtypes = [C.POINTER(capi.Surface), C.c_int32, dp, dp, dp]
     lib.acts_oracle_propagate.restype = C.c_int
     lib.acts_oracle_propagate.argtypes = [C.POINTER(capi.Field), C.POINTER(capi.Options),
                                           dp, dp, dp, C.POINTER(capi.Surface), C.c_double,
@@ -177,6 +179,16 @@ def surface_local(surface, to_global, v3):
     inp = np.ascontiguousarray(v3, np.float64)
     out = np.zeros(3)
     rc = lib.acts_oracle_surface_local(C.byref(surface), int(to_global), _dp(inp), _dp(out))
+    return rc, out
+
+
+def line_local(surface, to_global, v3, direction):
+    """LineSurface local <-> global; the frame depends on the direction."""
+    lib = load()
+    inp = np.ascontiguousarray(v3, np.float64)
+    d = np.ascontiguousarray(direction, np.float64)
+    out = np.zeros(3)
+    rc = lib.acts_oracle_line_local(C.byref(surface), int(to_global), _dp(inp), _dp(d), _dp(out))
     return rc, out
 
 

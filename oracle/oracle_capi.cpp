@@ -4,6 +4,7 @@
 #include <atomic>
 #include <chrono>
 #include <cstring>
+#include <limits>
 #include <thread>
 #include <vector>
 
@@ -17,6 +18,28 @@ void buildGeometry(Geometry& geo, const acts_kf_surface* surfaces, int n,
                    const acts_kf_material_slab* mats, int nm,
                    const acts_kf_field* field, const acts_kf_options* opt) {
   geo.opt = *opt;
+  // the extrapolation to the reference surface runs with default propagator
+  // and stepper options (KalmanFitter.hpp:960-961; PropagatorOptions.hpp,
+  // StepperOptions.hpp, DirectNavigator.hpp:51-61)
+  geo.extrapolationOpt = *opt;
+  geo.extrapolationOpt.step_tolerance = 1e-4;
+  geo.extrapolationOpt.step_size_cutoff = 0.0;
+  geo.extrapolationOpt.initial_step_size = 10000.0;
+  geo.extrapolationOpt.max_step_size = std::numeric_limits<double>::max();
+  geo.extrapolationOpt.max_runge_kutta_step_trials = 10000;
+  geo.extrapolationOpt.max_steps = 1000;
+  geo.extrapolationOpt.max_target_skipping = 100;
+  geo.extrapolationOpt.loop_protection = 1;
+  geo.extrapolationOpt.path_limit = std::numeric_limits<double>::max();
+  geo.extrapolationOpt.loop_fraction = 0.5;
+  geo.extrapolationOpt.surface_tolerance = 1e-4;
+  geo.extrapolationOpt.near_limit = -0.1;
+  geo.reference = Surface();
+  geo.reference.type = ACTS_KF_SURFACE_LINE;
+  {
+    const double rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    geo.reference.pl.set(rot, opt->reference_position);
+  }
   geo.particle.absPdg = opt->abs_pdg;
   geo.particle.mass = opt->mass;
   geo.particle.absQ = opt->abs_charge;
@@ -99,6 +122,10 @@ void storeTrack(const acts_kf_batch_in* in, const acts_kf_batch_out* out,
     if (out->smoothed) store(out->smoothed + 6 * slot, s.smoothed);
     if (out->smoothed_cov) store(out->smoothed_cov + 36 * slot, s.smoothedCov);
     if (out->jacobian) store(out->jacobian + 36 * slot, s.jacobian);
+  }
+  if (tr.hasReference) {
+    if (out->reference_params) store(out->reference_params + 6 * t, tr.referenceParams);
+    if (out->reference_cov) store(out->reference_cov + 36 * t, tr.referenceCov);
   }
 }
 
@@ -248,6 +275,27 @@ int acts_oracle_surface_local(const acts_kf_surface* s, int32_t to_global,
   }
   double l0, l1;
   if (!sf.globalToLocal(v3(in3[0], in3[1], in3[2]), l0, l1))
+    return ACTS_KF_STATUS_NOT_ON_SURFACE;
+  out3[0] = l0;
+  out3[1] = l1;
+  out3[2] = 0;
+  return 0;
+}
+
+// LineSurface local <-> global with the direction the frame depends on.
+int acts_oracle_line_local(const acts_kf_surface* s, int32_t to_global, const double* in3,
+                           const double* dir, double* out3) {
+  Surface sf;
+  sf.type = s->type;
+  sf.pl.set(s->rotation, s->translation);
+  const V3 d = v3(dir[0], dir[1], dir[2]);
+  if (to_global) {
+    V3 g = sf.localToGlobal(in3[0], in3[1], d);
+    for (int i = 0; i < 3; ++i) out3[i] = g[i];
+    return 0;
+  }
+  double l0, l1;
+  if (!sf.globalToLocal(v3(in3[0], in3[1], in3[2]), d, l0, l1))
     return ACTS_KF_STATUS_NOT_ON_SURFACE;
   out3[0] = l0;
   out3[1] = l1;

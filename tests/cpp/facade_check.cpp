@@ -107,7 +107,13 @@ int main(int argc, char** argv) {
   }
 
   try {
-    acts_amd::KalmanFitter fitter(surfaces, mats, field, opts, 0);
+    // as the Examples configure the fit (TrackFittingAlgorithm.cpp:89-90,
+    // KalmanFitterFunction.cpp:114-115): fitted parameters at a perigee
+    // surface at the origin, extrapolated from the first measurement
+    acts_kf_options fitOpts = opts;
+    fitOpts.reference_surface = ACTS_KF_REFERENCE_PERIGEE;
+    fitOpts.reference_strategy = ACTS_KF_EXTRAPOLATE_FIRST;
+    acts_amd::KalmanFitter fitter(surfaces, mats, field, fitOpts, 0);
     for (int64_t t = 0; t < nt; ++t)
       fitter.enqueue(meas[t].begin(), meas[t].end(), starts[t], seqs[t]);
     const std::vector<acts_amd::FitResult> batch = fitter.fitBatch();
@@ -119,7 +125,12 @@ int main(int argc, char** argv) {
           fitter.fit(meas[t].begin(), meas[t].end(), starts[t], seqs[t]);
       const acts_amd::FitResult& b = batch[t];
       bool same = one.status == b.status && one.states.size() == b.states.size() &&
-                  one.chi2 == b.chi2 && one.ndf == b.ndf;
+                  one.chi2 == b.chi2 && one.ndf == b.ndf && one.hasReference == b.hasReference &&
+                  (!b.ok() || b.hasReference) &&
+                  std::memcmp(one.referenceParameters.data(), b.referenceParameters.data(),
+                              6 * sizeof(double)) == 0 &&
+                  std::memcmp(one.referenceCovariance.data(), b.referenceCovariance.data(),
+                              36 * sizeof(double)) == 0;
       for (size_t k = 0; same && k < one.states.size(); ++k) {
         same = one.states[k].surface == b.states[k].surface &&
                one.states[k].flags == b.states[k].flags &&
