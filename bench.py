@@ -156,9 +156,10 @@ def main() -> int:
     elif args.input_order:
         tuned = {"length_sorted": False}
 
-    def run(precision):
-        opts = dataclasses.replace(wl_opts, precision=precision, kernel_timing=True, **tuned)
-        out = DeviceResult(n, dev_in.n_slots, OUTPUTS, dev)
+    def run(precision, extra=None, outputs=OUTPUTS):
+        opts = dataclasses.replace(wl_opts, precision=precision, kernel_timing=True, **tuned,
+                                   **(extra or {}))
+        out = DeviceResult(n, dev_in.n_slots, outputs, dev)
         fitter = A.KalmanFitter(det, field, opts)
         ms, wall, kt = timed_fits(fitter, bin_, out.batch_out(), stream, args.steps,
                                   args.warmup)
@@ -218,6 +219,18 @@ def main() -> int:
                      "smoother_ms": smooth_ms, "fit_flop_per_step": flop_all},
         "wall_s_timed": wall64,
     }
+
+    # the Examples' configuration: fitted parameters extrapolated from the
+    # first measurement to a perigee surface at the origin
+    msp, _, _, outp = run(capi.PRECISION_FP64,
+                          {"reference_surface": capi.REFERENCE_PERIGEE,
+                           "reference_strategy": capi.EXTRAPOLATE_FIRST},
+                          OUTPUTS + ("reference_params", "reference_cov"))
+    line["with_perigee"] = {
+        "value": n / (msp * 1e-3), "ms_per_step": msp,
+        "extrapolation_failures": int(np.count_nonzero(outp.host("status")) -
+                                      np.count_nonzero(r.status))}
+    del outp
 
     first64 = quality.first_states(out64, events)
     line["pulls"] = quality.pulls(first64, events)
