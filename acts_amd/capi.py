@@ -50,6 +50,7 @@ EXTRAPOLATE_FIRST, EXTRAPOLATE_LAST, EXTRAPOLATE_FIRST_OR_LAST = 0, 1, 2
 SURFACE_SENSITIVE = 1
 
 FIELD_CONSTANT, FIELD_GRID_XYZ = 0, 1
+SMOOTHER_GAIN_MATRIX, SMOOTHER_MBF = 0, 1
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 
 
@@ -95,6 +96,8 @@ class Options(C.Structure):
         ("energy_loss", C.c_int32),
         ("use_joseph", C.c_int32),
         ("precision", C.c_int32),
+        ("smoother", C.c_int32),
+        ("reserved_kf", C.c_int32),
         ("outlier_chi2_cut", C.c_double),
         ("step_tolerance", C.c_double),
         ("step_size_cutoff", C.c_double),

@@ -120,14 +120,16 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
   }
 }
 
-// RTS smoother + track summary + outputs: one lane per track
+// Smoother (RTS, or MBF with Mbf) + track summary + outputs: one lane per
+// track
+template <bool Mbf>
 __global__ __launch_bounds__(kBlock) void smoothKernel(DevBatch in, DevOut out,
                                                        DevScratch sc,
                                                        const int32_t* __restrict__ perm) {
   const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool valid = g < in.n;
   const int64_t t = !valid ? 0 : (perm ? perm[g] : g);
-  smoothTrack(t, valid, in, out, sc);
+  smoothTrack<Mbf>(t, valid, in, out, sc);
 }
 
 // Extrapolation of every successfully smoothed track to the reference
@@ -354,6 +356,7 @@ struct acts_kf_ctx {
   int32_t innerSteps = 64;
   int32_t waitFrac = 64;
   int32_t quarantine = 0;
+  bool mbf = false;  // MbfSmoother instead of GainMatrixSmoother
   // reference surface of the fitted parameters, and the default propagator
   // and stepper options its extrapolation runs with
   int32_t refSurface = ACTS_KF_REFERENCE_NONE;
@@ -459,6 +462,9 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   if (options->precision != ACTS_KF_PRECISION_FP64 &&
       options->precision != ACTS_KF_PRECISION_MIXED)
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: precision mode not supported");
+  if (options->smoother != ACTS_KF_SMOOTHER_GAIN_MATRIX &&
+      options->smoother != ACTS_KF_SMOOTHER_MBF)
+    return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: smoother not supported");
   if (options->reference_surface != ACTS_KF_REFERENCE_NONE &&
       options->reference_surface != ACTS_KF_REFERENCE_PERIGEE)
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: reference surface not supported");
@@ -580,6 +586,7 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   o.mass = options->mass;
   o.absQ = options->abs_charge;
   c->pc = makePhysConst();
+  c->mbf = options->smoother == ACTS_KF_SMOOTHER_MBF;
   c->refSurface = options->reference_surface;
   c->refStrategy = options->reference_strategy;
   for (int i = 0; i < 3; ++i) c->refLine.c[i] = options->reference_position[i];
@@ -763,8 +770,8 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
                      dim3(static_cast<unsigned>(waves)), dim3(kBlock), 0, stream, fa, queue,
                      c->innerSteps, c->waitFrac, c->quarantine, perm);
   if (timing) KF_HIP(hipEventRecord(c->ev[1], stream));
-  hipLaunchKernelGGL(smoothKernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream,
-                     b, d, sc, perm);
+  hipLaunchKernelGGL(c->mbf ? smoothKernel<true> : smoothKernel<false>,
+                     dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream, b, d, sc, perm);
   if (withRef) {
     if (c->mixed)
       hipLaunchKernelGGL(referenceKernel<true>, dim3(static_cast<unsigned>(blocks)), dim3(kBlock),

@@ -40,7 +40,9 @@ struct DevBatch {
 // Covariances are stored as packed upper triangles (kf::sym).
 constexpr int kPred = 0, kPCov = 6, kFilt = 27, kFCov = 33, kJac = 54,
               kPath = 90, kChi2 = 91, kScratchComps = 92;
-constexpr int kISurf = 0, kIFlags = 1, kIScratchComps = 2;
+// ints: surface, type flags, index of the state's measurement in the
+// track's measurements (-1: none)
+constexpr int kISurf = 0, kIFlags = 1, kIMeas = 2, kIScratchComps = 3;
 // per track with a reference surface: smoothed parameters (6) and packed
 // covariance (21) of the first, then of the last measurement state
 constexpr int kRefComps = 54;

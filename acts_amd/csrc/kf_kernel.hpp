@@ -351,6 +351,7 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
           sd(sc, k, kChi2, t) = chi2f;
           si(sc, k, kISurf, t) = reached;
           si(sc, k, kIFlags, t) = static_cast<int32_t>(flags);
+          si(sc, k, kIMeas, t) = mi;
           L.nStates = k + 1;
           if (mi >= 0) L.tip = k;
           if (hasMat) L.status = materialInteraction(T, s, slabs[s.mat], a.pc, o);
@@ -391,6 +392,133 @@ __device__ inline void storeSym(double* dst, const double* p) {
 // (calculateTrackQuantities) and the per-state outputs.
 // `valid` is false for the padding lanes of the last wavefront: they take
 // part in the wavefront-wide reduction but touch no memory.
+// MbfSmoother::calculateSmoothed (MbfSmoother.cpp:16-27) with the carried
+// Lambda (full 6x6) and lambda: Cs = Cf - (Cf Lambda) Cf, xs = xf - Cf lambda
+__device__ inline void mbfSmoothed(const double xf[6], const double Cf[21], const double L[36],
+                                   const double l[6], double xs[6], double Cs[36]) {
+  double CfL[36];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double acc = Cf[sym(i, 0)] * L[j];
+#pragma unroll
+      for (int m = 1; m < 6; ++m) acc = acc + Cf[sym(i, m)] * L[6 * m + j];
+      CfL[6 * i + j] = acc;
+    }
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double acc = CfL[6 * i] * Cf[sym(0, j)];
+#pragma unroll
+      for (int m = 1; m < 6; ++m) acc = acc + CfL[6 * i + m] * Cf[sym(m, j)];
+      Cs[6 * i + j] = Cf[sym(i, j)] - acc;
+    }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double acc = Cf[sym(i, 0)] * l[0];
+#pragma unroll
+    for (int m = 1; m < 6; ++m) acc = acc + Cf[sym(i, m)] * l[m];
+    xs[i] = xf[i] - acc;
+  }
+  normalizePhiTheta(xs[2], xs[3]);
+}
+
+// (F^T A) F for the full 6x6 A, in place
+__device__ inline void mbfTransport(const double F[36], double A[36]) {
+  double T[36];
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double acc = F[i] * A[j];
+#pragma unroll
+      for (int m = 1; m < 6; ++m) acc = acc + F[6 * m + i] * A[6 * m + j];
+      T[6 * i + j] = acc;
+    }
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double acc = T[6 * i] * F[j];
+#pragma unroll
+      for (int m = 1; m < 6; ++m) acc = acc + T[6 * i + m] * F[6 * m + j];
+      A[6 * i + j] = acc;
+    }
+}
+
+// The Lambda / lambda update of state k on the way to state k-1
+// (MbfSmoother.cpp:29-48, detail/MbfSmootherImpl.hpp:17-57): through the
+// measurement update if state k is a measurement, then through its
+// transport jacobian F.
+__device__ inline void mbfUpdate(int64_t t, int k, uint32_t flags, const DevBatch& in,
+                                 const DevScratch& sc, double L[36], double l[6]) {
+  double F[36];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) F[i] = sd(sc, k, kJac + i, t);
+  const bool isMeasurement =
+      (flags & ACTS_KF_STATE_HAS_MEASUREMENT) && !(flags & ACTS_KF_STATE_IS_OUTLIER);
+  if (isMeasurement) {
+    double P[21];
+#pragma unroll
+    for (int i = 0; i < 21; ++i) P[i] = sd(sc, k, kPCov + i, t);
+    const int64_t mg = in.measOff[t] + si(sc, k, kIMeas, t);
+    const double m0 = in.measVal[2 * mg], m1 = in.measVal[2 * mg + 1];
+    const double* V = in.measCov + 4 * mg;
+    // S = H P H^T + V and Eigen's closed-form 2x2 inverse
+    const double S00 = P[sym(0, 0)] + V[0], S01 = P[sym(0, 1)] + V[1];
+    const double S10 = P[sym(1, 0)] + V[2], S11 = P[sym(1, 1)] + V[3];
+    const double det = S00 * S11 - S10 * S01;
+    const double invdet = 1.0 / det;
+    const double Si[2][2] = {{S11 * invdet, -S01 * invdet}, {-S10 * invdet, S00 * invdet}};
+    // CHat = I - (P H^T S^-1) H
+    double CH[36];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const double id = (i == j) ? 1.0 : 0.0;
+        CH[6 * i + j] =
+            j < 2 ? id - (P[sym(i, 0)] * Si[0][j] + P[sym(i, 1)] * Si[1][j]) : id;
+      }
+    const double y0 = m0 - sd(sc, k, kPred + 0, t), y1 = m1 - sd(sc, k, kPred + 1, t);
+    // Lambda~ = (H^T S^-1) H + (CHat^T Lambda) CHat, by the transport helper
+    // with CHat in place of F
+    mbfTransport(CH, L);
+    L[0] = Si[0][0] + L[0];
+    L[1] = Si[0][1] + L[1];
+    L[6] = Si[1][0] + L[6];
+    L[7] = Si[1][1] + L[7];
+    // lambda~ = (-H^T S^-1) y + CHat^T lambda
+    double lt[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double acc = CH[i] * l[0];
+#pragma unroll
+      for (int m = 1; m < 6; ++m) acc = acc + CH[6 * m + i] * l[m];
+      lt[i] = acc;
+    }
+    lt[0] = (-Si[0][0] * y0 + -Si[0][1] * y1) + lt[0];
+    lt[1] = (-Si[1][0] * y0 + -Si[1][1] * y1) + lt[1];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) l[i] = lt[i];
+  }
+  mbfTransport(F, L);
+  double ln[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double acc = F[i] * l[0];
+#pragma unroll
+    for (int m = 1; m < 6; ++m) acc = acc + F[6 * m + i] * l[m];
+    ln[i] = acc;
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) l[i] = ln[i];
+}
+
+// Mbf = false: GainMatrixSmoother (RTS); true: MbfSmoother.
+template <bool Mbf>
 __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevBatch& in,
                                    const DevOut& out, const DevScratch& sc) {
   int status = valid ? sc.track[t] : 0;
@@ -408,15 +536,28 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
   for (int off = 32; off > 0; off >>= 1) kTop = max(kTop, __shfl_xor(kTop, off));
   const int64_t base = valid ? in.seqOff[t] : 0;
   double xsN[6], CsN[21];
+  double L[36], l[6];  // MbfSmoother's Lambda-hat and lambda-hat
+  if constexpr (Mbf) {
+#pragma unroll
+    for (int i = 0; i < 36; ++i) L[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) l[i] = 0.0;
+  }
   bool active = tip >= 0;
   for (int k = kTop; k >= 0; --k) {
     if (!active || k > tip) continue;
-    double xf[6], Cf[21], xs[6], Cs[21];
+    double xf[6], Cf[21], xs[6], Cs[21], CsFull[36];
 #pragma unroll
     for (int i = 0; i < 6; ++i) xf[i] = sd(sc, k, kFilt + i, t);
 #pragma unroll
     for (int i = 0; i < 21; ++i) Cf[i] = sd(sc, k, kFCov + i, t);
-    if (k == tip) {
+    if constexpr (Mbf) {
+      mbfSmoothed(xf, Cf, L, l, xs, CsFull);
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = i; j < 6; ++j) Cs[sym(i, j)] = CsFull[6 * i + j];
+    } else if (k == tip) {
 #pragma unroll
       for (int i = 0; i < 6; ++i) xs[i] = xf[i];
 #pragma unroll
@@ -461,7 +602,13 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
     const int64_t slot = base + k;
     if (out.smoothed)
       for (int i = 0; i < 6; ++i) out.smoothed[6 * slot + i] = xs[i];
-    if (out.smoothedCov) storeSym(out.smoothedCov + 36 * slot, Cs);
+    if (out.smoothedCov) {
+      if constexpr (Mbf) {
+        for (int i = 0; i < 36; ++i) out.smoothedCov[36 * slot + i] = CsFull[i];
+      } else {
+        storeSym(out.smoothedCov + 36 * slot, Cs);
+      }
+    }
     if (out.filtered)
       for (int i = 0; i < 6; ++i) out.filtered[6 * slot + i] = xf[i];
     if (out.filteredCov) storeSym(out.filteredCov + 36 * slot, Cf);
@@ -478,10 +625,14 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
     if (out.stateFlags) out.stateFlags[slot] = flags;
     if (out.statePath) out.statePath[slot] = sd(sc, k, kPath, t);
     if (out.stateChi2) out.stateChi2[slot] = c2;
+    if constexpr (Mbf) {
+      if (k > 0) mbfUpdate(t, k, flags, in, sc, L, l);
+    } else {
 #pragma unroll
-    for (int i = 0; i < 6; ++i) xsN[i] = xs[i];
+      for (int i = 0; i < 6; ++i) xsN[i] = xs[i];
 #pragma unroll
-    for (int i = 0; i < 21; ++i) CsN[i] = Cs[i];
+      for (int i = 0; i < 21; ++i) CsN[i] = Cs[i];
+    }
   }
   if (!valid) return;
   // the status after smoothing, for the extrapolation kernel

@@ -233,6 +233,14 @@ def main() -> int:
     del outp
 
     first64 = quality.first_states(out64, events)
+    # the MbfSmoother (the Examples' default smoother) in place of the RTS one
+    msb, _, ktb, outb = run(capi.PRECISION_FP64, {"smoother": capi.SMOOTHER_MBF})
+    line["with_mbf"] = {
+        "value": n / (msb * 1e-3), "ms_per_step": msb,
+        "smoother_ms": float(np.mean([k[1] for k in ktb])),
+        "first_state_rel_err_vs_rts": quality.relative_error(quality.first_states(outb, events),
+                                                             first64)}
+    del outb
     line["pulls"] = quality.pulls(first64, events)
     if not args.no_mixed:
         msm, _, ktm, outm = run(capi.PRECISION_MIXED)

@@ -145,12 +145,20 @@ typedef struct acts_kf_field {
  *      PropagatorPlainOptions + DirectNavigator::Options + particle -------- */
 #define ACTS_KF_PRECISION_FP64 0  /* plain FP64 fit (the drop-in) */
 #define ACTS_KF_PRECISION_MIXED 1 /* FP32 stepping/field, FP64 Kalman algebra */
+/* KalmanFitterExtensions::smoother: GainMatrixSmoother (RTS,
+ * GainMatrixSmoother.cpp:18-97) or MbfSmoother (modified Bryson-Frazier,
+ * MbfSmoother.cpp, detail/MbfSmootherImpl.hpp), which the Examples use
+ * (KalmanFitterFunction.cpp:39-44,100-102)                                 */
+#define ACTS_KF_SMOOTHER_GAIN_MATRIX 0
+#define ACTS_KF_SMOOTHER_MBF 1
 
 typedef struct acts_kf_options {
   int32_t multiple_scattering; /* KalmanFitter.hpp:166, default 1 */
   int32_t energy_loss;         /* KalmanFitter.hpp:169, default 1 */
   int32_t use_joseph;          /* GainMatrixUpdater, default 0 */
   int32_t precision;           /* ACTS_KF_PRECISION_* */
+  int32_t smoother;            /* ACTS_KF_SMOOTHER_*, default GAIN_MATRIX */
+  int32_t reserved_kf;
   double outlier_chi2_cut;     /* predicted chi2 cut; <= 0 disables the finder */
   /* StepperPlainOptions (StepperOptions.hpp:36-49) */
   double step_tolerance;       /* 1e-4 */

@@ -1369,6 +1369,7 @@ int kfFilter(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
     if (r) return r;
     KFState ks;
     ks.isMeasurementBranch = true;
+    ks.s.meas = mi;
     Vec6 bp;
     r = boundState(surf, st, false, bp);
     if (r) return r;
@@ -1440,6 +1441,64 @@ int kfFilter(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
 }
 
 }  // namespace
+
+// MbfSmoother::operator() (MbfSmoother.hpp:48-89, MbfSmoother.cpp:16-48,
+// detail/MbfSmootherImpl.hpp:17-57): from the tip backwards, smoothed =
+// filtered - C_f lambda, C_s = C_f - C_f Lambda C_f; then Lambda, lambda
+// pass through the state's measurement update (2D, projector on loc0/loc1)
+// if it is a measurement, and through its transport jacobian F.  Products
+// are associated as Eigen evaluates them (left to right).
+void mbfSmooth(const TrackIn& in, std::vector<KFState>& S, int tip) {
+  M6 bigL = M6::zero();
+  Vec6 smallL = Vec6::zero();
+  for (int k = tip; k >= 0; --k) {
+    StateOut& s = S[k].s;
+    // calculateSmoothed
+    const M6 CfLCf = mul(mul(s.filteredCov, bigL), s.filteredCov);
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) s.smoothedCov.a[i][j] = s.filteredCov.a[i][j] - CfLCf.a[i][j];
+    const Vec6 Cfl = mul(s.filteredCov, smallL);
+    for (int i = 0; i < 6; ++i) s.smoothed.a[i][0] = s.filtered.a[i][0] - Cfl.a[i][0];
+    normalizePhiTheta(s.smoothed.a[2][0], s.smoothed.a[3][0]);
+    if (k == 0) break;
+    const M6& F = s.jacobian;
+    const M6 Ft = transpose(F);
+    const bool isMeasurement =
+        (s.flags & ACTS_KF_STATE_HAS_MEASUREMENT) && !(s.flags & ACTS_KF_STATE_IS_OUTLIER);
+    if (isMeasurement) {
+      const M6& P = s.predictedCov;
+      const double* m = &in.measValue[2 * s.meas];
+      const double* V = &in.measCov[4 * s.meas];
+      // S = H P H^T + V and its inverse (Eigen's closed-form 2x2 inverse)
+      const double S00 = P.a[0][0] + V[0], S01 = P.a[0][1] + V[1];
+      const double S10 = P.a[1][0] + V[2], S11 = P.a[1][1] + V[3];
+      const double det = S00 * S11 - S10 * S01;
+      const double invdet = 1.0 / det;
+      const double Si[2][2] = {{S11 * invdet, -S01 * invdet}, {-S10 * invdet, S00 * invdet}};
+      // K = (P H^T) S^-1, CHat = I - K H, y = m - H x_pred
+      double K[6][2];
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 2; ++j) K[i][j] = P.a[i][0] * Si[0][j] + P.a[i][1] * Si[1][j];
+      M6 CHat = M6::identity();
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 2; ++j) CHat.a[i][j] = CHat.a[i][j] - K[i][j];
+      const double y0 = m[0] - s.predicted.a[0][0], y1 = m[1] - s.predicted.a[1][0];
+      // Lambda~ = (H^T S^-1) H + (CHat^T Lambda) CHat
+      M6 bigLT = mul(mul(transpose(CHat), bigL), CHat);
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) bigLT.a[i][j] = Si[i][j] + bigLT.a[i][j];
+      // lambda~ = (-H^T S^-1) y + CHat^T lambda
+      Vec6 smallLT = mul(transpose(CHat), smallL);
+      for (int i = 0; i < 2; ++i)
+        smallLT.a[i][0] = (-Si[i][0] * y0 + -Si[i][1] * y1) + smallLT.a[i][0];
+      bigL = mul(mul(Ft, bigLT), F);
+      smallL = mul(Ft, smallLT);
+    } else {
+      bigL = mul(mul(Ft, bigL), F);
+      smallL = mul(Ft, smallL);
+    }
+  }
+}
 
 void fitTrack(const Geometry& geo, const TrackIn& in, TrackOut& out) {
   const auto& o = geo.opt;
@@ -1561,18 +1620,22 @@ void fitTrack(const Geometry& geo, const TrackIn& in, TrackOut& out) {
   const int tip = res.lastMeasurementIndex;
   std::vector<KFState>& S = res.states;
 
-  // GainMatrixSmoother::operator() (GainMatrixSmoother.hpp:43-99)
-  S[tip].s.smoothed = S[tip].s.filtered;
-  S[tip].s.smoothedCov = S[tip].s.filteredCov;
-  for (int k = tip - 1; k >= 0; --k) {
-    int r = gainMatrixSmoothStep(S[k].s.filtered, S[k].s.filteredCov,
-                                 S[k + 1].s.predicted, S[k + 1].s.predictedCov,
-                                 S[k + 1].s.smoothed, S[k + 1].s.smoothedCov,
-                                 S[k + 1].s.jacobian, S[k].s.smoothed,
-                                 S[k].s.smoothedCov);
-    if (r) {
-      out.status = r;
-      return;
+  if (o.smoother == ACTS_KF_SMOOTHER_MBF) {
+    mbfSmooth(in, S, tip);
+  } else {
+    // GainMatrixSmoother::operator() (GainMatrixSmoother.hpp:43-99)
+    S[tip].s.smoothed = S[tip].s.filtered;
+    S[tip].s.smoothedCov = S[tip].s.filteredCov;
+    for (int k = tip - 1; k >= 0; --k) {
+      int r = gainMatrixSmoothStep(S[k].s.filtered, S[k].s.filteredCov,
+                                   S[k + 1].s.predicted, S[k + 1].s.predictedCov,
+                                   S[k + 1].s.smoothed, S[k + 1].s.smoothedCov,
+                                   S[k + 1].s.jacobian, S[k].s.smoothed,
+                                   S[k].s.smoothedCov);
+      if (r) {
+        out.status = r;
+        return;
+      }
     }
   }
 

@@ -338,6 +338,7 @@ struct Geometry {
 
 struct StateOut {
   int surface = -1;
+  int meas = -1;                    // index of the state's measurement
   uint32_t flags = 0;
   double path = 0;
   float chi2 = 0.f;

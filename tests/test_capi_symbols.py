@@ -67,3 +67,24 @@ def test_structs_match_header_sizes():
     assert C.sizeof(capi.MaterialSlab) == 6 * 4
     assert C.sizeof(capi.BatchOut) == 8 * len(capi.BATCH_OUT_FIELDS)
     assert C.sizeof(capi.BatchIn) == 8 * 10 + 8
+
+
+@pytest.mark.parametrize("field,value", [("smoother", 7), ("precision", 5),
+                                         ("reference_surface", 9), ("reference_strategy", 3)])
+def test_create_rejects_unknown_options(lib, field, value):
+    """Options are validated before any device call: an unknown smoother,
+    precision mode, reference surface or extrapolation strategy fails
+    acts_kf_create with an error code and a message naming the option."""
+    import acts_amd as A
+    det = A.barrel_detector()
+    o = A.KalmanFitterOptions(reference_surface=capi.REFERENCE_PERIGEE).struct()
+    setattr(o, field, value)
+    surf, mats = det.surface_table(), det.material_table()
+    fld = A.ConstantField().struct()
+    ctx = C.c_void_p()
+    rc = lib.acts_kf_create(surf, len(det.surfaces), mats, len(det.materials), C.byref(fld),
+                            C.byref(o), 0, C.byref(ctx))
+    assert rc in (capi.ERR_UNSUPPORTED, capi.ERR_INVALID_ARGUMENT)
+    assert not ctx.value
+    msg = capi.last_error().lower()
+    assert field.split("_")[-1] in msg or field.split("_")[0] in msg, msg
