@@ -51,6 +51,18 @@ SURFACE_SENSITIVE = 1
 
 FIELD_CONSTANT, FIELD_GRID_XYZ = 0, 1
 SMOOTHER_GAIN_MATRIX, SMOOTHER_MBF = 0, 1
+# track-state table (acts_kf_track_states_device): parameter sets, float and
+# int column indices
+TS_PREDICTED, TS_FILTERED, TS_SMOOTHED, TS_UNBIASED = 0, 1, 2, 3
+TS_CHI2, TS_PATH_LENGTH, TS_T_X, TS_T_PARAMS, TS_L_X_HIT, TS_RES_X_HIT = 0, 1, 2, 9, 15, 20
+TS_NFLOAT = 154
+TS_STATE_TYPE, TS_DIM_HIT, TS_HAS_PARAMS, TS_TRACK = 0, 1, 2, 3
+TS_NINT = 4
+
+
+def ts_params(p: int) -> int:
+    """First float column of parameter set p (ACTS_KF_TS_PARAMS)."""
+    return 26 + 32 * p
 PRECISION_FP64, PRECISION_MIXED = 0, 1
 
 
@@ -151,12 +163,16 @@ BATCH_OUT_FIELDS = [
     "n_outliers", "n_steps", "n_step_trials", "state_surface", "state_flags",
     "state_path", "state_chi2", "predicted", "predicted_cov", "filtered",
     "filtered_cov", "smoothed", "smoothed_cov", "jacobian", "reference_params",
-    "reference_cov",
+    "reference_cov", "state_measurement",
 ]
 
 
 class BatchOut(C.Structure):
     _fields_ = [(name, _P) for name in BATCH_OUT_FIELDS]
+
+
+class TrackStatesOut(C.Structure):
+    _fields_ = [("n_slots", C.c_int64), ("columns", _P), ("ints", _P)]
 
 
 class GenConfig(C.Structure):
@@ -208,6 +224,9 @@ SIGNATURES = {
     "acts_kf_sync": (C.c_int, [C.c_void_p]),
     "acts_kf_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]),
+    "acts_kf_track_states_device": (C.c_int, [C.c_void_p, C.POINTER(BatchIn),
+                                              C.POINTER(BatchOut), C.c_void_p,
+                                              C.POINTER(TrackStatesOut), C.c_void_p]),
     "acts_kf_destroy": (None, [C.c_void_p]),
     "acts_kf_status_string": (C.c_char_p, [C.c_int32]),
     "acts_kf_last_error": (C.c_char_p, []),

@@ -13,6 +13,7 @@
 
 #include "kf_field_host.hpp"
 #include "kf_reference.hpp"
+#include "kf_states.hpp"
 
 using namespace kf;
 
@@ -118,6 +119,13 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
       phase = kPhaseIdle;
     }
   }
+}
+
+// Track-state table: one thread per row
+__global__ __launch_bounds__(256) void trackStatesKernel(TsArgs a) {
+  const int64_t slot = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (slot >= a.nSlots) return;
+  trackStateRow(a, slot);
 }
 
 // Smoother (RTS, or MBF with Mbf) + track summary + outputs: one lane per
@@ -723,6 +731,7 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   d.jacobian = out->jacobian;
   d.referenceParams = out->reference_params;
   d.referenceCov = out->reference_cov;
+  d.stateMeas = out->state_measurement;
   DevScratch sc;
   sc.ref = withRef ? static_cast<double*>(c->refBuf.p) : nullptr;
   sc.d = static_cast<double*>(c->scratchD.p);
@@ -909,7 +918,8 @@ int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
       {out->smoothed, sizeof(double) * 6 * nSeq},  {out->smoothed_cov, sizeof(double) * 36 * nSeq},
       {out->jacobian, sizeof(double) * 36 * nSeq},
       {out->reference_params, sizeof(double) * 6 * n},
-      {out->reference_cov, sizeof(double) * 36 * n}};
+      {out->reference_cov, sizeof(double) * 36 * n},
+      {out->state_measurement, sizeof(int32_t) * nSeq}};
   constexpr int nOut = sizeof(outs) / sizeof(outs[0]);
   size_t totalO = 0;
   for (const OSec& q : outs)
@@ -945,11 +955,64 @@ int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
   dout.jacobian = static_cast<double*>(optr[19]);
   dout.reference_params = static_cast<double*>(optr[20]);
   dout.reference_cov = static_cast<double*>(optr[21]);
+  dout.state_measurement = static_cast<int32_t*>(optr[22]);
   int r = launchFit(c, &din, &dout, s, maxLen);
   if (r) return r;
   for (int i = 0; i < nOut; ++i)
     if (outs[i].host && outs[i].bytes)
       KF_HIP(hipMemcpyAsync(outs[i].host, optr[i], outs[i].bytes, hipMemcpyDeviceToHost, s));
+  return ACTS_KF_OK;
+}
+
+int acts_kf_track_states_device(acts_kf_ctx* c, const acts_kf_batch_in* in,
+                                const acts_kf_batch_out* f, const double* truth,
+                                const acts_kf_track_states_out* out, void* stream) {
+  if (!c || !in || !f || !out)
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_track_states_device: null argument");
+  if (in->n_tracks < 0 || out->n_slots < 0)
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_track_states_device: negative size");
+  if (in->n_tracks == 0 || out->n_slots == 0) return ACTS_KF_OK;
+  if (!in->seq_offsets || !in->meas_offsets || !in->meas_values || !in->meas_cov)
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT,
+                "acts_kf_track_states_device: batch offsets or measurements are null");
+  if (!f->status || !f->n_states || !f->state_surface || !f->state_flags || !f->state_path ||
+      !f->state_chi2 || !f->state_measurement || !f->predicted || !f->predicted_cov ||
+      !f->filtered || !f->filtered_cov || !f->smoothed || !f->smoothed_cov)
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT,
+                "acts_kf_track_states_device: a required fit output is null");
+  if (!out->columns || !out->ints)
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_track_states_device: output is null");
+  KF_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  TsArgs a;
+  a.nTracks = in->n_tracks;
+  a.nSlots = out->n_slots;
+  a.nSurfaces = c->nSurfaces;
+  a.pad = 0;
+  a.seqOff = in->seq_offsets;
+  a.measOff = in->meas_offsets;
+  a.measVal = in->meas_values;
+  a.measCov = in->meas_cov;
+  a.status = f->status;
+  a.nStates = f->n_states;
+  a.stateSurf = f->state_surface;
+  a.stateFlags = f->state_flags;
+  a.statePath = f->state_path;
+  a.stateChi2 = f->state_chi2;
+  a.stateMeas = f->state_measurement;
+  a.par[0] = f->predicted;
+  a.par[1] = f->filtered;
+  a.par[2] = f->smoothed;
+  a.cov[0] = f->predicted_cov;
+  a.cov[1] = f->filtered_cov;
+  a.cov[2] = f->smoothed_cov;
+  a.truth = truth;
+  a.geo = c->geo;
+  a.cols = out->columns;
+  a.ints = out->ints;
+  const int64_t blocks = (out->n_slots + 255) / 256;
+  hipLaunchKernelGGL(trackStatesKernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, a);
+  KF_HIP(hipGetLastError());
   return ACTS_KF_OK;
 }
 

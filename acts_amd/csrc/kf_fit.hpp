@@ -28,6 +28,7 @@ struct DevOut {
   double* jacobian;
   double* referenceParams;
   double* referenceCov;
+  int32_t* stateMeas;
 };
 
 // ---------------------------------------------------------------------------

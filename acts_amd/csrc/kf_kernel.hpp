@@ -625,6 +625,10 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
     if (out.stateFlags) out.stateFlags[slot] = flags;
     if (out.statePath) out.statePath[slot] = sd(sc, k, kPath, t);
     if (out.stateChi2) out.stateChi2[slot] = c2;
+    if (out.stateMeas) {
+      const int32_t mi = si(sc, k, kIMeas, t);
+      out.stateMeas[slot] = mi >= 0 ? static_cast<int32_t>(in.measOff[t] + mi) : -1;
+    }
     if constexpr (Mbf) {
       if (k > 0) mbfUpdate(t, k, flags, in, sc, L, l);
     } else {

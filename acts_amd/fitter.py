@@ -80,6 +80,7 @@ STATE_ARRAYS = {
     "smoothed": (np.float64, 6),
     "smoothed_cov": (np.float64, 36),
     "jacobian": (np.float64, 36),
+    "state_measurement": (np.int32, 1),
 }
 TRACK_ARRAYS = {
     "status": np.int32, "n_states": np.int32, "chi2": np.float32, "ndf": np.int32,
@@ -153,6 +154,14 @@ class KalmanFitter:
         """Fit a batch whose buffers are device memory; asynchronous on `stream`."""
         capi.check(self.lib.acts_kf_fit_batch_device(self.ctx, C.byref(bin_), C.byref(bout),
                                                      stream), "acts_kf_fit_batch_device")
+
+    def track_states_device(self, bin_: capi.BatchIn, fitted: capi.BatchOut, truth_ptr,
+                            out: capi.TrackStatesOut, stream=None) -> None:
+        """Fill the track-state table of a fitted batch (device buffers);
+        asynchronous on `stream`."""
+        capi.check(self.lib.acts_kf_track_states_device(self.ctx, C.byref(bin_), C.byref(fitted),
+                                                        truth_ptr, C.byref(out), stream),
+                   "acts_kf_track_states_device")
 
     def sync(self) -> None:
         capi.check(self.lib.acts_kf_sync(self.ctx), "acts_kf_sync")

@@ -126,6 +126,63 @@ def timed_fits(fitter, bin_, bout, stream, steps, warmup):
     return start.elapsed_time(end) / steps, wall, fwd
 
 
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+# algorithmic bytes of the track-state table kernel: every slot writes its
+# row (154 float + 4 int columns) and reads its track lookup; a state reads
+# three parameter vectors and covariances (1008 B) and its surface, flags,
+# path, chi2 and measurement index (24 B); a state with a measurement also
+# reads the measurement (48 B) and the truth (48 B)
+TS_SLOT_BYTES = 154 * 4 + 4 * 4 + 16
+TS_STATE_BYTES = 3 * (6 + 36) * 8 + 24
+TS_LINK_BYTES = 48 + 48
+
+
+def track_states_bench(det, field, opts, events, dev_in, stream, steps):
+    """Time the track-state table kernel (RootTrackStatesWriter's columns)
+    on one fitted batch: HBM-bound, priced against the HBM peak."""
+    import torch
+    import acts_amd as A
+    from acts_amd import capi
+    from acts_amd.device import DeviceResult, to_device
+    outs = ("state_surface", "state_flags", "state_path", "state_chi2", "state_measurement",
+            "predicted", "predicted_cov", "filtered", "filtered_cov", "smoothed",
+            "smoothed_cov")
+    dev = stream.device
+    res = DeviceResult(events.n_tracks, dev_in.n_slots, outs, dev)
+    fitter = A.KalmanFitter(det, field, opts)
+    fitter.fit_device(dev_in.batch_in(), res.batch_out(), stream.cuda_stream)
+    truth = to_device(events.truth_meas, dev)
+    floats = torch.empty((capi.TS_NFLOAT, dev_in.n_slots), dtype=torch.float32, device=dev)
+    ints = torch.empty((capi.TS_NINT, dev_in.n_slots), dtype=torch.int32, device=dev)
+    o = capi.TrackStatesOut()
+    o.n_slots, o.columns, o.ints = dev_in.n_slots, floats.data_ptr(), ints.data_ptr()
+    bin_, bout = dev_in.batch_in(), res.batch_out()
+
+    def launch():
+        fitter.track_states_device(bin_, bout, truth.data_ptr(), o, stream.cuda_stream)
+    for _ in range(3):
+        launch()
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        launch()
+    end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end) / steps
+    fitter.close()
+    st = ints[capi.TS_STATE_TYPE].cpu().numpy()
+    n_states = int(np.count_nonzero(st >= 0))
+    n_link = int(np.count_nonzero(ints[capi.TS_DIM_HIT].cpu().numpy() > 0))
+    nbytes = (dev_in.n_slots * TS_SLOT_BYTES + n_states * TS_STATE_BYTES
+              + n_link * TS_LINK_BYTES)
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    return {"kernel": "trackStatesKernel", "ms": ms, "states": n_states,
+            "states_per_s": n_states / (ms * 1e-3), "bytes_per_launch": nbytes,
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS}}
+
+
 def main() -> int:
     args = parse()
     if args.gpus != 1:
@@ -231,6 +288,9 @@ def main() -> int:
         "extrapolation_failures": int(np.count_nonzero(outp.host("status")) -
                                       np.count_nonzero(r.status))}
     del outp
+
+    line["track_states"] = track_states_bench(det, field, wl_opts, events, dev_in, stream,
+                                              args.steps)
 
     first64 = quality.first_states(out64, events)
     # the MbfSmoother (the Examples' default smoother) in place of the RTS one

@@ -263,6 +263,11 @@ typedef struct acts_kf_batch_out {
    * covariance on it (TrackProxy::parameters()/covariance())              */
   double* reference_params; /* [n][6] */
   double* reference_cov;    /* [n][36] */
+  /* [slots] the state's measurement as an index into the batch's
+   * measurement arrays (meas_surfaces/meas_values/meas_cov), i.e. its
+   * uncalibrated source link (IndexSourceLink::index); -1 for hole and
+   * material-only states                                                   */
+  int32_t* state_measurement;
 } acts_kf_batch_out;
 
 typedef struct acts_kf_ctx acts_kf_ctx;
@@ -310,6 +315,70 @@ void acts_kf_destroy(acts_kf_ctx* ctx);
  * smoother kernel of the context's last fit (needs options.kernel_timing;
  * waits for that fit).  No reference counterpart: a profiling aid.          */
 int acts_kf_kernel_times(acts_kf_ctx* ctx, double* forward_ms, double* smoother_ms);
+
+/* ---- track-state table ---------------------------------------------------
+ * The per-state columns RootTrackStatesWriter derives from a fitted track
+ * (Examples/Io/Root/src/RootTrackStatesWriter.cpp:364-735): for each of the
+ * predicted, filtered, smoothed and unbiased parameters (the last from
+ * calculateUnbiasedParametersCovariance, TrackHelpersUnbiased.cpp:21-54)
+ * the values, their errors, residuals and pulls against the truth, the
+ * global position and momentum, eta and pT; the hit's local and global
+ * position and its residual, error and pull against the prediction; the
+ * truth at the hit.  Values are computed in double and stored as float
+ * through clampValue<float> (Helpers.hpp:170-188), NaN where the writer
+ * pushes NaN.  One row per state slot (the sequence layout of
+ * acts_kf_batch_out), stored column-major: column c of slot i is
+ * columns[c * n_slots + i].  Rows of failed tracks and unused slots get
+ * state type -1.  Where the writer pushes nothing for a state (residuals
+ * and pulls of states without a measurement; hit residuals of states
+ * without a measurement) the column holds NaN and the host-side writer
+ * skips it.                                                                 */
+#define ACTS_KF_TS_PREDICTED 0
+#define ACTS_KF_TS_FILTERED 1
+#define ACTS_KF_TS_SMOOTHED 2
+#define ACTS_KF_TS_UNBIASED 3
+/* float columns */
+#define ACTS_KF_TS_CHI2 0        /* chi2 */
+#define ACTS_KF_TS_PATH_LENGTH 1 /* pathLength */
+#define ACTS_KF_TS_T_X 2         /* t_x t_y t_z t_r t_dx t_dy t_dz (2..8) */
+#define ACTS_KF_TS_T_PARAMS 9    /* t_eLOC0 .. t_eT (9..14) */
+#define ACTS_KF_TS_L_X_HIT 15    /* l_x_hit l_y_hit g_x_hit g_y_hit g_z_hit */
+#define ACTS_KF_TS_RES_X_HIT 20  /* res_x_hit res_y_hit err_x_hit err_y_hit
+                                    pull_x_hit pull_y_hit (20..25) */
+/* per parameter set p (ACTS_KF_TS_PREDICTED..UNBIASED), 32 columns from
+ * ACTS_KF_TS_PARAMS(p): eLOC0..eT (+0..5), res_ (+6..11), err_ (+12..17),
+ * pull_ (+18..23), g_x g_y g_z (+24..26), px py pz (+27..29), eta (+30),
+ * pT (+31)                                                                  */
+#define ACTS_KF_TS_PARAMS(p) (26 + 32 * (p))
+#define ACTS_KF_TS_NFLOAT 154
+/* int columns */
+#define ACTS_KF_TS_STATE_TYPE 0 /* stateType: 0 measurement, 1 outlier, 2 hole,
+                                   3 material, 4 unknown; -1 no state */
+#define ACTS_KF_TS_DIM_HIT 1    /* dim_hit (calibrated size, 0 without) */
+#define ACTS_KF_TS_HAS_PARAMS 2 /* bit p set: parameter set p exists */
+#define ACTS_KF_TS_TRACK 3      /* index of the state's track in the batch */
+#define ACTS_KF_TS_NINT 4
+
+typedef struct acts_kf_track_states_out {
+  int64_t n_slots;  /* column stride, >= seq_offsets[n_tracks] */
+  float* columns;   /* [ACTS_KF_TS_NFLOAT][n_slots] */
+  int32_t* ints;    /* [ACTS_KF_TS_NINT][n_slots] */
+} acts_kf_track_states_out;
+
+/* Fill the track-state table of a fitted batch.  All buffers are device
+ * memory on the context's device: `in` is the batch that was fitted,
+ * `fitted` its outputs, which must hold status, n_states, state_surface,
+ * state_flags, state_path, state_chi2, state_measurement and the
+ * predicted, filtered and smoothed parameters and covariances.
+ * `truth_meas` ([m][6] bound parameters of the truth particle at each
+ * measurement, the writer's averaged sim hits) may be NULL: the truth
+ * columns and the residuals and pulls are then NaN.  Asynchronous on
+ * `stream`.  Replaces the per-state loop of RootTrackStatesWriter::writeT
+ * (RootTrackStatesWriter.cpp:310-740); the host-side writer orders the
+ * rows as the reference's trackStatesReversed() vectors.                    */
+int acts_kf_track_states_device(acts_kf_ctx* ctx, const acts_kf_batch_in* in,
+                                const acts_kf_batch_out* fitted, const double* truth_meas,
+                                const acts_kf_track_states_out* out, void* stream);
 
 /* Message for a per-track status code: the reference's error messages
  * (std::error_category::message of KalmanFitterError.cpp:21,

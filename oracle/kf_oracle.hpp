@@ -401,4 +401,10 @@ int propagateToSurface(const Geometry& geo, const Vec6& startParams,
 // reference surface; returns a status
 int extrapolateToReference(const Geometry& geo, TrackOut& out);
 
+// The per-state columns of RootTrackStatesWriter for a fitted batch, in
+// the layout of acts_kf_track_states_out (track_states.cpp).
+void trackStatesTable(const Geometry& geo, const acts_kf_batch_in& in,
+                      const acts_kf_batch_out& fitted, const double* truthMeas,
+                      const acts_kf_track_states_out& out);
+
 }  // namespace acts_oracle

@@ -47,6 +47,12 @@ def load():
                                     C.POINTER(capi.Field), C.POINTER(capi.Options),
                                     C.POINTER(capi.BatchIn), C.POINTER(capi.BatchOut),
                                     C.c_int32, dp]
+    lib.acts_oracle_track_states.restype = C.c_int
+    lib.acts_oracle_track_states.argtypes = [C.POINTER(capi.Surface), C.c_int32,
+                                             C.POINTER(capi.MaterialSlab), C.c_int32,
+                                             C.POINTER(capi.Field), C.POINTER(capi.Options),
+                                             C.POINTER(capi.BatchIn), C.POINTER(capi.BatchOut),
+                                             dp, C.POINTER(capi.TrackStatesOut)]
     lib.acts_oracle_gain_update.restype = C.c_int
     lib.acts_oracle_gain_update.argtypes = [dp, dp, dp, dp, C.c_int32, dp, dp, dp]
     lib.acts_oracle_smooth.restype = C.c_int
@@ -210,3 +216,28 @@ def propagate(field, options, start_plane, start_params, start_cov, target, dire
                                    direction, _dp(ep), _dp(ec), _dp(jac), _dp(path),
                                    C.byref(nsteps))
     return rc, ep, ec.reshape(6, 6), jac.reshape(6, 6), float(path[0]), nsteps.value
+
+
+def track_states(detector, field, options, events, fitted, truth=None):
+    """The track-state table of a fitted batch (FitResult `fitted` with the
+    per-state outputs the table needs) on the CPU: a TrackStatesTable."""
+    from acts_amd.track_states import TrackStatesTable
+    lib = load()
+    table = TrackStatesTable.empty(int(events.seq_offsets[-1]))
+    surf = detector.surface_table()
+    mats = detector.material_table()
+    fld = field.struct()
+    opt = options.struct() if hasattr(options, "struct") else options
+    bin_ = events.batch_in()
+    bout = fitted.batch_out()
+    t = None
+    if truth is not None:
+        truth = np.ascontiguousarray(truth, dtype=np.float64)
+        t = _dp(truth)
+    o = table.out_struct()
+    rc = lib.acts_oracle_track_states(surf, len(detector.surfaces), mats,
+                                      len(detector.materials), C.byref(fld), C.byref(opt),
+                                      C.byref(bin_), C.byref(bout), t, C.byref(o))
+    if rc != 0:
+        raise RuntimeError(f"acts_oracle_track_states failed: {rc}")
+    return table

@@ -122,6 +122,9 @@ void storeTrack(const acts_kf_batch_in* in, const acts_kf_batch_out* out,
     if (out->smoothed) store(out->smoothed + 6 * slot, s.smoothed);
     if (out->smoothed_cov) store(out->smoothed_cov + 36 * slot, s.smoothedCov);
     if (out->jacobian) store(out->jacobian + 36 * slot, s.jacobian);
+    if (out->state_measurement)
+      out->state_measurement[slot] =
+          s.meas >= 0 ? static_cast<int32_t>(in->meas_offsets[t] + s.meas) : -1;
   }
   if (tr.hasReference) {
     if (out->reference_params) store(out->reference_params + 6 * t, tr.referenceParams);
@@ -169,6 +172,20 @@ int acts_oracle_fit(const acts_kf_surface* surfaces, int32_t n_surfaces,
   }
   auto t1 = std::chrono::steady_clock::now();
   if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return ACTS_KF_OK;
+}
+
+// The track-state table of a fitted batch (host buffers throughout).
+int acts_oracle_track_states(const acts_kf_surface* surfaces, int32_t n_surfaces,
+                             const acts_kf_material_slab* materials, int32_t n_materials,
+                             const acts_kf_field* field, const acts_kf_options* options,
+                             const acts_kf_batch_in* in, const acts_kf_batch_out* fitted,
+                             const double* truth_meas, const acts_kf_track_states_out* out) {
+  if (!surfaces || !field || !options || !in || !fitted || !out) return ACTS_KF_ERR_INVALID_ARGUMENT;
+  if (in->n_tracks <= 0 || out->n_slots <= 0) return ACTS_KF_OK;
+  Geometry geo;
+  buildGeometry(geo, surfaces, n_surfaces, materials, n_materials, field, options);
+  trackStatesTable(geo, *in, *fitted, truth_meas, *out);
   return ACTS_KF_OK;
 }
 
