@@ -122,10 +122,9 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
 }
 
 // Track-state table: one thread per row
-__global__ __launch_bounds__(256) void trackStatesKernel(TsArgs a) {
-  const int64_t slot = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (slot >= a.nSlots) return;
-  trackStateRow(a, slot);
+__global__ __launch_bounds__(kTsBlock) void trackStatesKernel(TsArgs a) {
+  const int64_t slot = static_cast<int64_t>(blockIdx.x) * kTsBlock + threadIdx.x;
+  if (slot < a.nSlots) trackStateRow(a, slot);
 }
 
 // Smoother (RTS, or MBF with Mbf) + track summary + outputs: one lane per
@@ -1010,8 +1009,9 @@ int acts_kf_track_states_device(acts_kf_ctx* c, const acts_kf_batch_in* in,
   a.geo = c->geo;
   a.cols = out->columns;
   a.ints = out->ints;
-  const int64_t blocks = (out->n_slots + 255) / 256;
-  hipLaunchKernelGGL(trackStatesKernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, a);
+  const int64_t blocks = (out->n_slots + kTsBlock - 1) / kTsBlock;
+  hipLaunchKernelGGL(trackStatesKernel, dim3(static_cast<unsigned>(blocks)), dim3(kTsBlock), 0, s,
+                     a);
   KF_HIP(hipGetLastError());
   return ACTS_KF_OK;
 }

@@ -39,11 +39,13 @@ struct TsArgs {
 
 // clampValue<float>(double) (Helpers.hpp:170-188): infinities and NaN kept,
 // finite values clamped to the float range, then rounded to nearest
+// Branch-free: the conversion keeps infinities and NaN and rounds finite
+// values to nearest; a finite value that rounds past the float range is
+// the clamped one, +-FLT_MAX (values that round down to FLT_MAX are
+// FLT_MAX either way).
 KF_HD float clampF(double v) {
-  if (isinf(v)) return v > 0 ? INFINITY : -INFINITY;
-  if (isnan(v)) return NAN;
-  const double lo = -static_cast<double>(FLT_MAX), hi = static_cast<double>(FLT_MAX);
-  return static_cast<float>(v < lo ? lo : (hi < v ? hi : v));
+  const float f = static_cast<float>(v);
+  return (isinf(f) && !isinf(v)) ? copysignf(FLT_MAX, f) : f;
 }
 
 // detail::difference_periodic (periodic.hpp:38-50)
@@ -65,21 +67,23 @@ KF_HD double etaOf(const V3& v) {
 
 // calculateUnbiasedParametersCovariance for a 2D measurement on (loc0,
 // loc1) (TrackHelpersUnbiased.cpp:37-52): K = C H^T (H C H^T - V)^-1,
-// s + K (m - H s), C - K H C; the 2x2 inverse in Eigen's closed form.
+// s + K (m - H s), C - K H C, with the 2x2 inverse in Eigen's closed form.
+// The table needs the parameters and the diagonal of the covariance; C is
+// read at rows 0-1, columns 0-1 and the diagonal (row-major 6x6).
 KF_HD void unbiasedState(const double* s, const double* C, const double* m, const double* V,
-                         double* su, double* Cu) {
+                         double* su, double* du) {
   const double S00 = C[0] - V[0], S01 = C[1] - V[1];
   const double S10 = C[6] - V[2], S11 = C[7] - V[3];
   const double det = S00 * S11 - S10 * S01;
   const double invdet = 1.0 / det;
   const double Si[2][2] = {{S11 * invdet, -S01 * invdet}, {-S10 * invdet, S00 * invdet}};
-  double K[6][2];
-  for (int i = 0; i < 6; ++i)
-    for (int j = 0; j < 2; ++j) K[i][j] = C[6 * i] * Si[0][j] + C[6 * i + 1] * Si[1][j];
   const double r0 = m[0] - s[0], r1 = m[1] - s[1];
-  for (int i = 0; i < 6; ++i) su[i] = s[i] + (K[i][0] * r0 + K[i][1] * r1);
-  for (int i = 0; i < 6; ++i)
-    for (int k = 0; k < 6; ++k) Cu[6 * i + k] = C[6 * i + k] - (K[i][0] * C[k] + K[i][1] * C[6 + k]);
+  for (int i = 0; i < 6; ++i) {
+    const double K0 = C[6 * i] * Si[0][0] + C[6 * i + 1] * Si[1][0];
+    const double K1 = C[6 * i] * Si[0][1] + C[6 * i + 1] * Si[1][1];
+    su[i] = s[i] + (K0 * r0 + K1 * r1);
+    du[i] = C[7 * i] - (K0 * C[i] + K1 * C[6 + i]);
+  }
 }
 
 // The 32 columns of one parameter set (values, residuals, errors, pulls,
@@ -87,19 +91,20 @@ KF_HD void unbiasedState(const double* s, const double* C, const double* m, cons
 // is null when the state has no measurement or no truth was given: the
 // residuals and pulls are then NaN.
 template <typename Store>
-KF_HD void paramColumns(const DevSurface& s, const double* x, const double* C,
+KF_HD void paramColumns(const DevSurface& s, const double* x, const double* diag,
                         const double* truth, Store&& put, int base) {
   double err[6];
   for (int i = 0; i < 6; ++i) {
-    const double var = C[7 * i];
+    const double var = diag[i];
     err[i] = var >= 0 ? sqrt(var) : static_cast<double>(NAN);
     put(base + i, clampF(x[i]));
     put(base + 12 + i, clampF(err[i]));
   }
   // transformBoundToFreeParameters: direction from (phi, theta), position
   // from the surface's localToGlobal
-  const double sphi = sin(x[2]), cphi = cos(x[2]);
-  const double sth = sin(x[3]), cth = cos(x[3]);
+  double sphi, cphi, sth, cth;
+  sincos(x[2], &sphi, &cphi);
+  sincos(x[3], &sth, &cth);
   const V3 dir = mk(cphi * sth, sphi * sth, cth);
   const V3 pos = localToGlobal(s, x[0], x[1]);
   const double p = fabs(1 / x[4]);
@@ -122,97 +127,114 @@ KF_HD void paramColumns(const DevSurface& s, const double* x, const double* C,
   }
 }
 
-// One row of the table: everything RootTrackStatesWriter derives from one
-// track state (the body of its state loop, RootTrackStatesWriter.cpp:
-// 364-735).
+constexpr int kTsBlock = 256;
+
+// One row of the table, for slot `slot`: everything RootTrackStatesWriter
+// derives from one track state (the body of its state loop,
+// RootTrackStatesWriter.cpp:364-735).  The row is computed in FP64 and
+// stored as float: the cost is the FP64 transcendental and division work
+// of four parameter sets (~5 k lane instructions per state), not the
+// 1.8 KB it moves.
 __device__ inline void trackStateRow(const TsArgs& a, int64_t slot) {
+  const int64_t nSeq = a.seqOff[a.nTracks];
+  const bool inTable = slot < a.nSlots;
   auto put = [&](int c, float v) { a.cols[c * a.nSlots + slot] = v; };
   auto puti = [&](int c, int32_t v) { a.ints[c * a.nSlots + slot] = v; };
   // the slot's track: the last t with seqOff[t] <= slot
-  int64_t lo = 0, hi = a.nTracks;
-  while (hi - lo > 1) {
-    const int64_t mid = (lo + hi) >> 1;
-    if (a.seqOff[mid] <= slot) lo = mid; else hi = mid;
+  int64_t t = -1;
+  if (inTable && slot < nSeq) {
+    int64_t lo = 0, hi = a.nTracks;
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (a.seqOff[mid] <= slot) lo = mid; else hi = mid;
+    }
+    t = lo;
   }
-  const int64_t t = lo;
-  const int64_t k = slot - a.seqOff[t];
-  const bool inSeq = slot < a.seqOff[a.nTracks] && k >= 0;
+  const int64_t k = t < 0 ? -1 : slot - a.seqOff[t];
+  const bool inSeq = t >= 0 && k >= 0 && slot < a.seqOff[t + 1];
   bool ok = inSeq && a.status[t] == 0 && k < a.nStates[t];
   // indices taken from the fit's outputs are checked before any lookup
   const int32_t sid = ok ? a.stateSurf[slot] : -1;
   const int32_t mi = ok ? a.stateMeas[slot] : -1;
   ok = ok && sid >= 0 && sid < a.nSurfaces && mi < a.measOff[a.nTracks];
-  puti(ACTS_KF_TS_TRACK, inSeq ? static_cast<int32_t>(t) : -1);
-  if (!ok) {
+  const bool link = ok && mi >= 0;
+  if (inTable) puti(ACTS_KF_TS_TRACK, inSeq ? static_cast<int32_t>(t) : -1);
+  if (inTable && !ok) {
     puti(ACTS_KF_TS_STATE_TYPE, -1);
     puti(ACTS_KF_TS_DIM_HIT, 0);
     puti(ACTS_KF_TS_HAS_PARAMS, 0);
     for (int c = 0; c < ACTS_KF_TS_NFLOAT; ++c) put(c, NAN);
-    return;
   }
-  const uint32_t flags = a.stateFlags[slot];
-  // RootTrackStatesWriter::getStateType (RootTrackStatesWriter.cpp:293-308)
-  const int32_t type = (flags & ACTS_KF_STATE_IS_OUTLIER)       ? 1
-                       : (flags & ACTS_KF_STATE_HAS_MEASUREMENT) ? 0
-                       : (flags & ACTS_KF_STATE_IS_HOLE)         ? 2
-                       : (flags & ACTS_KF_STATE_HAS_MATERIAL)    ? 3
-                                                                 : 4;
-  const bool link = mi >= 0;
-  const DevSurface& s = a.geo[sid];
-  puti(ACTS_KF_TS_STATE_TYPE, type);
-  puti(ACTS_KF_TS_DIM_HIT, link ? 2 : 0);
-  puti(ACTS_KF_TS_HAS_PARAMS, link ? 0xF : 0x7);
-  put(ACTS_KF_TS_CHI2, a.stateChi2[slot]);
-  put(ACTS_KF_TS_PATH_LENGTH, static_cast<float>(a.statePath[slot]));
-
+  const DevSurface& s = a.geo[ok ? sid : 0];
   const double* truth = (link && a.truth) ? a.truth + 6 * static_cast<int64_t>(mi) : nullptr;
-  // truth at the hit: the truth particle's bound state on the surface
-  if (truth) {
-    const double sphi = sin(truth[2]), cphi = cos(truth[2]);
-    const double sth = sin(truth[3]), cth = cos(truth[3]);
-    const V3 tdir = mk(cphi * sth, sphi * sth, cth);
-    const V3 tpos = localToGlobal(s, truth[0], truth[1]);
-    put(ACTS_KF_TS_T_X + 0, clampF(tpos.x));
-    put(ACTS_KF_TS_T_X + 1, clampF(tpos.y));
-    put(ACTS_KF_TS_T_X + 2, clampF(tpos.z));
-    put(ACTS_KF_TS_T_X + 3, clampF(perp(tpos)));
-    put(ACTS_KF_TS_T_X + 4, clampF(tdir.x));
-    put(ACTS_KF_TS_T_X + 5, clampF(tdir.y));
-    put(ACTS_KF_TS_T_X + 6, clampF(tdir.z));
-    for (int i = 0; i < 6; ++i) put(ACTS_KF_TS_T_PARAMS + i, clampF(truth[i]));
-  } else {
-    for (int c = ACTS_KF_TS_T_X; c < ACTS_KF_TS_L_X_HIT; ++c) put(c, NAN);
-  }
-  // the hit: local and global position (plane, disc and cylinder
-  // localToGlobal do not depend on the direction)
   double m[2] = {0, 0}, V[4] = {0, 0, 0, 0};
-  if (link) {
-    for (int i = 0; i < 2; ++i) m[i] = a.measVal[2 * static_cast<int64_t>(mi) + i];
-    for (int i = 0; i < 4; ++i) V[i] = a.measCov[4 * static_cast<int64_t>(mi) + i];
-    const V3 g = localToGlobal(s, m[0], m[1]);
-    put(ACTS_KF_TS_L_X_HIT + 0, clampF(m[0]));
-    put(ACTS_KF_TS_L_X_HIT + 1, clampF(m[1]));
-    put(ACTS_KF_TS_L_X_HIT + 2, clampF(g.x));
-    put(ACTS_KF_TS_L_X_HIT + 3, clampF(g.y));
-    put(ACTS_KF_TS_L_X_HIT + 4, clampF(g.z));
-  } else {
-    for (int c = ACTS_KF_TS_L_X_HIT; c < ACTS_KF_TS_RES_X_HIT; ++c) put(c, NAN);
+  if (ok) {
+    const uint32_t flags = a.stateFlags[slot];
+    // RootTrackStatesWriter::getStateType (RootTrackStatesWriter.cpp:293-308)
+    const int32_t type = (flags & ACTS_KF_STATE_IS_OUTLIER)       ? 1
+                         : (flags & ACTS_KF_STATE_HAS_MEASUREMENT) ? 0
+                         : (flags & ACTS_KF_STATE_IS_HOLE)         ? 2
+                         : (flags & ACTS_KF_STATE_HAS_MATERIAL)    ? 3
+                                                                   : 4;
+    puti(ACTS_KF_TS_STATE_TYPE, type);
+    puti(ACTS_KF_TS_DIM_HIT, link ? 2 : 0);
+    puti(ACTS_KF_TS_HAS_PARAMS, link ? 0xF : 0x7);
+    put(ACTS_KF_TS_CHI2, a.stateChi2[slot]);
+    put(ACTS_KF_TS_PATH_LENGTH, static_cast<float>(a.statePath[slot]));
+    // truth at the hit: the truth particle's bound state on the surface
+    if (truth) {
+      double sphi, cphi, sth, cth;
+      sincos(truth[2], &sphi, &cphi);
+      sincos(truth[3], &sth, &cth);
+      const V3 tdir = mk(cphi * sth, sphi * sth, cth);
+      const V3 tpos = localToGlobal(s, truth[0], truth[1]);
+      put(ACTS_KF_TS_T_X + 0, clampF(tpos.x));
+      put(ACTS_KF_TS_T_X + 1, clampF(tpos.y));
+      put(ACTS_KF_TS_T_X + 2, clampF(tpos.z));
+      put(ACTS_KF_TS_T_X + 3, clampF(perp(tpos)));
+      put(ACTS_KF_TS_T_X + 4, clampF(tdir.x));
+      put(ACTS_KF_TS_T_X + 5, clampF(tdir.y));
+      put(ACTS_KF_TS_T_X + 6, clampF(tdir.z));
+      for (int i = 0; i < 6; ++i) put(ACTS_KF_TS_T_PARAMS + i, clampF(truth[i]));
+    } else {
+      for (int c = ACTS_KF_TS_T_X; c < ACTS_KF_TS_L_X_HIT; ++c) put(c, NAN);
+    }
+    // the hit: local and global position (plane, disc and cylinder
+    // localToGlobal do not depend on the direction)
+    if (link) {
+      for (int i = 0; i < 2; ++i) m[i] = a.measVal[2 * static_cast<int64_t>(mi) + i];
+      for (int i = 0; i < 4; ++i) V[i] = a.measCov[4 * static_cast<int64_t>(mi) + i];
+      const V3 g = localToGlobal(s, m[0], m[1]);
+      put(ACTS_KF_TS_L_X_HIT + 0, clampF(m[0]));
+      put(ACTS_KF_TS_L_X_HIT + 1, clampF(m[1]));
+      put(ACTS_KF_TS_L_X_HIT + 2, clampF(g.x));
+      put(ACTS_KF_TS_L_X_HIT + 3, clampF(g.y));
+      put(ACTS_KF_TS_L_X_HIT + 4, clampF(g.z));
+    } else {
+      for (int c = ACTS_KF_TS_L_X_HIT; c < ACTS_KF_TS_RES_X_HIT; ++c) put(c, NAN);
+    }
   }
 
-  double x[6], C[36];
+  if (!ok) return;
+  // the predicted, filtered and smoothed sets; the unbiased set is formed
+  // from the smoothed one (not unrolled: one copy of the transcendental
+  // calls)
+  double x[6], d[6], xu[6], du[6];
+#pragma unroll 1
   for (int p = 0; p < 3; ++p) {
+    const double* C = a.cov[p] + 36 * slot;
     const double* xp = a.par[p] + 6 * slot;
-    const double* Cp = a.cov[p] + 36 * slot;
     for (int i = 0; i < 6; ++i) x[i] = xp[i];
-    for (int i = 0; i < 36; ++i) C[i] = Cp[i];
-    paramColumns(s, x, C, truth, put, ACTS_KF_TS_PARAMS(p));
+    for (int i = 0; i < 6; ++i) d[i] = C[7 * i];
+    if (p == ACTS_KF_TS_SMOOTHED && link) unbiasedState(x, C, m, V, xu, du);
+    paramColumns(s, x, d, truth, put, ACTS_KF_TS_PARAMS(p));
     if (p == ACTS_KF_TS_PREDICTED) {
       // hit residual against the prediction: res = m - H x, resCov = V + H C H^T
       if (link) {
         const double resX = m[0] - x[0], resY = m[1] - x[1];
         const double errX = V[0] >= 0 ? sqrt(V[0]) : static_cast<double>(NAN);
         const double errY = V[3] >= 0 ? sqrt(V[3]) : static_cast<double>(NAN);
-        const double rcX = V[0] + C[0], rcY = V[3] + C[7];
+        const double rcX = V[0] + d[0], rcY = V[3] + d[1];
         const double pullX = rcX > 0 ? resX / sqrt(rcX) : static_cast<double>(NAN);
         const double pullY = rcY > 0 ? resY / sqrt(rcY) : static_cast<double>(NAN);
         put(ACTS_KF_TS_RES_X_HIT + 0, clampF(resX));
@@ -226,11 +248,8 @@ __device__ inline void trackStateRow(const TsArgs& a, int64_t slot) {
       }
     }
   }
-  // unbiased: the smoothed state (still in x, C) with the measurement removed
   if (link) {
-    double xu[6], Cu[36];
-    unbiasedState(x, C, m, V, xu, Cu);
-    paramColumns(s, xu, Cu, truth, put, ACTS_KF_TS_PARAMS(ACTS_KF_TS_UNBIASED));
+    paramColumns(s, xu, du, truth, put, ACTS_KF_TS_PARAMS(ACTS_KF_TS_UNBIASED));
   } else {
     for (int c = ACTS_KF_TS_PARAMS(ACTS_KF_TS_UNBIASED); c < ACTS_KF_TS_NFLOAT; ++c) put(c, NAN);
   }

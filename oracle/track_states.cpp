@@ -179,7 +179,7 @@ void trackStatesTable(const Geometry& geo, const acts_kf_batch_in& in,
       t = lo;
     }
     const int64_t k = slot - in.seq_offsets[t];
-    const bool inSeq = slot < in.seq_offsets[n] && k >= 0;
+    const bool inSeq = slot < in.seq_offsets[n] && k >= 0 && slot < in.seq_offsets[t + 1];
     row.puti(ACTS_KF_TS_TRACK, inSeq ? static_cast<int32_t>(t) : -1);
     if (!inSeq || f.status[t] != 0 || k >= f.n_states[t]) {
       emptyRow(row);
