@@ -135,6 +135,8 @@ class Options(C.Structure):
         ("sched_quarantine_steps", C.c_int32),
         ("kernel_timing", C.c_int32),
         ("use_graph", C.c_int32),
+        ("sched_transport_skip_steps", C.c_int32),
+        ("reserved_sched", C.c_int32),
     ]
 
 

@@ -113,6 +113,13 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
       const int nWait = __popcll(__ballot(phase == kPhaseSurface || phase == kPhaseFinished));
       if (nStep == 0 || 64 * nWait >= waitFrac * (nStep + nWait)) break;
     }
+    // a lane that reached a surface while stepping without the transport
+    // goes back to its checkpoint and repeats the steps with it
+    if (phase == kPhaseSurface && L.noTransport) {
+      L = a.ckpt[static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x];
+      L.replaying = true;
+      phase = kPhaseStepping;
+    }
     if (phase == kPhaseSurface) phase = laneFilter<Mixed>(L, a, geo, slabs);
     if (phase == kPhaseFinished) {
       laneFinish(L, a);
@@ -364,6 +371,10 @@ struct acts_kf_ctx {
   int32_t waitFrac = 64;
   int32_t quarantine = 0;
   bool mbf = false;  // MbfSmoother instead of GainMatrixSmoother
+  // lanes stepping without the transport save their state here, one Lane
+  // per resident lane of the forward kernel
+  int32_t skipAfter = 64;
+  DevBuf ckptBuf;
   // reference surface of the fitted parameters, and the default propagator
   // and stepper options its extrapolation runs with
   int32_t refSurface = ACTS_KF_REFERENCE_NONE;
@@ -416,6 +427,7 @@ void acts_kf_default_options(acts_kf_options* o) {
   o->mass = 0.1056583755f;
   o->abs_charge = 1.0f;
   o->length_sorted = 1;
+  o->sched_transport_skip_steps = 64;
 }
 
 const char* acts_kf_last_error(void) { return g_lastError.c_str(); }
@@ -631,6 +643,7 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   if (options->sched_inner_steps > 0) c->innerSteps = options->sched_inner_steps;
   if (options->sched_wait_64ths > 0) c->waitFrac = std::min(options->sched_wait_64ths, 64);
   c->quarantine = std::max(options->sched_quarantine_steps, 0);
+  c->skipAfter = std::max(options->sched_transport_skip_steps, 0);
   {
     int blocksPerCu = 0, cus = 0;
     const hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -643,6 +656,11 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
       return fail(ACTS_KF_ERR_DEVICE, "occupancy query failed");
     }
     c->residentWaves = static_cast<int64_t>(blocksPerCu) * cus;
+    if (c->skipAfter > 0 &&
+        c->ckptBuf.ensure(static_cast<size_t>(c->residentWaves) * kBlock * sizeof(Lane))) {
+      acts_kf_destroy(c);
+      return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "acts_kf_create: checkpoint buffer");
+    }
   }
   *outCtx = c;
   return ACTS_KF_OK;
@@ -770,6 +788,9 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   fa.pc = c->pc;
   fa.nSurfaces = c->nSurfaces;
   fa.nSlabs = c->nSlabs;
+  fa.ckpt = static_cast<Lane*>(c->ckptBuf.p);
+  fa.skipAfter = c->ckptBuf.p ? c->skipAfter : 0;
+  fa.pad = 0;
   int32_t* queue = static_cast<int32_t*>(c->queueBuf.p);
   hipLaunchKernelGGL(zeroKernel, dim3(1), dim3(64), 0, stream, queue, 1);
   const int64_t waves = std::min(blocks, c->residentWaves);
@@ -1045,6 +1066,7 @@ void acts_kf_destroy(acts_kf_ctx* c) {
   c->scratchI.release();
   c->scratchT.release();
   c->refBuf.release();
+  c->ckptBuf.release();
   c->lenBuf.release();
   c->permBuf.release();
   c->histBuf.release();

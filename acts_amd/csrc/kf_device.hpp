@@ -588,13 +588,15 @@ __device__ inline void rkApply(Track& T, const DevOpts& o, const RkStage& S) {
 
 // EigenStepper::step (EigenStepper.ipp:169-388) + transportMatrix
 // (EigenStepperDefaultExtension.hpp:137-228), propagation direction +1.
+// Without `transport` the step leaves the transport blocks (tr, tr37, br)
+// alone: the trajectory and the step-size control do not depend on them.
 __device__ inline int rkStep(Track& T, const DevField& F, const DevOpts& o,
-                             double propDir = 1.0) {
+                             double propDir = 1.0, bool transport = true) {
   RkStage S;
   if (const int st = rkTrials(T, F, o, S, propDir)) return st;
   double p;
   const double dtds = rkTime(T, o, S, p);
-  rkTransport(T, o, S, p, dtds);
+  if (transport) rkTransport(T, o, S, p, dtds);
   rkApply(T, o, S);
   return 0;
 }

@@ -58,6 +58,10 @@ struct Lane {
   int status;
   uint32_t iter;
   int reached;
+  // steps without the jacobian transport (see FwdArgs::skipAfter)
+  uint32_t lastSurfIter;  // step count at the last surface
+  bool noTransport;       // stepping without transport since a checkpoint
+  bool replaying;         // back at the checkpoint: steps with transport
 };
 
 struct FwdArgs {
@@ -71,6 +75,13 @@ struct FwdArgs {
   PhysConst pc;
   int32_t nSurfaces;
   int32_t nSlabs;
+  // A track that has taken skipAfter steps since its last surface copies its
+  // lane to ckpt[lane id] and steps on without the transport; if it then
+  // reaches a surface, the lane is restored and the steps are repeated with
+  // the transport (same trajectory, same results).  0 = never.
+  Lane* ckpt;
+  int32_t skipAfter;
+  int32_t pad;
 };
 
 // EigenStepper::initialize (EigenStepper.ipp:37-70), DirectNavigator
@@ -97,6 +108,9 @@ __device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a,
   L.iter = 0;
   L.reached = -1;
   L.finished = false;
+  L.lastSurfIter = 0;
+  L.noTransport = false;
+  L.replaying = false;
   L.T.nSteps = 0;
   L.T.nTrials = 0;
   // surface indices come from the caller: reject the track if any is out of
@@ -199,8 +213,14 @@ __device__ inline int laneStep(Lane& L, const FwdArgs& a, const DevSurface* geo)
     L.status = ACTS_KF_STATUS_STEP_COUNT_LIMIT;
     return kPhaseFinished;
   }
+  if (a.skipAfter > 0 && !L.noTransport && !L.replaying &&
+      L.iter - L.lastSurfIter >= static_cast<uint32_t>(a.skipAfter)) {
+    a.ckpt[static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x] = L;
+    L.noTransport = true;
+  }
   ++L.iter;
-  const int sst = Mixed ? rkStepMixed(T, a.F, o) : rkStep(T, a.F, o);
+  const bool transport = !L.noTransport;
+  const int sst = Mixed ? rkStepMixed(T, a.F, o, 1.0, transport) : rkStep(T, a.F, o, 1.0, transport);
   if (sst) {
     L.status = sst;
     return kPhaseFinished;
@@ -248,6 +268,8 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
   const int reached = L.reached;
   const int64_t t = L.t;
   L.reached = -1;
+  L.lastSurfIter = L.iter;
+  L.replaying = false;
   if (!L.finished) {
     if (L.kfLimit == DBL_MAX) loopProtection<Mixed>(T, a.F, o, L.kfLimit, true);
     const DevSurface& s = geo[reached];

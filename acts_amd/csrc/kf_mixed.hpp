@@ -82,7 +82,7 @@ __device__ inline float stepScalingF(float tol, float err) {
 }
 
 __device__ inline int rkStepMixed(Track& T, const DevField& F, const DevOpts& o,
-                                  double propDir = 1.0) {
+                                  double propDir = 1.0, bool transport = true) {
   const F3 pos = tof(T.pos);
   const F3 dir = tof(T.dir);
   const float qop = static_cast<float>(T.qop);
@@ -141,8 +141,10 @@ __device__ inline int rkStepMixed(Track& T, const DevField& F, const DevOpts& o,
   T.time += hD * dtds;
   T.der[3] = dtds;
 
-  // transport matrix D (only the non-trivial blocks)
   const float h6 = h / 6.f;
+  // transport matrix D (only the non-trivial blocks); skipped without
+  // `transport` (the trajectory does not depend on it)
+  if (transport) {
   const float hh6 = (h * h) / 6.f;
   const float qh2 = qop * half_h;
   const float qh = qop * h;
@@ -222,6 +224,7 @@ __device__ inline int rkStepMixed(Track& T, const DevField& F, const DevOpts& o,
     }
   }
   T.tr37 += D37;
+  }
 
   // the straight-line part h*dir in double (in float it would carry an
   // absolute error of |h| * 6e-8, i.e. 6e-5 mm for a 1 m step, a 1e-5

@@ -208,6 +208,15 @@ typedef struct acts_kf_options {
    * later fits with the same buffers and sizes (no reference counterpart;
    * kernel_timing is ignored then)                                           */
   int32_t use_graph;
+  /* a track that has taken this many steps since its last surface saves its
+   * state and steps on without the jacobian transport; should it reach a
+   * surface after all, it goes back to the saved state and repeats those
+   * steps with the transport, so the results do not depend on it (a track
+   * that never reaches another surface, e.g. a looper that ends at
+   * max_steps, uses no transport).  0 = off; default 64.  No reference
+   * counterpart.                                                             */
+  int32_t sched_transport_skip_steps;
+  int32_t reserved_sched;
 } acts_kf_options;
 
 /* ---- one batch of tracks ------------------------------------------------- */

@@ -1,10 +1,14 @@
 """Scheduling knobs change only the speed of a fit: results are bit for bit
-the same with quarantine, other inner step counts and wait fractions, and in
-input order."""
+the same with quarantine, other inner step counts and wait fractions, in
+input order, and with the jacobian transport skipped after any number of
+steps since the last surface (a track that reaches a surface after all
+goes back to its checkpoint and repeats the steps with the transport; 1
+and 2 steps make nearly every track do so)."""
 import numpy as np
 import pytest
 
 import acts_amd as A
+from acts_amd import capi
 from acts_amd.fitter import ALL_STATE_OUTPUTS
 
 pytestmark = pytest.mark.gpu
@@ -14,7 +18,12 @@ VARIANTS = [
     dict(sched_quarantine_steps=20, sched_inner_steps=64, sched_wait_64ths=64),
     dict(sched_inner_steps=4, sched_wait_64ths=16),
     dict(length_sorted=False),
+    dict(sched_transport_skip_steps=0),
+    dict(sched_transport_skip_steps=1),
+    dict(sched_transport_skip_steps=2, sched_quarantine_steps=20),
 ]
+FIELDS = ("status", "n_states", "chi2", "n_steps", "n_step_trials", "smoothed", "smoothed_cov",
+          "predicted", "predicted_cov", "filtered_cov", "jacobian", "state_path")
 
 
 def test_scheduling_does_not_change_results():
@@ -31,6 +40,26 @@ def test_scheduling_does_not_change_results():
             got = f.fit(ev, outputs=ALL_STATE_OUTPUTS)
         finally:
             f.close()
-        for name in ("status", "n_states", "chi2", "n_steps", "n_step_trials", "smoothed",
-                     "smoothed_cov", "predicted", "filtered_cov", "jacobian", "state_path"):
+        for name in FIELDS:
             assert np.array_equal(getattr(got, name), getattr(ref, name)), (kw, name)
+
+
+@pytest.mark.parametrize("precision", [capi.PRECISION_FP64, capi.PRECISION_MIXED])
+def test_transport_skip_field_map(precision):
+    """The same in the field map, in both precisions, with the perigee
+    extrapolation."""
+    det, field, ev = A.workload("solenoid", n_tracks=3000, seed=52)
+    out = {}
+    for skip in (0, 1, 64):
+        print("skip", skip, precision, flush=True)
+        f = A.KalmanFitter(det, field, A.KalmanFitterOptions(
+            precision=precision, sched_transport_skip_steps=skip,
+            reference_surface=capi.REFERENCE_PERIGEE))
+        try:
+            out[skip] = f.fit(ev, outputs=ALL_STATE_OUTPUTS + ("reference_params", "reference_cov"))
+        finally:
+            f.close()
+    for skip in (1, 64):
+        for name in FIELDS + ("reference_params", "reference_cov"):
+            assert np.array_equal(getattr(out[skip], name), getattr(out[0], name),
+                                  equal_nan=True), (skip, name)
