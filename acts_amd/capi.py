@@ -51,6 +51,7 @@ SURFACE_SENSITIVE = 1
 
 FIELD_CONSTANT, FIELD_GRID_XYZ = 0, 1
 SMOOTHER_GAIN_MATRIX, SMOOTHER_MBF = 0, 1
+STEPPER_EIGEN, STEPPER_SYMPY = 0, 1
 # track-state table (acts_kf_track_states_device): parameter sets, float and
 # int column indices
 TS_PREDICTED, TS_FILTERED, TS_SMOOTHED, TS_UNBIASED = 0, 1, 2, 3
@@ -109,7 +110,7 @@ class Options(C.Structure):
         ("use_joseph", C.c_int32),
         ("precision", C.c_int32),
         ("smoother", C.c_int32),
-        ("reserved_kf", C.c_int32),
+        ("stepper", C.c_int32),
         ("outlier_chi2_cut", C.c_double),
         ("step_tolerance", C.c_double),
         ("step_size_cutoff", C.c_double),

@@ -53,7 +53,7 @@ constexpr int64_t kMaxBatch = (int64_t{1} << 31) - (int64_t{1} << 24);
 // least `waitFrac`/64 of the busy lanes wait at a surface, or `maxInner`
 // steps have been taken; then the waiting lanes run the surface update
 // together, and finished lanes write their summary and go idle.
-template <bool Mixed, bool GeoInLds>
+template <bool Mixed, bool GeoInLds, bool Sympy = false>
 __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKernel(
     FwdArgs a, int32_t* __restrict__ next, int32_t maxInner, int32_t waitFrac,
     int32_t quarantine, const int32_t* __restrict__ order) {
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
       base = __shfl(base, leader);
       if (phase == kPhaseIdle) {
         const int64_t q = static_cast<int64_t>(base) + __popcll(idle & lanesBelow);
-        phase = (q < n) ? laneInit<Mixed>(L, order ? order[q] : q, a, geo) : kPhaseRetired;
+        phase = (q < n) ? laneInit<Mixed, Sympy>(L, order ? order[q] : q, a, geo) : kPhaseRetired;
       }
     }
     if (phase == kPhaseFinished) {
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
     const uint64_t busy = __ballot(phase != kPhaseRetired);
     if (!busy) break;
     for (int it = 0; it < maxInner; ++it) {
-      if (phase == kPhaseStepping) phase = laneStep<Mixed>(L, a, geo);
+      if (phase == kPhaseStepping) phase = laneStep<Mixed, Sympy>(L, a, geo);
       const int nStep = __popcll(__ballot(phase == kPhaseStepping));
       const int nWait = __popcll(__ballot(phase == kPhaseSurface || phase == kPhaseFinished));
       if (nStep == 0 || 64 * nWait >= waitFrac * (nStep + nWait)) break;
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
       L.replaying = true;
       phase = kPhaseStepping;
     }
-    if (phase == kPhaseSurface) phase = laneFilter<Mixed>(L, a, geo, slabs);
+    if (phase == kPhaseSurface) phase = laneFilter<Mixed, Sympy>(L, a, geo, slabs);
     if (phase == kPhaseFinished) {
       laneFinish(L, a);
       phase = kPhaseIdle;
@@ -371,6 +371,7 @@ struct acts_kf_ctx {
   int32_t waitFrac = 64;
   int32_t quarantine = 0;
   bool mbf = false;  // MbfSmoother instead of GainMatrixSmoother
+  bool sympy = false;  // SympyStepper instead of EigenStepper
   // lanes stepping without the transport save their state here, one Lane
   // per resident lane of the forward kernel
   int32_t skipAfter = 64;
@@ -394,7 +395,8 @@ bool geometryFitsLds(const acts_kf_ctx* c) {
   return c->nSurfaces <= kLdsSurfaces && c->nSlabs <= kLdsSlabs;
 }
 
-ForwardFn forwardKernel(bool mixed, bool geoInLds) {
+ForwardFn forwardKernel(bool mixed, bool geoInLds, bool sympy = false) {
+  if (sympy) return geoInLds ? fitForwardKernel<false, true, true> : fitForwardKernel<false, false, true>;
   if (mixed) return geoInLds ? fitForwardKernel<true, true> : fitForwardKernel<true, false>;
   return geoInLds ? fitForwardKernel<false, true> : fitForwardKernel<false, false>;
 }
@@ -481,6 +483,13 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   if (options->precision != ACTS_KF_PRECISION_FP64 &&
       options->precision != ACTS_KF_PRECISION_MIXED)
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: precision mode not supported");
+  if (options->stepper != ACTS_KF_STEPPER_EIGEN && options->stepper != ACTS_KF_STEPPER_SYMPY)
+    return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: stepper not supported");
+  if (options->stepper == ACTS_KF_STEPPER_SYMPY &&
+      (options->precision != ACTS_KF_PRECISION_FP64 ||
+       options->reference_surface != ACTS_KF_REFERENCE_NONE))
+    return fail(ACTS_KF_ERR_UNSUPPORTED,
+                "acts_kf_create: the SympyStepper runs in FP64 without a reference surface");
   if (options->smoother != ACTS_KF_SMOOTHER_GAIN_MATRIX &&
       options->smoother != ACTS_KF_SMOOTHER_MBF)
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: smoother not supported");
@@ -606,6 +615,7 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   o.absQ = options->abs_charge;
   c->pc = makePhysConst();
   c->mbf = options->smoother == ACTS_KF_SMOOTHER_MBF;
+  c->sympy = options->stepper == ACTS_KF_STEPPER_SYMPY;
   c->refSurface = options->reference_surface;
   c->refStrategy = options->reference_strategy;
   for (int i = 0; i < 3; ++i) c->refLine.c[i] = options->reference_position[i];
@@ -647,7 +657,7 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   {
     int blocksPerCu = 0, cus = 0;
     const hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &blocksPerCu, forwardKernel(c->mixed, geometryFitsLds(c)), kBlock, 0);
+        &blocksPerCu, forwardKernel(c->mixed, geometryFitsLds(c), c->sympy), kBlock, 0);
     if (oe != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) !=
             hipSuccess ||
@@ -795,7 +805,7 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   hipLaunchKernelGGL(zeroKernel, dim3(1), dim3(64), 0, stream, queue, 1);
   const int64_t waves = std::min(blocks, c->residentWaves);
   if (timing) KF_HIP(hipEventRecord(c->ev[0], stream));
-  hipLaunchKernelGGL(forwardKernel(c->mixed, geometryFitsLds(c)),
+  hipLaunchKernelGGL(forwardKernel(c->mixed, geometryFitsLds(c), c->sympy),
                      dim3(static_cast<unsigned>(waves)), dim3(kBlock), 0, stream, fa, queue,
                      c->innerSteps, c->waitFrac, c->quarantine, perm);
   if (timing) KF_HIP(hipEventRecord(c->ev[1], stream));

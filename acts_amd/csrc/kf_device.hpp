@@ -601,4 +601,154 @@ __device__ inline int rkStep(Track& T, const DevField& F, const DevOpts& o,
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// SympyStepper (SympyStepper.cpp:141-305, vacuum kernel of
+// generate_sympy_stepper.py:157-444).  The stepper carries the
+// bound-to-free jacobian M itself instead of an 8x8 transport matrix; M
+// lives in the Track's transport fields: tr[3 i + j] = M(i, 2 + j) (position
+// rows, phi / theta / q/p columns), br[3 i + j] = M(4 + i, 2 + j) (direction
+// rows), tr37 = M(3, 4); its loc columns are jp, M(3, 5) = M(7, 4) = 1.
+// ---------------------------------------------------------------------------
+
+// M = the surface's boundToFreeJacobian (jp, jd already set)
+__device__ inline void sympyResetM(Track& T) {
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    T.tr[i] = 0.0;
+    T.br[i] = 0.0;
+  }
+  T.br[0] = T.jd[0];
+  T.br[1] = T.jd[1];
+  T.br[3] = T.jd[2];
+  T.br[4] = T.jd[3];
+  T.br[7] = T.jd[4];
+  T.tr37 = 0.0;
+#pragma unroll
+  for (int i = 0; i < 7; ++i) T.der[i] = 0.0;
+}
+
+__device__ inline V3 add3(const V3& a, const V3& b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ inline V3 sub3(const V3& a, const V3& b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ inline V3 scale3(double s, const V3& a) { return mk(s * a.x, s * a.y, s * a.z); }
+
+// One step: trials of the ATLAS-form RK4 over half-step bend vectors
+// b_i = (h q/p / 2) B_i until the error estimate 2|h| |(dir + dir2 + kick4)
+// - (dir3 + dir4)|_1 (= h^2 |k1 - k2 - k3 + k4|_1) is within 4 x the
+// tolerance; then the new position, normalised direction and time, and
+// (with `transport`) M's phi, theta and q/p columns through the same
+// recursion (tangents of the stages) and the path derivatives.
+__device__ inline int rkStepSympy(Track& T, const DevField& F, const DevOpts& o,
+                                  double propDir = 1.0, bool transport = true) {
+  const V3 pos = T.pos;
+  const V3 dir = T.dir;
+  const double qop = T.qop;
+  const double pabs = copysign(static_cast<double>(o.absQ), qop) / qop;
+  const double m = o.mass;
+  V3 B1;
+  if (const int fs = fieldAt(F, pos, B1)) return fs;
+  const double initialH = stepValue(T) * propDir;
+  double h = initialH, err = 0, hThird = 0;
+  int nTrials = 0;
+  V3 bend1, bend2, bend3, kick1, dir2, dir3, dir4, dirEnd, kick4;
+  while (true) {
+    ++nTrials;
+    const double halfHQop = h * qop / 2;
+    hThird = h / 3;
+    const double hQuarter = h / 4;
+    bend1 = scale3(halfHQop, B1);
+    kick1 = cross(dir, bend1);
+    dir2 = add3(kick1, dir);
+    const V3 dirHalfSum = add3(dir2, dir);
+    const V3 pos2 = add3(pos, scale3(hQuarter, dirHalfSum));
+    V3 B2;
+    if (const int fs = fieldAt(F, pos2, B2)) return fs;
+    bend2 = scale3(halfHQop, B2);
+    dir3 = add3(dir, cross(dir2, bend2));
+    dir4 = add3(dir, cross(dir3, bend2));
+    dirEnd = sub3(scale3(2, dir4), dir);
+    const V3 pos3 = add3(pos, scale3(h, dir4));
+    V3 B3;
+    if (const int fs = fieldAt(F, pos3, B3)) return fs;
+    bend3 = scale3(halfHQop, B3);
+    kick4 = cross(dirEnd, bend3);
+    const V3 e = sub3(add3(dirHalfSum, kick4), add3(dir3, dir4));
+    err = 2 * fabs(h) * ((fabs(e.x) + fabs(e.y)) + fabs(e.z));
+    const bool ok = !(err > 4.0 * o.stepTolerance);
+    err = fmax(1e-20, err);
+    if (ok) break;
+    h *= stepScaling(o.stepTolerance, err);
+    if (fabs(h) < fabs(o.stepSizeCutOff)) return ACTS_KF_STATUS_STEP_SIZE_STALLED;
+    if (static_cast<uint32_t>(nTrials) > o.maxTrials) return ACTS_KF_STATUS_STEP_ADJUSTMENT_FAILED;
+  }
+  const V3 newPos = add3(pos, scale3(hThird, add3(add3(dir2, dir3), dir4)));
+  const V3 newDirX3 = add3(scale3(2, dir3), add3(add3(kick1, dirEnd), kick4));
+  const double invNorm = 1 / sqrt(dot(newDirX3, newDirX3));
+  const V3 newDir = scale3(invNorm, newDirX3);
+  const double dtds = sqrt(1 + (m * m) / (pabs * pabs));
+  if (transport) {
+    // phi and theta columns
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const V3 seed = mk(T.br[c], T.br[3 + c], T.br[6 + c]);
+      const V3 tKick1 = cross(seed, bend1);
+      const V3 tDir2 = add3(tKick1, seed);
+      const V3 tDir3 = add3(seed, cross(tDir2, bend2));
+      const V3 tDir4 = add3(seed, cross(tDir3, bend2));
+      const V3 tDirEnd = sub3(scale3(2, tDir4), seed);
+      const V3 tKick4 = cross(tDirEnd, bend3);
+      const V3 dp = add3(add3(tDir2, tDir3), tDir4);
+      const V3 dd = add3(add3(add3(tKick1, scale3(2, tDir3)), tDirEnd), tKick4);
+      T.tr[c] = T.tr[c] + hThird * dp.x;
+      T.tr[3 + c] = T.tr[3 + c] + hThird * dp.y;
+      T.tr[6 + c] = T.tr[6 + c] + hThird * dp.z;
+      T.br[c] = dd.x / 3;
+      T.br[3 + c] = dd.y / 3;
+      T.br[6 + c] = dd.z / 3;
+    }
+    // q/p column: direction part scaled by q/p plus the stages' own q/p
+    // dependence (M(7, 4) = 1 in vacuum)
+    {
+      const double invQop = 1 / qop;
+      const double posW = hThird * invQop, dirW = invQop / 3;
+      const V3 seed = scale3(qop, mk(T.br[2], T.br[5], T.br[8]));
+      const V3 tKick1 = add3(scale3(1.0, kick1), cross(seed, bend1));
+      const V3 tDir2 = add3(tKick1, seed);
+      const V3 tDir3 = add3(add3(scale3(1.0, sub3(dir3, dir)), seed), cross(tDir2, bend2));
+      const V3 tDir4 = add3(add3(scale3(1.0, sub3(dir4, dir)), seed), cross(tDir3, bend2));
+      const V3 tDirEnd = sub3(scale3(2, tDir4), seed);
+      const V3 tKick4 = add3(scale3(1.0, kick4), cross(tDirEnd, bend3));
+      const V3 dp = add3(add3(tDir2, tDir3), tDir4);
+      const V3 dd = add3(add3(add3(tKick1, scale3(2, tDir3)), tDirEnd), tKick4);
+      T.tr[2] = T.tr[2] + posW * dp.x;
+      T.tr[5] = T.tr[5] + posW * dp.y;
+      T.tr[8] = T.tr[8] + posW * dp.z;
+      T.br[2] = dirW * dd.x;
+      T.br[5] = dirW * dd.y;
+      T.br[8] = dirW * dd.z;
+      const double dtDqop = h * (m * m) / ((pabs * pabs) * qop * dtds);
+      T.tr37 = T.tr37 + dtDqop * 1.0;
+    }
+    const double twoOverH = 2 / h;
+    T.der[0] = newDir.x;
+    T.der[1] = newDir.y;
+    T.der[2] = newDir.z;
+    T.der[3] = dtds;
+    T.der[4] = twoOverH * kick4.x;
+    T.der[5] = twoOverH * kick4.y;
+    T.der[6] = twoOverH * kick4.z;
+  }
+  T.pos = newPos;
+  T.time = T.time + h * dtds;
+  T.dir = newDir;
+  T.pathAcc += h;
+  ++T.nSteps;
+  T.nTrials += nTrials;
+  const double nextAccuracy = fabs(h * stepScaling(o.stepTolerance, err));
+  const double previousAccuracy = fabs(T.cAcc);
+  const double initialStepLength = fabs(initialH);
+  if (nextAccuracy < initialStepLength || nextAccuracy > previousAccuracy)
+    T.cAcc = nextAccuracy;
+  return 0;
+}
+
 }  // namespace kf

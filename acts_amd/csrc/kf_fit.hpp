@@ -40,6 +40,11 @@ struct DevOut {
 // globalToLocal at the current position (reinitialization keeps the old
 // jacToGlobal when it fails, as the reference ignores that Result).
 // ---------------------------------------------------------------------------
+// With Sympy the jacobian is F2B * (I + d s) * M, M the bound-to-free
+// jacobian the SympyStepper carries (sympy::boundToBoundTransportJacobian,
+// SympyJacobianEngine.cpp:21-45), in the same association order with its
+// structural zeros skipped.
+template <bool Sympy = false>
 __device__ inline void transportToBound(Track& T, const DevSurface& s,
                                         bool onSurface, double J[36]) {
   const V3 dir = T.dir;
@@ -82,6 +87,32 @@ __device__ inline void transportToBound(Track& T, const DevSurface& s,
     A1[4][j] = 0.0;
     A1[5][j] = A[3][j];
   }
+  if constexpr (Sympy) {
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      const double a0 = (r == 4) ? 0.0 : A1[r][0];
+      const double a1 = (r == 4) ? 0.0 : A1[r][1];
+      const double a2 = (r == 4) ? 0.0 : A1[r][2];
+      J[6 * r + 0] = (a0 * T.jp[0] + a1 * T.jp[2]) + a2 * T.jp[4];
+      J[6 * r + 1] = (a0 * T.jp[1] + a1 * T.jp[3]) + a2 * T.jp[5];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double m0 = T.tr[c], m1 = T.tr[3 + c], m2 = T.tr[6 + c];
+        const double d0 = T.br[c], d1 = T.br[3 + c], d2 = T.br[6 + c];
+        double acc;
+        if (r == 4) {
+          acc = (c == 2) ? 1.0 : 0.0;  // M(7, 2 + c)
+        } else {
+          acc = (a0 * m0 + a1 * m1) + a2 * m2;
+          if (r == 2) acc = (acc + F24 * d0) + F25 * d1;
+          if (r == 3) acc = ((acc + F34 * d0) + F35 * d1) + F36 * d2;
+          if (r == 5 && c == 2) acc = acc + T.tr37;  // M(3, 4)
+        }
+        J[6 * r + 2 + c] = acc;
+      }
+      J[6 * r + 5] = (r == 5) ? 1.0 : 0.0;
+    }
+  } else {
   // A2 = A1 * Jt, columns 4..7 (columns 0..3 equal A1's)
   double A2[6][4];
 #pragma unroll
@@ -110,6 +141,7 @@ __device__ inline void transportToBound(Track& T, const DevSurface& s,
     J[6 * r + 3] = (A2[r][0] * T.jd[1] + A2[r][1] * T.jd[3]) + A2[r][2] * T.jd[4];
     J[6 * r + 4] = A2[r][3];
     J[6 * r + 5] = (r == 5) ? 1.0 : 0.0;
+  }
   }
   // cov = (J * cov) * J^T on the packed symmetric covariance; row 4 of J is
   // e4 and column 5 is e5, so those products are skipped exactly.
@@ -151,6 +183,7 @@ __device__ inline void transportToBound(Track& T, const DevSurface& s,
     for (int i = 0; i < 6; ++i) T.jp[i] = b2f[i];
     directionB2F(dir, T.jd);
   }
+  if constexpr (Sympy) sympyResetM(T);
 }
 
 // ---------------------------------------------------------------------------

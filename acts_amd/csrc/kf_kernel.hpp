@@ -87,7 +87,7 @@ struct FwdArgs {
 // EigenStepper::initialize (EigenStepper.ipp:37-70), DirectNavigator
 // initialisation and the pre-propagation act of the KalmanFitter actor.
 // Returns the lane's next phase.
-template <bool Mixed>
+template <bool Mixed, bool Sympy = false>
 __device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a,
                                const DevSurface* geo) {
   const DevBatch& in = a.in;
@@ -176,6 +176,7 @@ __device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a,
     T.jp[4] = pl[6]; T.jp[5] = pl[7];
     directionB2F(T.dir, T.jd);
     resetTransport(T);
+    if constexpr (Sympy) sympyResetM(T);
     T.pathAcc = 0;
     T.cNav = kNotSet;
     T.cActor = kNotSet;
@@ -205,7 +206,7 @@ __device__ inline int laneAfterStep(Lane& L, const FwdArgs& a, const DevSurface*
 
 // One iteration of the propagation loop (Propagator.ipp:25-210): step, target
 // status, the actor's finalisation checks, aborters, next target.
-template <bool Mixed>
+template <bool Mixed, bool Sympy = false>
 __device__ inline int laneStep(Lane& L, const FwdArgs& a, const DevSurface* geo) {
   const DevOpts& o = a.o;
   Track& T = L.T;
@@ -220,7 +221,13 @@ __device__ inline int laneStep(Lane& L, const FwdArgs& a, const DevSurface* geo)
   }
   ++L.iter;
   const bool transport = !L.noTransport;
-  const int sst = Mixed ? rkStepMixed(T, a.F, o, 1.0, transport) : rkStep(T, a.F, o, 1.0, transport);
+  int sst;
+  if constexpr (Mixed)
+    sst = rkStepMixed(T, a.F, o, 1.0, transport);
+  else if constexpr (Sympy)
+    sst = rkStepSympy(T, a.F, o, 1.0, transport);
+  else
+    sst = rkStep(T, a.F, o, 1.0, transport);
   if (sst) {
     L.status = sst;
     return kPhaseFinished;
@@ -258,7 +265,7 @@ __device__ inline int laneAfterStep(Lane& L, const FwdArgs& a, const DevSurface*
 
 // KalmanFitter::Actor::filter on the reached surface (KalmanFitter.hpp:
 // 339-671), then the rest of the propagation-loop iteration.
-template <bool Mixed>
+template <bool Mixed, bool Sympy = false>
 __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* geo,
                                  const DevSlab* slabs) {
   const DevBatch& in = a.in;
@@ -294,7 +301,7 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
       double l0 = 0, l1 = 0;
       const bool onS = globalToLocal(s, T.pos, l0, l1);
       double J[36];
-      transportToBound(T, s, onS, J);
+      transportToBound<Sympy>(T, s, onS, J);
       if (!onS) {
         L.status = ACTS_KF_STATUS_NOT_ON_SURFACE;
       } else if (L.nStates >= sc.maxStates) {
@@ -353,6 +360,7 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
 #pragma unroll
               for (int i = 0; i < 6; ++i) T.jp[i] = b2f[i];
               directionB2F(T.dir, T.jd);
+              if constexpr (Sympy) sympyResetM(T);
               ++L.nMeasStates;
               if (L.firstMeas < 0) L.firstMeas = k;
               L.lastMeas = k;

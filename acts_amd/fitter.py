@@ -28,7 +28,7 @@ class KalmanFitterOptions:
     outlier_chi2_cut: float = 0.0           # SimpleOutlierFinder; <= 0 disables
     precision: int = capi.PRECISION_FP64
     smoother: int = capi.SMOOTHER_GAIN_MATRIX  # capi.SMOOTHER_MBF: as the Examples
-    reserved_kf: int = 0
+    stepper: int = 0                        # capi.STEPPER_*
     step_tolerance: float = 1e-4            # StepperOptions.hpp:37
     step_size_cutoff: float = 0.0
     initial_step_size: float = 10000.0

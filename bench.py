@@ -301,6 +301,14 @@ def main() -> int:
         "first_state_rel_err_vs_rts": quality.relative_error(quality.first_states(outb, events),
                                                              first64)}
     del outb
+    # the SympyStepper (the Examples' stepper) in place of the EigenStepper
+    mss, _, kts, outs_ = run(capi.PRECISION_FP64, {"stepper": capi.STEPPER_SYMPY})
+    line["with_sympy_stepper"] = {
+        "value": n / (mss * 1e-3), "ms_per_step": mss,
+        "forward_kernel_ms": float(np.mean([k[0] for k in kts])),
+        "first_state_rel_err_vs_eigen": quality.relative_error(
+            quality.first_states(outs_, events), first64)}
+    del outs_
     line["pulls"] = quality.pulls(first64, events)
     if not args.no_mixed:
         msm, _, ktm, outm = run(capi.PRECISION_MIXED)

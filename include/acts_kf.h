@@ -149,6 +149,14 @@ typedef struct acts_kf_field {
  * GainMatrixSmoother.cpp:18-97) or MbfSmoother (modified Bryson-Frazier,
  * MbfSmoother.cpp, detail/MbfSmootherImpl.hpp), which the Examples use
  * (KalmanFitterFunction.cpp:39-44,100-102)                                 */
+/* the stepper of the propagator: EigenStepper (EigenStepper.ipp:169-388,
+ * the drop-in) or SympyStepper (Core/src/Propagator/SympyStepper.cpp:
+ * 141-305), the Examples' stepper (KalmanFitterFunction.cpp:39-44), whose
+ * vacuum step carries the bound-to-free jacobian through an ATLAS-form RK4
+ * (Core/src/Propagator/generate_sympy_stepper.py:157-444).  SYMPY is
+ * supported with ACTS_KF_PRECISION_FP64 and without a reference surface.   */
+#define ACTS_KF_STEPPER_EIGEN 0
+#define ACTS_KF_STEPPER_SYMPY 1
 #define ACTS_KF_SMOOTHER_GAIN_MATRIX 0
 #define ACTS_KF_SMOOTHER_MBF 1
 
@@ -158,7 +166,7 @@ typedef struct acts_kf_options {
   int32_t use_joseph;          /* GainMatrixUpdater, default 0 */
   int32_t precision;           /* ACTS_KF_PRECISION_* */
   int32_t smoother;            /* ACTS_KF_SMOOTHER_*, default GAIN_MATRIX */
-  int32_t reserved_kf;
+  int32_t stepper;             /* ACTS_KF_STEPPER_*, default EIGEN */
   double outlier_chi2_cut;     /* predicted chi2 cut; <= 0 disables the finder */
   /* StepperPlainOptions (StepperOptions.hpp:36-49) */
   double step_tolerance;       /* 1e-4 */

@@ -760,9 +760,147 @@ inline Mat<3, 3> crossCols(const Mat<3, 3>& m, const V3& v) {
   return r;
 }
 
+// SympyStepper::step (SympyStepper.cpp:141-305) with the vacuum kernel
+// rk4_vacuum that generate_sympy_stepper.py writes (its
+// _atlas_rk4_stages and rk4_vacuum_b2f_atlasexpr, :157-444): an ATLAS-form
+// RK4 step over half-step "bend" vectors b_i = (h q/p / 2) B_i, whose
+// error estimate is 2|h| |(2d + h k1/2 + h k4/2) - (dir3 + dir4)|_1 = h^2
+// |k1 - k2 - k3 + k4|_1, and which carries the bound-to-free jacobian M
+// (here st.jacToGlobal; st.jacTransport stays the identity) through the
+// same recursion: the phi, theta and q/p columns of M as tangents of the
+// stages.  The generated code's operation order is sympy's and is not
+// available here; this is the recursion in its natural order.
+StepResult stepSympy(Stepper& st, const Field& field, const Particle& ph,
+                     const acts_kf_options& o, double propDir) {
+  const V3 pos = st.position();
+  const V3 dir = st.direction();
+  const double t = st.pars.a[3][0];
+  const double qop = st.qop();
+  const double pabs = ph.extractMomentum(qop);
+  const double m = static_cast<float>(ph.mass);
+  auto add = [](const V3& a, const V3& b) { return v3(a[0] + b[0], a[1] + b[1], a[2] + b[2]); };
+  auto sub = [](const V3& a, const V3& b) { return v3(a[0] - b[0], a[1] - b[1], a[2] - b[2]); };
+  auto scale = [](double s, const V3& a) { return v3(s * a[0], s * a[1], s * a[2]); };
+  auto scaling = [&](double err) {
+    double x = o.step_tolerance / err;
+    x = std::sqrt(std::sqrt(x));
+    return std::clamp(x, 0.25, 4.0);
+  };
+  V3 B1;
+  if (int fs = field.get(pos, B1)) return {fs, 0};
+
+  const double initialH = st.stepSize.value() * propDir;
+  double h = initialH;
+  double err = 0;
+  unsigned nTrials = 0;
+  V3 bend1, bend2, bend3, kick1, dir2, dir3, dir4, dirEnd, kick4, dirHalfSum;
+  double hThird = 0;
+  while (true) {
+    ++nTrials;
+    const double halfHQop = h * qop / 2;
+    hThird = h / 3;
+    const double hQuarter = h / 4;
+    bend1 = scale(halfHQop, B1);
+    kick1 = cross(dir, bend1);
+    dir2 = add(kick1, dir);
+    dirHalfSum = add(dir2, dir);
+    const V3 pos2 = add(pos, scale(hQuarter, dirHalfSum));
+    V3 B2;
+    if (int fs = field.get(pos2, B2)) return {fs, 0};
+    bend2 = scale(halfHQop, B2);
+    dir3 = add(dir, cross(dir2, bend2));
+    dir4 = add(dir, cross(dir3, bend2));
+    dirEnd = sub(scale(2, dir4), dir);
+    const V3 pos3 = add(pos, scale(h, dir4));
+    V3 B3;
+    if (int fs = field.get(pos3, B3)) return {fs, 0};
+    bend3 = scale(halfHQop, B3);
+    kick4 = cross(dirEnd, bend3);
+    const V3 e = sub(add(dirHalfSum, kick4), add(dir3, dir4));
+    err = 2 * std::abs(h) * ((std::abs(e[0]) + std::abs(e[1])) + std::abs(e[2]));
+    const bool ok = !(err > 4.0 * o.step_tolerance);
+    err = std::max(1e-20, err);
+    if (ok) break;
+    h *= scaling(err);
+    if (std::abs(h) < std::abs(o.step_size_cutoff))
+      return {ACTS_KF_STATUS_STEP_SIZE_STALLED, 0};
+    if (nTrials > o.max_runge_kutta_step_trials)
+      return {ACTS_KF_STATUS_STEP_ADJUSTMENT_FAILED, 0};
+  }
+  const V3 newPos = add(pos, scale(hThird, add(add(dir2, dir3), dir4)));
+  const V3 newDirX3 = add(scale(2, dir3), add(add(kick1, dirEnd), kick4));
+  const double invNorm = 1 / std::sqrt(squaredNorm(newDirX3));
+  const V3 newDir = scale(invNorm, newDirX3);
+  const double dtds = std::sqrt(1 + (m * m) / (pabs * pabs));
+  const double newTime = t + h * dtds;
+  if (st.covTransport) {
+    Mat<8, 6>& M = st.jacToGlobal;
+    // the phi and theta columns: tangents of the stages along the column's
+    // direction part
+    for (int c = 2; c <= 3; ++c) {
+      const V3 seed = v3(M.a[4][c], M.a[5][c], M.a[6][c]);
+      const V3 tKick1 = cross(seed, bend1);
+      const V3 tDir2 = add(tKick1, seed);
+      const V3 tDir3 = add(seed, cross(tDir2, bend2));
+      const V3 tDir4 = add(seed, cross(tDir3, bend2));
+      const V3 tDirEnd = sub(scale(2, tDir4), seed);
+      const V3 tKick4 = cross(tDirEnd, bend3);
+      const V3 dp = add(add(tDir2, tDir3), tDir4);
+      const V3 dd = add(add(add(tKick1, scale(2, tDir3)), tDirEnd), tKick4);
+      for (int i = 0; i < 3; ++i) {
+        M.a[i][c] = M.a[i][c] + hThird * dp[i];
+        M.a[4 + i][c] = dd[i] / 3;
+      }
+    }
+    // the q/p column: direction part scaled by q/p, plus the stages' own
+    // q/p dependence (the bend vectors are linear in q/p)
+    {
+      const double m74 = M.a[7][4];
+      const double invQop = 1 / qop;
+      const double posW = hThird * invQop, dirW = invQop / 3;
+      const V3 seed = scale(qop, v3(M.a[4][4], M.a[5][4], M.a[6][4]));
+      const V3 tKick1 = add(scale(m74, kick1), cross(seed, bend1));
+      const V3 tDir2 = add(tKick1, seed);
+      const V3 tDir3 = add(add(scale(m74, sub(dir3, dir)), seed), cross(tDir2, bend2));
+      const V3 tDir4 = add(add(scale(m74, sub(dir4, dir)), seed), cross(tDir3, bend2));
+      const V3 tDirEnd = sub(scale(2, tDir4), seed);
+      const V3 tKick4 = add(scale(m74, kick4), cross(tDirEnd, bend3));
+      const V3 dp = add(add(tDir2, tDir3), tDir4);
+      const V3 dd = add(add(add(tKick1, scale(2, tDir3)), tDirEnd), tKick4);
+      for (int i = 0; i < 3; ++i) {
+        M.a[i][4] = M.a[i][4] + posW * dp[i];
+        M.a[4 + i][4] = dirW * dd[i];
+      }
+      // time row: d(time)/d(q/p) = h m^2 / (p^2 q/p dt/ds)
+      const double dtDqop = h * (m * m) / ((pabs * pabs) * qop * dtds);
+      M.a[3][4] = M.a[3][4] + dtDqop * m74;
+    }
+    // path derivatives: direction, dt/ds, k4 = (2/h) kick4, 0
+    const double twoOverH = 2 / h;
+    for (int i = 0; i < 3; ++i) st.derivative.a[i][0] = newDir[i];
+    st.derivative.a[3][0] = dtds;
+    for (int i = 0; i < 3; ++i) st.derivative.a[4 + i][0] = twoOverH * kick4[i];
+    st.derivative.a[7][0] = 0;
+  }
+  for (int i = 0; i < 3; ++i) st.pars.a[i][0] = newPos[i];
+  st.pars.a[3][0] = newTime;
+  for (int i = 0; i < 3; ++i) st.pars.a[4 + i][0] = newDir[i];
+  st.pathAccumulated += h;
+  ++st.nSteps;
+  st.nStepTrials += nTrials;
+  const double sc = scaling(err);
+  const double nextAccuracy = std::abs(h * sc);
+  const double previousAccuracy = std::abs(st.stepSize.accuracy);
+  const double initialStepLength = std::abs(initialH);
+  if (nextAccuracy < initialStepLength || nextAccuracy > previousAccuracy)
+    st.stepSize.accuracy = nextAccuracy;
+  return {0, h};
+}
+
 // EigenStepper.ipp:169-388 with EigenStepperDefaultExtension.hpp:104-228
 StepResult step(Stepper& st, const Field& field, const Particle& ph,
                 const acts_kf_options& o, double propDir) {
+  if (o.stepper == ACTS_KF_STEPPER_SYMPY) return stepSympy(st, field, ph, o, propDir);
   double errorEstimate = 0, h2 = 0, half_h = 0;
   const V3 pos = st.position();
   const V3 dir = st.direction();

@@ -46,7 +46,8 @@ def cov_rel_err(a: np.ndarray, b: np.ndarray, scale_cov: np.ndarray | None = Non
     return (np.abs(a - b) / scale).reshape(len(a), -1)
 
 
-def compare(got, ref, events, rel: float = 1e-10, rel_smoothed_cov: float = 5e-8) -> dict:
+def compare(got, ref, events, rel: float = 1e-10, rel_smoothed_cov: float = 5e-8,
+            rel_jacobian: float | None = None) -> dict:
     """Return a report; `ok` is True if everything agrees within tolerance."""
     n = events.n_tracks
     rep = {"n_tracks": n}
@@ -107,7 +108,8 @@ def compare(got, ref, events, rel: float = 1e-10, rel_smoothed_cov: float = 5e-8
             np.maximum(np.abs(ref.state_chi2[idx]), 1.0)))
     rep["ok"] = (rep["status_mismatch"] == 0 and rep["n_states_mismatch"] == 0
                  and rep["state_sequence_mismatch_slots"] == 0
-                 and all(v <= (rel_smoothed_cov if k == "smoothed_cov" else rel)
+                 and all(v <= (rel_smoothed_cov if k == "smoothed_cov" else
+                               rel_jacobian if k == "jacobian" and rel_jacobian else rel)
                          for k, v in worst.items())
                  and all(rep[f"{k}_mismatch"] == 0 for k in
                          ("chi2", "ndf", "n_measurements", "n_holes", "n_outliers")))

@@ -69,7 +69,7 @@ def test_structs_match_header_sizes():
     assert C.sizeof(capi.BatchIn) == 8 * 10 + 8
 
 
-@pytest.mark.parametrize("field,value", [("smoother", 7), ("precision", 5),
+@pytest.mark.parametrize("field,value", [("smoother", 7), ("stepper", 2), ("precision", 5),
                                          ("reference_surface", 9), ("reference_strategy", 3)])
 def test_create_rejects_unknown_options(lib, field, value):
     """Options are validated before any device call: an unknown smoother,
@@ -88,3 +88,20 @@ def test_create_rejects_unknown_options(lib, field, value):
     assert not ctx.value
     msg = capi.last_error().lower()
     assert field.split("_")[-1] in msg or field.split("_")[0] in msg, msg
+
+
+@pytest.mark.parametrize("field,value", [("precision", capi.PRECISION_MIXED),
+                                         ("reference_surface", capi.REFERENCE_PERIGEE)])
+def test_sympy_stepper_combinations_rejected(lib, field, value):
+    """The SympyStepper runs in FP64 without a reference surface; other
+    combinations fail acts_kf_create before any device call."""
+    import acts_amd as A
+    det = A.barrel_detector()
+    o = A.KalmanFitterOptions(stepper=capi.STEPPER_SYMPY).struct()
+    setattr(o, field, value)
+    fld = A.ConstantField().struct()
+    ctx = C.c_void_p()
+    rc = lib.acts_kf_create(det.surface_table(), len(det.surfaces), det.material_table(),
+                            len(det.materials), C.byref(fld), C.byref(o), 0, C.byref(ctx))
+    assert rc == capi.ERR_UNSUPPORTED and not ctx.value
+    assert "sympystepper" in capi.last_error().lower()
