@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void smoothKernel(DevBatch in, DevOut out,
 // surface, one lane per track (KalmanFitter.hpp:959-971).  A failure fails
 // the fit: the status becomes the extrapolation's error and the summary is
 // cleared, as for any failed fit.
-template <bool Mixed>
+template <bool Mixed, bool Sympy = false>
 __global__ __launch_bounds__(kBlock) void referenceKernel(DevOut out, DevScratch sc,
                                                           const DevSurface* __restrict__ geo,
                                                           DevField F, DevOpts o, LineRef L,
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(kBlock) void referenceKernel(DevOut out, DevScratch
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= n || sc.track[t] != 0) return;
   double p[6], C[36];
-  const int st = extrapolateTrack<Mixed>(t, sc, sc.ref, geo, F, o, L, strategy, p, C);
+  const int st = extrapolateTrack<Mixed, Sympy>(t, sc, sc.ref, geo, F, o, L, strategy, p, C);
   if (st) {
     if (out.status) out.status[t] = st;
     if (out.nStates) out.nStates[t] = 0;
@@ -485,11 +485,8 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: precision mode not supported");
   if (options->stepper != ACTS_KF_STEPPER_EIGEN && options->stepper != ACTS_KF_STEPPER_SYMPY)
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: stepper not supported");
-  if (options->stepper == ACTS_KF_STEPPER_SYMPY &&
-      (options->precision != ACTS_KF_PRECISION_FP64 ||
-       options->reference_surface != ACTS_KF_REFERENCE_NONE))
-    return fail(ACTS_KF_ERR_UNSUPPORTED,
-                "acts_kf_create: the SympyStepper runs in FP64 without a reference surface");
+  if (options->stepper == ACTS_KF_STEPPER_SYMPY && options->precision != ACTS_KF_PRECISION_FP64)
+    return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: the SympyStepper runs in FP64");
   if (options->smoother != ACTS_KF_SMOOTHER_GAIN_MATRIX &&
       options->smoother != ACTS_KF_SMOOTHER_MBF)
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: smoother not supported");
@@ -812,7 +809,11 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   hipLaunchKernelGGL(c->mbf ? smoothKernel<true> : smoothKernel<false>,
                      dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream, b, d, sc, perm);
   if (withRef) {
-    if (c->mixed)
+    if (c->sympy)
+      hipLaunchKernelGGL((referenceKernel<false, true>), dim3(static_cast<unsigned>(blocks)),
+                         dim3(kBlock), 0, stream, d, sc, c->geo, c->field, c->extOpts,
+                         c->refLine, c->refStrategy, n);
+    else if (c->mixed)
       hipLaunchKernelGGL(referenceKernel<true>, dim3(static_cast<unsigned>(blocks)), dim3(kBlock),
                          0, stream, d, sc, c->geo, c->field, c->extOpts, c->refLine,
                          c->refStrategy, n);

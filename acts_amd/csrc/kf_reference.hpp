@@ -80,6 +80,9 @@ __device__ inline void mulDense(const double* A, const double* B, double* out) {
 // F2B * (I + d s) * Jt * B2F, with the line's path derivative, which depends
 // on the direction too (LineSurface.cpp:227-250).  Writes the bound
 // parameters and the full covariance; returns a status.
+// With Sympy, Jt is the identity and B2F is the jacobian M the SympyStepper
+// carried.
+template <bool Sympy = false>
 __device__ inline int lineBoundState(const Track& T, const LineRef& L, double tol,
                                      double params[6], double cov[36]) {
   const V3 pos = T.pos, dir = T.dir;
@@ -139,14 +142,16 @@ __device__ inline int lineBoundState(const Track& T, const LineRef& L, double to
   double Jt[64];
 #pragma unroll
   for (int i = 0; i < 64; ++i) Jt[i] = (i % 9 == 0) ? 1.0 : 0.0;
+  if constexpr (!Sympy) {
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+    for (int i = 0; i < 3; ++i)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      Jt[8 * i + 4 + c] = T.tr[4 * i + c];
-      Jt[8 * (4 + i) + 4 + c] = T.br[4 * i + c];
-    }
-  Jt[8 * 3 + 7] = T.tr37;
+      for (int c = 0; c < 4; ++c) {
+        Jt[8 * i + 4 + c] = T.tr[4 * i + c];
+        Jt[8 * (4 + i) + 4 + c] = T.br[4 * i + c];
+      }
+    Jt[8 * 3 + 7] = T.tr37;
+  }
   // jacToGlobal from its blocks
   double B2F[48];
 #pragma unroll
@@ -158,12 +163,23 @@ __device__ inline int lineBoundState(const Track& T, const LineRef& L, double to
   B2F[12] = T.jp[4];
   B2F[13] = T.jp[5];
   B2F[3 * 6 + 5] = 1;
-  B2F[4 * 6 + 2] = T.jd[0];
-  B2F[4 * 6 + 3] = T.jd[1];
-  B2F[5 * 6 + 2] = T.jd[2];
-  B2F[5 * 6 + 3] = T.jd[3];
-  B2F[6 * 6 + 3] = T.jd[4];
   B2F[7 * 6 + 4] = 1;
+  if constexpr (Sympy) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        B2F[i * 6 + 2 + c] = T.tr[3 * i + c];
+        B2F[(4 + i) * 6 + 2 + c] = T.br[3 * i + c];
+      }
+    B2F[3 * 6 + 4] = T.tr37;
+  } else {
+    B2F[4 * 6 + 2] = T.jd[0];
+    B2F[4 * 6 + 3] = T.jd[1];
+    B2F[5 * 6 + 2] = T.jd[2];
+    B2F[5 * 6 + 3] = T.jd[3];
+    B2F[6 * 6 + 3] = T.jd[4];
+  }
   double A[48], B[48], J[36];
   mulDense<6, 8, 8>(F2B, corr, A);
   mulDense<6, 8, 8>(A, Jt, B);
@@ -189,6 +205,7 @@ __device__ inline int lineBoundState(const Track& T, const LineRef& L, double to
 
 // EigenStepper::initialize (EigenStepper.ipp:37-70) from bound parameters
 // and a packed covariance on surface s
+template <bool Sympy = false>
 __device__ inline void initializeFromBound(Track& T, const DevSurface& s, const double p[6],
                                            const double c[21], const DevOpts& o) {
   const double sinTheta = sin(p[3]);
@@ -204,6 +221,7 @@ __device__ inline void initializeFromBound(Track& T, const DevSurface& s, const 
   for (int i = 0; i < 6; ++i) T.jp[i] = b2f[i];
   directionB2F(T.dir, T.jd);
   resetTransport(T);
+  if constexpr (Sympy) sympyResetM(T);
   T.pathAcc = 0;
   T.nSteps = 0;
   T.nTrials = 0;
@@ -216,7 +234,7 @@ __device__ inline void initializeFromBound(Track& T, const DevSurface& s, const 
 // Propagator::propagate(start, target) to the line in direction `direction`
 // with ForcedSurfaceReached + PathLimitReached (loop protection on);
 // returns a status, T holds the state at the end.
-template <bool Mixed>
+template <bool Mixed, bool Sympy = false>
 __device__ inline int propagateToLine(Track& T, const LineRef& L, const DevField& F,
                                       const DevOpts& o, double direction) {
   double limit = o.pathLimit;
@@ -238,7 +256,13 @@ __device__ inline int propagateToLine(Track& T, const LineRef& L, const DevField
   };
   if (checkAbort()) return 0;
   for (uint32_t steps = 0; steps < o.maxSteps; ++steps) {
-    const int sst = Mixed ? rkStepMixed(T, F, o, direction) : rkStep(T, F, o, direction);
+    int sst;
+    if constexpr (Mixed)
+      sst = rkStepMixed(T, F, o, direction);
+    else if constexpr (Sympy)
+      sst = rkStepSympy(T, F, o, direction);
+    else
+      sst = rkStep(T, F, o, direction);
     if (sst) return sst;
     T.cNav = kNotSet;
     T.cActor = kNotSet;
@@ -250,7 +274,7 @@ __device__ inline int propagateToLine(Track& T, const LineRef& L, const DevField
 // extrapolateTrackToReferenceSurface for track t (states in the scratch,
 // smoothed first/last measurement states in `ref`); returns a status and
 // writes the parameters and covariance on the line.
-template <bool Mixed>
+template <bool Mixed, bool Sympy = false>
 __device__ inline int extrapolateTrack(int64_t t, const DevScratch& sc, const double* ref,
                                        const DevSurface* geo, const DevField& F,
                                        const DevOpts& o, const LineRef& L, int32_t strategy,
@@ -295,9 +319,9 @@ __device__ inline int extrapolateTrack(int64_t t, const DevScratch& sc, const do
   // Direction::fromScalarZeroAsPositive
   const double direction = distance >= 0 ? 1.0 : -1.0;
   Track T;
-  initializeFromBound(T, geo[si(sc, k, kISurf, t)], p, p + 6, o);
-  if (const int st = propagateToLine<Mixed>(T, L, F, o, direction)) return st;
-  return lineBoundState(T, L, o.surfaceTolerance, params, cov);
+  initializeFromBound<Sympy>(T, geo[si(sc, k, kISurf, t)], p, p + 6, o);
+  if (const int st = propagateToLine<Mixed, Sympy>(T, L, F, o, direction)) return st;
+  return lineBoundState<Sympy>(T, L, o.surfaceTolerance, params, cov);
 }
 
 }  // namespace kf

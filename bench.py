@@ -309,6 +309,20 @@ def main() -> int:
         "first_state_rel_err_vs_eigen": quality.relative_error(
             quality.first_states(outs_, events), first64)}
     del outs_
+    # the Examples' fit (KalmanFitterFunction.cpp, TrackFittingAlgorithm.cpp):
+    # SympyStepper, MbfSmoother, parameters at a perigee from the first
+    # measurement
+    mse, _, kte, oute = run(capi.PRECISION_FP64,
+                            {"stepper": capi.STEPPER_SYMPY, "smoother": capi.SMOOTHER_MBF,
+                             "reference_surface": capi.REFERENCE_PERIGEE,
+                             "reference_strategy": capi.EXTRAPOLATE_FIRST},
+                            OUTPUTS + ("reference_params", "reference_cov"))
+    line["examples_configuration"] = {
+        "value": n / (mse * 1e-3), "ms_per_step": mse,
+        "forward_kernel_ms": float(np.mean([k[0] for k in kte])),
+        "config": "SympyStepper + MbfSmoother + perigee (first measurement)",
+        "fit_failures": int(np.count_nonzero(oute.host("status")))}
+    del oute
     line["pulls"] = quality.pulls(first64, events)
     if not args.no_mixed:
         msm, _, ktm, outm = run(capi.PRECISION_MIXED)

@@ -154,7 +154,7 @@ typedef struct acts_kf_field {
  * 141-305), the Examples' stepper (KalmanFitterFunction.cpp:39-44), whose
  * vacuum step carries the bound-to-free jacobian through an ATLAS-form RK4
  * (Core/src/Propagator/generate_sympy_stepper.py:157-444).  SYMPY is
- * supported with ACTS_KF_PRECISION_FP64 and without a reference surface.   */
+ * supported with ACTS_KF_PRECISION_FP64.                                    */
 #define ACTS_KF_STEPPER_EIGEN 0
 #define ACTS_KF_STEPPER_SYMPY 1
 #define ACTS_KF_SMOOTHER_GAIN_MATRIX 0

@@ -90,11 +90,10 @@ def test_create_rejects_unknown_options(lib, field, value):
     assert field.split("_")[-1] in msg or field.split("_")[0] in msg, msg
 
 
-@pytest.mark.parametrize("field,value", [("precision", capi.PRECISION_MIXED),
-                                         ("reference_surface", capi.REFERENCE_PERIGEE)])
+@pytest.mark.parametrize("field,value", [("precision", capi.PRECISION_MIXED)])
 def test_sympy_stepper_combinations_rejected(lib, field, value):
-    """The SympyStepper runs in FP64 without a reference surface; other
-    combinations fail acts_kf_create before any device call."""
+    """The SympyStepper runs in FP64; mixed precision with it fails
+    acts_kf_create before any device call."""
     import acts_amd as A
     det = A.barrel_detector()
     o = A.KalmanFitterOptions(stepper=capi.STEPPER_SYMPY).struct()

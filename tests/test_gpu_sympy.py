@@ -21,7 +21,7 @@ import acts_amd as A
 from acts_amd import capi
 from acts_amd.fitter import ALL_STATE_OUTPUTS
 
-from parity import compare
+from parity import compare, cov_rel_err, param_rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +47,29 @@ def test_sympy_parity(oracle_mod, workload, n, seed, smoother):
     print(rep["max_rel_err"], flush=True)
     assert rep["ok"], rep
     assert np.count_nonzero(got.status == 0) >= 0.98 * n
+
+
+def test_examples_configuration(oracle_mod):
+    """The Examples' fit: SympyStepper, MbfSmoother and the fitted
+    parameters extrapolated from the first measurement to a perigee at the
+    origin, against the oracle in the same configuration."""
+    det, field, ev = A.workload("solenoid", n_tracks=3000, seed=116)
+    opts = A.KalmanFitterOptions(stepper=capi.STEPPER_SYMPY, smoother=capi.SMOOTHER_MBF,
+                                 reference_surface=capi.REFERENCE_PERIGEE,
+                                 reference_strategy=capi.EXTRAPOLATE_FIRST)
+    out = ALL_STATE_OUTPUTS + ("reference_params", "reference_cov")
+    fitter = A.KalmanFitter(det, field, opts)
+    try:
+        got = fitter.fit(ev, outputs=out)
+    finally:
+        fitter.close()
+    ref, _ = oracle_mod.fit(det, field, opts, ev, n_threads=0, outputs=out)
+    rep = compare(got, ref, ev, 1e-10, rel_jacobian=1e-8)
+    print(rep["max_rel_err"], flush=True)
+    assert rep["ok"], rep
+    ok = (got.status == 0) & (ref.status == 0)
+    assert param_rel_err(got.reference_params[ok], ref.reference_params[ok]).max() <= 1e-10
+    assert cov_rel_err(got.reference_cov[ok], ref.reference_cov[ok]).max() <= 5e-8
 
 
 def test_sympy_transport_skip_bitwise():
