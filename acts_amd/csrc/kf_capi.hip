@@ -114,10 +114,13 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
       if (nStep == 0 || 64 * nWait >= waitFrac * (nStep + nWait)) break;
     }
     // a lane that reached a surface while stepping without the transport
-    // goes back to its checkpoint and repeats the steps with it
+    // goes back to its checkpoint and repeats the steps with it; a track
+    // that did so once (e.g. a looper that crosses a surface every turn)
+    // keeps the transport from then on
     if (phase == kPhaseSurface && L.noTransport) {
       L = a.ckpt[static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x];
       L.replaying = true;
+      L.noSkip = true;
       phase = kPhaseStepping;
     }
     if (phase == kPhaseSurface) phase = laneFilter<Mixed, Sympy>(L, a, geo, slabs);
@@ -374,7 +377,7 @@ struct acts_kf_ctx {
   bool sympy = false;  // SympyStepper instead of EigenStepper
   // lanes stepping without the transport save their state here, one Lane
   // per resident lane of the forward kernel
-  int32_t skipAfter = 64;
+  int32_t skipAfter = 0;
   DevBuf ckptBuf;
   // reference surface of the fitted parameters, and the default propagator
   // and stepper options its extrapolation runs with
@@ -429,7 +432,6 @@ void acts_kf_default_options(acts_kf_options* o) {
   o->mass = 0.1056583755f;
   o->abs_charge = 1.0f;
   o->length_sorted = 1;
-  o->sched_transport_skip_steps = 64;
 }
 
 const char* acts_kf_last_error(void) { return g_lastError.c_str(); }

@@ -62,6 +62,7 @@ struct Lane {
   uint32_t lastSurfIter;  // step count at the last surface
   bool noTransport;       // stepping without transport since a checkpoint
   bool replaying;         // back at the checkpoint: steps with transport
+  bool noSkip;            // replayed once: keeps the transport from now on
 };
 
 struct FwdArgs {
@@ -111,6 +112,7 @@ __device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a,
   L.lastSurfIter = 0;
   L.noTransport = false;
   L.replaying = false;
+  L.noSkip = false;
   L.T.nSteps = 0;
   L.T.nTrials = 0;
   // surface indices come from the caller: reject the track if any is out of
@@ -214,7 +216,7 @@ __device__ inline int laneStep(Lane& L, const FwdArgs& a, const DevSurface* geo)
     L.status = ACTS_KF_STATUS_STEP_COUNT_LIMIT;
     return kPhaseFinished;
   }
-  if (a.skipAfter > 0 && !L.noTransport && !L.replaying &&
+  if (a.skipAfter > 0 && !L.noTransport && !L.replaying && !L.noSkip &&
       L.iter - L.lastSurfIter >= static_cast<uint32_t>(a.skipAfter)) {
     a.ckpt[static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x] = L;
     L.noTransport = true;

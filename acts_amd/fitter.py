@@ -56,7 +56,7 @@ class KalmanFitterOptions:
     sched_quarantine_steps: int = 0         # 0 = off
     kernel_timing: bool = False             # events around the kernels
     use_graph: bool = False                 # replay fits as a captured hipGraph
-    sched_transport_skip_steps: int = 64    # 0 = off
+    sched_transport_skip_steps: int = 0     # 0 = off
     reserved_sched: int = 0
 
     def struct(self) -> capi.Options:

@@ -221,8 +221,9 @@ typedef struct acts_kf_options {
    * surface after all, it goes back to the saved state and repeats those
    * steps with the transport, so the results do not depend on it (a track
    * that never reaches another surface, e.g. a looper that ends at
-   * max_steps, uses no transport).  0 = off; default 64.  No reference
-   * counterpart.                                                             */
+   * max_steps, uses no transport).  0 = off (default): a track that does
+   * reach a surface repeats the skipped steps, which costs more than the
+   * loopers save in the field map.  No reference counterpart.               */
   int32_t sched_transport_skip_steps;
   int32_t reserved_sched;
 } acts_kf_options;
