@@ -1,6 +1,6 @@
 """Autotuner for the scheduling knobs of the fit kernels.
 
-The persistent forward kernel has four knobs that change only the speed of
+The persistent forward kernel has five knobs that change only the speed of
 a fit, never its results (acts_kf_options):
   sched_inner_steps  RK steps taken before the lanes waiting at a surface are
                      updated;
@@ -8,7 +8,12 @@ a fit, never its results (acts_kf_options):
                      before the surface update runs early;
   length_sorted      whether tracks enter the queue longest-first;
   sched_quarantine_steps  a wavefront holding a track older than this many
-                     steps takes no new tracks until it ends.
+                     steps takes no new tracks until it ends;
+  sched_transport_skip_steps  steps since the last surface after which a
+                     track steps without the jacobian transport (helps
+                     where loopers never reach another surface, as with
+                     outliers that end at the loop-protection limit; hurts
+                     where tracks do reach one, as in the field map).
 `autotune` times candidate settings on a device-resident sample of the
 workload by coordinate descent and keeps the fastest; results are cached in
 a JSON file keyed by the workload's shape.
@@ -27,9 +32,10 @@ SPACE = {
     "sched_wait_64ths": [16, 32, 48, 64],
     "length_sorted": [True, False],
     "sched_quarantine_steps": [0, 100, 200, 400],
+    "sched_transport_skip_steps": [0, 64, 256],
 }
 DEFAULTS = {"sched_inner_steps": 64, "sched_wait_64ths": 64, "length_sorted": True,
-            "sched_quarantine_steps": 0}
+            "sched_quarantine_steps": 0, "sched_transport_skip_steps": 0}
 
 
 def coordinate_descent(evaluate: Callable[[dict], float], space=SPACE, start=None,
