@@ -314,7 +314,8 @@ def main() -> int:
                      "achieved": achieved, "peak": FP64_VECTOR_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_VECTOR_PEAK_TFLOPS, "traffic": traffic,
                      "flop_per_launch": flop_fwd, "kernel_ms": fwd_ms,
-                     "smoother_ms": smooth_ms, "fit_flop_per_step": flop_all},
+                     "smoother_ms": smooth_ms, "fit_flop_per_step": flop_all,
+                     "forward_flop_rate_in_flight_tflops": flop_fwd / (ms64 * 1e-3) / 1e12},
         "wall_s_timed": wall64,
     }
     ms1 = single[(capi.PRECISION_FP64, ())]
