@@ -90,10 +90,10 @@ def batch_of(pool: Events, ranges) -> Events:
 
 
 def batch_bytes(n_tracks: int, max_len: int) -> float:
-    """Device memory of one batch: inputs, scratch (744 B per state slot),
+    """Device memory of one batch: inputs, scratch (748 B per state slot),
     serving outputs and per-track scratch."""
     per_track = 48 + 288 + 96 + 16 + 9 * 4 + 8
-    per_slot = 744 + 4 + 4 + 48 + 4 + 52
+    per_slot = 748 + 4 + 4 + 48 + 4 + 52
     return n_tracks * (per_track + max_len * per_slot)
 
 
