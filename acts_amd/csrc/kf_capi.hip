@@ -399,6 +399,8 @@ bool geometryFitsLds(const acts_kf_ctx* c) {
 }
 
 ForwardFn forwardKernel(bool mixed, bool geoInLds, bool sympy = false) {
+  if (sympy && mixed)
+    return geoInLds ? fitForwardKernel<true, true, true> : fitForwardKernel<true, false, true>;
   if (sympy) return geoInLds ? fitForwardKernel<false, true, true> : fitForwardKernel<false, false, true>;
   if (mixed) return geoInLds ? fitForwardKernel<true, true> : fitForwardKernel<true, false>;
   return geoInLds ? fitForwardKernel<false, true> : fitForwardKernel<false, false>;
@@ -487,8 +489,6 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: precision mode not supported");
   if (options->stepper != ACTS_KF_STEPPER_EIGEN && options->stepper != ACTS_KF_STEPPER_SYMPY)
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: stepper not supported");
-  if (options->stepper == ACTS_KF_STEPPER_SYMPY && options->precision != ACTS_KF_PRECISION_FP64)
-    return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: the SympyStepper runs in FP64");
   if (options->smoother != ACTS_KF_SMOOTHER_GAIN_MATRIX &&
       options->smoother != ACTS_KF_SMOOTHER_MBF)
     return fail(ACTS_KF_ERR_UNSUPPORTED, "acts_kf_create: smoother not supported");
@@ -811,7 +811,11 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   hipLaunchKernelGGL(c->mbf ? smoothKernel<true> : smoothKernel<false>,
                      dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream, b, d, sc, perm);
   if (withRef) {
-    if (c->sympy)
+    if (c->sympy && c->mixed)
+      hipLaunchKernelGGL((referenceKernel<true, true>), dim3(static_cast<unsigned>(blocks)),
+                         dim3(kBlock), 0, stream, d, sc, c->geo, c->field, c->extOpts,
+                         c->refLine, c->refStrategy, n);
+    else if (c->sympy)
       hipLaunchKernelGGL((referenceKernel<false, true>), dim3(static_cast<unsigned>(blocks)),
                          dim3(kBlock), 0, stream, d, sc, c->geo, c->field, c->extOpts,
                          c->refLine, c->refStrategy, n);

@@ -224,7 +224,9 @@ __device__ inline int laneStep(Lane& L, const FwdArgs& a, const DevSurface* geo)
   ++L.iter;
   const bool transport = !L.noTransport;
   int sst;
-  if constexpr (Mixed)
+  if constexpr (Mixed && Sympy)
+    sst = rkStepSympyMixed(T, a.F, o, 1.0, transport);
+  else if constexpr (Mixed)
     sst = rkStepMixed(T, a.F, o, 1.0, transport);
   else if constexpr (Sympy)
     sst = rkStepSympy(T, a.F, o, 1.0, transport);

@@ -258,4 +258,141 @@ __device__ inline int rkStepMixed(Track& T, const DevField& F, const DevOpts& o,
   return 0;
 }
 
+// The SympyStepper's step (rkStepSympy) in mixed precision: the stages,
+// the error estimate and the jacobian recursion in float; the straight-line
+// parts h dir of position and direction, the time, dt/ds, d(time)/d(q/p)
+// and the renormalisation in double, as in rkStepMixed.  In float the
+// error estimate is summed from the small stage terms,
+// kick1 - c2 - c3 + kick4 = h/2 (k1 - k2 - k3 + k4) with c_i the bends of
+// dir2 and dir3, and the q/p column's field terms dir3 - dir and dir4 - dir
+// are c2 and c3: the FP64 form's differences of O(1) vectors would leave
+// float noise of ~1e-7 in them.
+__device__ inline int rkStepSympyMixed(Track& T, const DevField& F, const DevOpts& o,
+                                       double propDir = 1.0, bool transport = true) {
+  auto add = [](const F3& a, const F3& b) { return mkf(a.x + b.x, a.y + b.y, a.z + b.z); };
+  auto sub = [](const F3& a, const F3& b) { return mkf(a.x - b.x, a.y - b.y, a.z - b.z); };
+  auto scl = [](float s, const F3& a) { return mkf(s * a.x, s * a.y, s * a.z); };
+  const F3 pos = tof(T.pos);
+  const F3 dir = tof(T.dir);
+  const float qop = static_cast<float>(T.qop);
+  const float tol = static_cast<float>(o.stepTolerance);
+  F3 B1;
+  if (const int fs = fieldAtF(F, pos, B1)) return fs;
+  const double initialH = stepValue(T) * propDir;
+  double hD = initialH;
+  float h = static_cast<float>(initialH);
+  float err, hThird;
+  F3 bend1, bend2, bend3, kick1, c2, c3, kick4, dir2, dir3, dirEnd;
+  int nTrials = 0;
+  while (true) {
+    ++nTrials;
+    const float halfHQop = h * qop * 0.5f;
+    hThird = h / 3.f;
+    const float hQuarter = h * 0.25f;
+    bend1 = scl(halfHQop, B1);
+    kick1 = crossf(dir, bend1);
+    dir2 = add(dir, kick1);
+    const F3 pos2 = add(pos, scl(hQuarter, add(dir2, dir)));
+    F3 B2;
+    if (const int fs = fieldAtF(F, pos2, B2)) return fs;
+    bend2 = scl(halfHQop, B2);
+    c2 = crossf(dir2, bend2);
+    dir3 = add(dir, c2);
+    c3 = crossf(dir3, bend2);
+    const F3 dir4 = add(dir, c3);
+    dirEnd = sub(scl(2.f, dir4), dir);
+    const F3 pos3 = add(pos, scl(h, dir4));
+    F3 B3;
+    if (const int fs = fieldAtF(F, pos3, B3)) return fs;
+    bend3 = scl(halfHQop, B3);
+    kick4 = crossf(dirEnd, bend3);
+    const F3 e = add(sub(sub(kick1, c2), c3), kick4);
+    err = 2.f * fabsf(h) * ((fabsf(e.x) + fabsf(e.y)) + fabsf(e.z));
+    err = fmaxf(1e-20f, err);
+    if (err <= 4.0f * tol) break;
+    h *= stepScalingF(tol, err);
+    hD = static_cast<double>(h);
+    if (fabs(hD) < fabs(o.stepSizeCutOff)) return ACTS_KF_STATUS_STEP_SIZE_STALLED;
+    if (static_cast<uint32_t>(nTrials) > o.maxTrials) return ACTS_KF_STATUS_STEP_ADJUSTMENT_FAILED;
+  }
+  const float m = o.mass;
+  const double pD = copysign(static_cast<double>(o.absQ), T.qop) / T.qop;
+  const double mpD = m / pD;
+  const double dtds = sqrt(1 + mpD * mpD);
+  T.time += hD * dtds;
+  if (transport) {
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      const F3 seed = mkf(static_cast<float>(T.br[c]), static_cast<float>(T.br[3 + c]),
+                          static_cast<float>(T.br[6 + c]));
+      const F3 tKick1 = crossf(seed, bend1);
+      const F3 tDir2 = add(tKick1, seed);
+      const F3 tDir3 = add(seed, crossf(tDir2, bend2));
+      const F3 tDir4 = add(seed, crossf(tDir3, bend2));
+      const F3 tDirEnd = sub(scl(2.f, tDir4), seed);
+      const F3 tKick4 = crossf(tDirEnd, bend3);
+      const F3 dp = add(add(tDir2, tDir3), tDir4);
+      const F3 dd = add(add(add(tKick1, scl(2.f, tDir3)), tDirEnd), tKick4);
+      T.tr[c] += static_cast<double>(hThird * dp.x);
+      T.tr[3 + c] += static_cast<double>(hThird * dp.y);
+      T.tr[6 + c] += static_cast<double>(hThird * dp.z);
+      T.br[c] = static_cast<double>(dd.x / 3.f);
+      T.br[3 + c] = static_cast<double>(dd.y / 3.f);
+      T.br[6 + c] = static_cast<double>(dd.z / 3.f);
+    }
+    {
+      const float invQop = 1.f / qop;
+      const float posW = hThird * invQop, dirW = invQop / 3.f;
+      const F3 seed = scl(qop, mkf(static_cast<float>(T.br[2]), static_cast<float>(T.br[5]),
+                                   static_cast<float>(T.br[8])));
+      const F3 tKick1 = add(kick1, crossf(seed, bend1));
+      const F3 tDir2 = add(tKick1, seed);
+      const F3 tDir3 = add(add(c2, seed), crossf(tDir2, bend2));
+      const F3 tDir4 = add(add(c3, seed), crossf(tDir3, bend2));
+      const F3 tDirEnd = sub(scl(2.f, tDir4), seed);
+      const F3 tKick4 = add(kick4, crossf(tDirEnd, bend3));
+      const F3 dp = add(add(tDir2, tDir3), tDir4);
+      const F3 dd = add(add(add(tKick1, scl(2.f, tDir3)), tDirEnd), tKick4);
+      T.tr[2] += static_cast<double>(posW * dp.x);
+      T.tr[5] += static_cast<double>(posW * dp.y);
+      T.tr[8] += static_cast<double>(posW * dp.z);
+      T.br[2] = static_cast<double>(dirW * dd.x);
+      T.br[5] = static_cast<double>(dirW * dd.y);
+      T.br[8] = static_cast<double>(dirW * dd.z);
+      T.tr37 += hD * m * m / (pD * pD * T.qop * dtds);
+    }
+  }
+  // position: h dir in double, the bends (h/3)(kick1 + c2 + c3) in float;
+  // direction: dir + (kick1 + 2 c2 + 2 c3 + kick4) / 3, renormalised
+  const F3 bp = scl(hThird, add(add(kick1, c2), c3));
+  T.pos.x += hD * T.dir.x + static_cast<double>(bp.x);
+  T.pos.y += hD * T.dir.y + static_cast<double>(bp.y);
+  T.pos.z += hD * T.dir.z + static_cast<double>(bp.z);
+  const F3 bd = scl(1.f / 3.f, add(add(add(kick1, scl(2.f, c2)), scl(2.f, c3)), kick4));
+  const V3 nd = mk(T.dir.x + static_cast<double>(bd.x), T.dir.y + static_cast<double>(bd.y),
+                   T.dir.z + static_cast<double>(bd.z));
+  const double nn = dot(nd, nd);
+  double inv = static_cast<double>(rsqrtf(static_cast<float>(nn)));
+  inv = inv * (1.5 - (0.5 * nn) * (inv * inv));
+  T.dir = mk(nd.x * inv, nd.y * inv, nd.z * inv);
+  if (transport) {
+    const float twoOverH = 2.f / h;
+    T.der[0] = T.dir.x;
+    T.der[1] = T.dir.y;
+    T.der[2] = T.dir.z;
+    T.der[3] = dtds;
+    T.der[4] = static_cast<double>(twoOverH * kick4.x);
+    T.der[5] = static_cast<double>(twoOverH * kick4.y);
+    T.der[6] = static_cast<double>(twoOverH * kick4.z);
+  }
+  T.pathAcc += hD;
+  ++T.nSteps;
+  T.nTrials += nTrials;
+  const double nextAccuracy = fabs(hD * static_cast<double>(stepScalingF(tol, err)));
+  const double previousAccuracy = fabs(T.cAcc);
+  const double initialStepLength = fabs(initialH);
+  if (nextAccuracy < initialStepLength || nextAccuracy > previousAccuracy) T.cAcc = nextAccuracy;
+  return 0;
+}
+
 }  // namespace kf

@@ -257,7 +257,9 @@ __device__ inline int propagateToLine(Track& T, const LineRef& L, const DevField
   if (checkAbort()) return 0;
   for (uint32_t steps = 0; steps < o.maxSteps; ++steps) {
     int sst;
-    if constexpr (Mixed)
+    if constexpr (Mixed && Sympy)
+      sst = rkStepSympyMixed(T, F, o, direction);
+    else if constexpr (Mixed)
       sst = rkStepMixed(T, F, o, direction);
     else if constexpr (Sympy)
       sst = rkStepSympy(T, F, o, direction);

@@ -355,6 +355,14 @@ def main() -> int:
         "first_state_rel_err_vs_eigen": quality.relative_error(
             quality.first_states(outs_, events), first64)}
     del outs_
+    if not args.no_mixed:
+        msx, _, ktx, outx = run(capi.PRECISION_MIXED, {"stepper": capi.STEPPER_SYMPY})
+        line["with_sympy_stepper"]["mixed"] = {
+            "value": n / (msx * 1e-3), "ms_per_step": msx,
+            "forward_kernel_ms": float(np.mean([k[0] for k in ktx])),
+            "first_state_rel_err_vs_fp64_eigen": quality.relative_error(
+                quality.first_states(outx, events), first64)}
+        del outx
     # the Examples' fit (KalmanFitterFunction.cpp, TrackFittingAlgorithm.cpp):
     # SympyStepper, MbfSmoother, parameters at a perigee from the first
     # measurement

@@ -153,8 +153,8 @@ typedef struct acts_kf_field {
  * the drop-in) or SympyStepper (Core/src/Propagator/SympyStepper.cpp:
  * 141-305), the Examples' stepper (KalmanFitterFunction.cpp:39-44), whose
  * vacuum step carries the bound-to-free jacobian through an ATLAS-form RK4
- * (Core/src/Propagator/generate_sympy_stepper.py:157-444).  SYMPY is
- * supported with ACTS_KF_PRECISION_FP64.                                    */
+ * (Core/src/Propagator/generate_sympy_stepper.py:157-444); with
+ * ACTS_KF_PRECISION_MIXED its stages and jacobian run in FP32.              */
 #define ACTS_KF_STEPPER_EIGEN 0
 #define ACTS_KF_STEPPER_SYMPY 1
 #define ACTS_KF_SMOOTHER_GAIN_MATRIX 0

@@ -89,18 +89,3 @@ def test_create_rejects_unknown_options(lib, field, value):
     msg = capi.last_error().lower()
     assert field.split("_")[-1] in msg or field.split("_")[0] in msg, msg
 
-
-@pytest.mark.parametrize("field,value", [("precision", capi.PRECISION_MIXED)])
-def test_sympy_stepper_combinations_rejected(lib, field, value):
-    """The SympyStepper runs in FP64; mixed precision with it fails
-    acts_kf_create before any device call."""
-    import acts_amd as A
-    det = A.barrel_detector()
-    o = A.KalmanFitterOptions(stepper=capi.STEPPER_SYMPY).struct()
-    setattr(o, field, value)
-    fld = A.ConstantField().struct()
-    ctx = C.c_void_p()
-    rc = lib.acts_kf_create(det.surface_table(), len(det.surfaces), det.material_table(),
-                            len(det.materials), C.byref(fld), C.byref(o), 0, C.byref(ctx))
-    assert rc == capi.ERR_UNSUPPORTED and not ctx.value
-    assert "sympystepper" in capi.last_error().lower()

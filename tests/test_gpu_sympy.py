@@ -21,7 +21,7 @@ import acts_amd as A
 from acts_amd import capi
 from acts_amd.fitter import ALL_STATE_OUTPUTS
 
-from parity import compare, cov_rel_err, param_rel_err
+from parity import compare, cov_rel_err, error_report, param_rel_err, pulls
 
 pytestmark = pytest.mark.gpu
 
@@ -70,6 +70,28 @@ def test_examples_configuration(oracle_mod):
     ok = (got.status == 0) & (ref.status == 0)
     assert param_rel_err(got.reference_params[ok], ref.reference_params[ok]).max() <= 1e-10
     assert cov_rel_err(got.reference_cov[ok], ref.reference_cov[ok]).max() <= 5e-8
+
+
+def test_sympy_mixed_precision(oracle_mod):
+    """The SympyStepper with FP32 stages and jacobian recursion against the
+    FP64 oracle's SympyStepper, to the mixed-precision bounds of the
+    EigenStepper (tests/test_gpu_mixed.py): status of >= 99.5 % of the tracks,
+    smoothed-parameter RMS < 1e-6 and 99.9th percentile < 1e-5 over the
+    converged fits, covariances within 1e-3, pulls as the FP64 fit's."""
+    det, field, ev = A.workload("solenoid", n_tracks=6000, seed=117)
+    got = _fit(det, field, ev, A.KalmanFitterOptions(stepper=capi.STEPPER_SYMPY,
+                                                     precision=capi.PRECISION_MIXED))
+    ref, _ = oracle_mod.fit(det, field, A.KalmanFitterOptions(stepper=capi.STEPPER_SYMPY), ev,
+                            n_threads=0)
+    rep = error_report(got, ref, ev, max_chi2_ndf=100.0)
+    pg, pr = pulls(got, ev), pulls(ref, ev)
+    print(rep, flush=True)
+    assert rep["status_mismatch"] <= 0.005 * ev.n_tracks
+    assert max(rep["smoothed_rms_rel"]) < 1e-6
+    assert max(rep["smoothed_p999_rel"]) < 1e-5
+    assert rep["smoothed_cov_max_rel"] < 1e-3
+    assert np.all(np.abs(np.array(pg["mean"]) - pr["mean"]) < 0.002)
+    assert np.all(np.abs(np.array(pg["std"]) / pr["std"] - 1) < 0.002)
 
 
 def test_sympy_transport_skip_bitwise():
