@@ -1,0 +1,141 @@
+"""GPU parity on edge cases (tests/edge_cases.py): tracks with no
+measurements, an empty surface sequence, a single measurement, a hole, a
+measurement on a surface the sequence does not contain (the fit runs on to
+the loop-protection path limit), and measurements on the last two surfaces
+only, among ordinary tracks; with both steppers and both smoothers, and
+through the track-state table.  An empty batch is a no-op.
+
+Bar: as tests/test_gpu_parity.py, and the step and trial counts of every
+track identical to the oracle's, except the smoothed covariance: a track
+with eight unmeasured states before its two measurements has predicted
+covariances so ill-conditioned that the oracle's own RTS and MBF smoothers
+(mathematically equal) differ by up to 5.8e-8 on it, so the bar there is
+four times the largest RTS-vs-MBF difference of the oracle on the batch
+(and at least the usual 5e-8).
+"""
+import numpy as np
+import pytest
+
+import acts_amd as A
+from acts_amd import capi
+from acts_amd import track_states as TS
+from acts_amd.device import DeviceEvents, DeviceResult, to_device
+from acts_amd.fitter import ALL_STATE_OUTPUTS, FitResult, TRACK_ARRAYS
+
+from edge_cases import edge_batch
+from parity import compare, cov_rel_err
+
+pytestmark = pytest.mark.gpu
+
+OPTIONS = {
+    "eigen": dict(),
+    "sympy": dict(stepper=capi.STEPPER_SYMPY),
+    "mbf": dict(smoother=capi.SMOOTHER_MBF),
+}
+
+
+def _smoother_spread(oracle_mod, det, field, workload, eb):
+    """Largest difference of the oracle's RTS and MBF smoothed covariances."""
+    out = []
+    for sm in (capi.SMOOTHER_GAIN_MATRIX, capi.SMOOTHER_MBF):
+        r, _ = oracle_mod.fit(det, field, A.workload_options(workload, smoother=sm), eb,
+                              n_threads=0, outputs=ALL_STATE_OUTPUTS)
+        out.append(r)
+    a, b = out
+    so = eb.seq_offsets
+    ok = np.nonzero(a.status == 0)[0]
+    idx = np.concatenate([np.arange(so[t], so[t] + a.n_states[t]) for t in ok])
+    return float(cov_rel_err(a.smoothed_cov[idx], b.smoothed_cov[idx]).max())
+
+
+@pytest.mark.parametrize("workload,seed", [("barrel", 9), ("endcap", 19)])
+@pytest.mark.parametrize("variant", sorted(OPTIONS))
+def test_edge_cases_parity(oracle_mod, workload, seed, variant):
+    print("edge cases", workload, seed, variant, flush=True)
+    det, field, ev = A.workload(workload, n_tracks=60, seed=seed)
+    eb, kinds = edge_batch(ev, n_ordinary=40)
+    opts = A.workload_options(workload, **OPTIONS[variant])
+    f = A.KalmanFitter(det, field, opts)
+    try:
+        got = f.fit(eb, outputs=ALL_STATE_OUTPUTS)
+    finally:
+        f.close()
+    ref, _ = oracle_mod.fit(det, field, opts, eb, n_threads=0, outputs=ALL_STATE_OUTPUTS)
+    bar = max(5e-8, 4 * _smoother_spread(oracle_mod, det, field, workload, eb))
+    rep = compare(got, ref, eb, 1e-10, rel_smoothed_cov=bar,
+                  rel_jacobian=1e-8 if variant == "sympy" else None)
+    print("smoothed covariance bar", bar, flush=True)
+    print(rep, flush=True)
+    assert rep["ok"], rep
+    assert np.array_equal(got.n_steps, ref.n_steps), [
+        (kinds[t], got.n_steps[t], ref.n_steps[t]) for t in np.nonzero(got.n_steps != ref.n_steps)[0]]
+    assert np.array_equal(got.n_step_trials, ref.n_step_trials)
+    for k in ("no measurements", "empty sequence"):
+        assert got.status[kinds.index(k)] == capi.STATUS_NO_MEASUREMENT_FOUND
+
+
+def test_edge_cases_mixed_precision():
+    det, field, ev = A.workload("barrel", n_tracks=60, seed=9)
+    eb, kinds = edge_batch(ev, n_ordinary=40)
+    out = {}
+    for p in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+        print("edge cases precision", p, flush=True)
+        f = A.KalmanFitter(det, field, A.KalmanFitterOptions(precision=p))
+        try:
+            out[p] = f.fit(eb, outputs=ALL_STATE_OUTPUTS)
+        finally:
+            f.close()
+    a, b = out[capi.PRECISION_FP64], out[capi.PRECISION_MIXED]
+    for name in ("status", "n_states", "n_measurements", "n_holes", "ndf"):
+        assert np.array_equal(getattr(a, name), getattr(b, name)), name
+
+
+def test_edge_cases_track_states(oracle_mod):
+    print("edge cases track states", flush=True)
+    det, field, ev = A.workload("barrel", n_tracks=60, seed=9)
+    eb, kinds = edge_batch(ev, n_ordinary=40)
+    opts = A.KalmanFitterOptions()
+    outputs = ("state_surface", "state_flags", "state_path", "state_chi2", "state_measurement",
+               "predicted", "predicted_cov", "filtered", "filtered_cov", "smoothed",
+               "smoothed_cov")
+    f = A.KalmanFitter(det, field, opts)
+    try:
+        din = DeviceEvents(eb, "cuda:0")
+        dres = DeviceResult(eb.n_tracks, din.n_slots, outputs, "cuda:0")
+        f.fit_device(din.batch_in(), dres.batch_out())
+        truth = to_device(eb.truth_meas, "cuda:0")
+        floats, ints = TS.device_table(f, din, dres, truth)
+        f.sync()
+        got = TS.host_table(floats, ints)
+    finally:
+        f.close()
+    fr = FitResult(eb.n_tracks, int(eb.seq_offsets[-1]), outputs)
+    for name in list(TRACK_ARRAYS) + list(outputs):
+        arr = getattr(fr, name)
+        arr[...] = dres.host(name).view(arr.dtype).reshape(arr.shape)
+    ref = oracle_mod.track_states(det, field, opts, eb, fr, eb.truth_meas)
+    assert np.array_equal(got.ints, ref.ints)
+    g, r = got.floats, ref.floats
+    assert np.array_equal(np.isnan(g), np.isnan(r))
+    fin = ~np.isnan(r)
+    ulp = np.spacing(np.maximum(np.abs(g[fin]), np.abs(r[fin])).astype(np.float32))
+    assert np.all(np.abs(g[fin].astype(np.float64) - r[fin]) <= 2 * ulp)
+    # the failed tracks have no rows with parameters
+    track = got.column("track")
+    for k in ("no measurements", "empty sequence"):
+        rows = track == kinds.index(k)
+        assert np.all(got.column("stateType")[rows] == -1)
+
+
+def test_empty_batch():
+    print("empty batch", flush=True)
+    det, field, ev = A.workload("barrel", n_tracks=10, seed=9)
+    empty = ev.subset(np.zeros(0, np.int64))
+    f = A.KalmanFitter(det, field, A.KalmanFitterOptions())
+    try:
+        res = f.fit(empty, outputs=ALL_STATE_OUTPUTS)
+        assert res.status.shape == (0,)
+        res = f.fit(ev, outputs=ALL_STATE_OUTPUTS)
+        assert np.all(res.status == 0)
+    finally:
+        f.close()
