@@ -2,18 +2,27 @@
 measurements, an empty surface sequence, a single measurement, a hole (a
 measurement taken away, its surface kept), a measurement on a surface that
 the track's sequence does not contain (never reached) together with the
-same track without that measurement, and measurements only on the last
-two surfaces, followed by ordinary tracks."""
+same track without that measurement, measurements only on the last two
+surfaces, and a second, displaced measurement on a surface that already
+has one together with the same track without it, followed by ordinary
+tracks.  `padded` moves a detector's surfaces to high indices."""
 from __future__ import annotations
+
+import dataclasses
 
 import numpy as np
 
+from acts_amd import capi
+from acts_amd.detector import Detector, SurfaceDesc
 from acts_amd.events import Events
 
 # (kind, index of the source track)
 CASES = (("no measurements", 0), ("empty sequence", 1), ("single measurement", 2),
          ("hole", 3), ("unreachable measurement", 4), ("unreachable removed", 4),
-         ("last two measurements only", 5))
+         ("last two measurements only", 5), ("duplicate measurement", 6),
+         ("duplicate control", 6))
+# loc0 displacement of the second measurement on a surface [mm]
+DUPLICATE_SHIFT = 0.5
 
 
 def _edit(kind, seq, mi, ev):
@@ -35,6 +44,9 @@ def _edit(kind, seq, mi, ev):
         return seq, (mi if kind == "unreachable measurement" else [x for x in mi if x != m])
     if kind == "last two measurements only":
         return seq, mi[-2:]
+    if kind == "duplicate measurement":
+        k = len(mi) // 2
+        return seq, mi[:k + 1] + [mi[k]] + mi[k + 1:]
     return seq, mi
 
 
@@ -70,4 +82,21 @@ def edge_batch(ev: Events, n_ordinary: int = 10) -> tuple[Events, tuple]:
         truth_start=np.ascontiguousarray(ev.truth_start[src]),
         truth_meas=np.ascontiguousarray(ev.truth_meas[mid]),
         meas_is_outlier=np.ascontiguousarray(ev.meas_is_outlier[mid]))
+    for i, (kind, t) in enumerate(cases):
+        if kind == "duplicate measurement":
+            k = int(mo[t + 1] - mo[t]) // 2
+            out.meas_values[new_mo[i] + k + 1, 0] += DUPLICATE_SHIFT
     return out, tuple(k for k, _ in cases)
+
+
+def padded(det: Detector, ev: Events, n_pad: int) -> tuple[Detector, Events]:
+    """The detector with `n_pad` unused cylinders (r = 5 m, no material)
+    placed before its surfaces, and the events with their surface indices
+    moved accordingly."""
+    pad = [SurfaceDesc(capi.SURFACE_CYLINDER, np.eye(3), np.zeros(3), (5000.0, 3000.0), -1, False)
+           for _ in range(n_pad)]
+    d = Detector(surfaces=pad + list(det.surfaces), materials=list(det.materials),
+                 name=det.name + "+pad")
+    e = dataclasses.replace(ev, seq_surfaces=(ev.seq_surfaces + n_pad).astype(np.int32),
+                            meas_surfaces=(ev.meas_surfaces + n_pad).astype(np.int32))
+    return d, e

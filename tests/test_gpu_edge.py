@@ -139,3 +139,61 @@ def test_empty_batch():
         assert np.all(res.status == 0)
     finally:
         f.close()
+
+
+def test_duplicate_measurement_first_wins(oracle_mod):
+    """The displaced second measurement on a surface is ignored on the GPU
+    too: the fit equals that of the track without it."""
+    det, field, ev = A.workload("endcap", n_tracks=60, seed=19)
+    eb, kinds = edge_batch(ev, n_ordinary=5)
+    f = A.KalmanFitter(det, field, A.KalmanFitterOptions())
+    try:
+        got = f.fit(eb, outputs=ALL_STATE_OUTPUTS)
+    finally:
+        f.close()
+    a, b = kinds.index("duplicate measurement"), kinds.index("duplicate control")
+    so = eb.seq_offsets
+    sa = np.arange(so[a], so[a] + got.n_states[a])
+    sb = np.arange(so[b], so[b] + got.n_states[b])
+    assert got.status[a] == got.status[b] == 0 and got.chi2[a] == got.chi2[b]
+    for name in ("filtered", "smoothed", "smoothed_cov"):
+        assert np.array_equal(getattr(got, name)[sa], getattr(got, name)[sb]), name
+
+
+@pytest.mark.parametrize("precision", [capi.PRECISION_FP64, capi.PRECISION_MIXED])
+@pytest.mark.parametrize("stepper", [capi.STEPPER_EIGEN, capi.STEPPER_SYMPY])
+def test_large_geometry(oracle_mod, precision, stepper):
+    """300 more surfaces: the geometry no longer fits the forward kernel's
+    on-chip copy (128 surfaces) and is read from global memory, and the
+    measurement surfaces lie above 255.  The fit is bit for bit the one of
+    the small geometry, with every state's surface index moved; in FP64 it
+    also matches the oracle."""
+    from edge_cases import padded
+    print("large geometry", precision, stepper, flush=True)
+    det, field, ev = A.workload("endcap", n_tracks=2000, seed=23)
+    eb, kinds = edge_batch(ev, n_ordinary=1500)
+    pdet, peb = padded(det, eb, 300)
+    opts = A.KalmanFitterOptions(precision=precision, stepper=stepper)
+    out = []
+    for d, e in ((det, eb), (pdet, peb)):
+        f = A.KalmanFitter(d, field, opts)
+        try:
+            out.append(f.fit(e, outputs=ALL_STATE_OUTPUTS))
+        finally:
+            f.close()
+    small, large = out
+    for name in ("status", "n_states", "chi2", "n_steps", "n_step_trials", "smoothed",
+                 "smoothed_cov", "predicted_cov", "jacobian", "state_path"):
+        assert np.array_equal(getattr(large, name), getattr(small, name)), name
+    ok = small.status == 0
+    so = eb.seq_offsets
+    idx = np.concatenate([np.arange(so[t], so[t] + small.n_states[t]) for t in np.nonzero(ok)[0]])
+    assert np.array_equal(large.state_surface[idx], small.state_surface[idx] + 300)
+    if precision == capi.PRECISION_FP64:
+        ref, _ = oracle_mod.fit(pdet, field, opts, peb, n_threads=0, outputs=ALL_STATE_OUTPUTS)
+        rep = compare(large, ref, peb, 1e-10,
+                      rel_jacobian=1e-8 if stepper == capi.STEPPER_SYMPY else None,
+                      rel_smoothed_cov=max(5e-8, 4 * _smoother_spread(
+                          oracle_mod, pdet, field, "endcap", peb)))
+        print(rep, flush=True)
+        assert rep["ok"], rep

@@ -94,3 +94,37 @@ def test_last_two_measurements(edge_fit):
     # states before the first measurement (material surfaces) are no holes
     assert np.count_nonzero(flags & FLAG_MEAS) == 2
     assert not np.any(flags & FLAG_HOLE)
+
+
+def test_duplicate_measurement(edge_fit):
+    """Of two measurements on one surface the first is used
+    (KalmanFitter.hpp:750-755, try_emplace keeps the first): the fit equals
+    that of the track without the displaced second one."""
+    _, eb, kinds, r = edge_fit
+    a, b = kinds.index("duplicate measurement"), kinds.index("duplicate control")
+    assert r.status[a] == r.status[b] == 0
+    for name in ("n_states", "n_measurements", "n_holes", "chi2", "ndf", "n_steps"):
+        assert getattr(r, name)[a] == getattr(r, name)[b], name
+    sa, sb = _states(r, eb, a), _states(r, eb, b)
+    for name in ("filtered", "smoothed", "smoothed_cov", "state_chi2"):
+        assert np.array_equal(getattr(r, name)[sa], getattr(r, name)[sb]), name
+
+
+def test_padded_surface_indices(oracle_mod):
+    """Surface indices above 255 (the fitter's bit-set of measurement
+    surfaces covers 0..255) and a geometry table larger than the fast
+    on-chip copy: the fit is unchanged."""
+    from edge_cases import padded
+    det, field, ev = A.workload("endcap", n_tracks=40, seed=19)
+    eb, kinds = edge_batch(ev, n_ordinary=20)
+    pdet, peb = padded(det, eb, 300)
+    r0, _ = oracle_mod.fit(det, field, A.KalmanFitterOptions(), eb, n_threads=0,
+                           outputs=ALL_STATE_OUTPUTS)
+    r1, _ = oracle_mod.fit(pdet, field, A.KalmanFitterOptions(), peb, n_threads=0,
+                           outputs=ALL_STATE_OUTPUTS)
+    for name in ("status", "n_states", "chi2", "n_steps", "smoothed", "smoothed_cov"):
+        assert np.array_equal(getattr(r0, name), getattr(r1, name)), name
+    ok = r0.status == 0
+    so = eb.seq_offsets
+    idx = np.concatenate([np.arange(so[t], so[t] + r0.n_states[t]) for t in np.nonzero(ok)[0]])
+    assert np.array_equal(r1.state_surface[idx], r0.state_surface[idx] + 300)
