@@ -100,3 +100,17 @@ def padded(det: Detector, ev: Events, n_pad: int) -> tuple[Detector, Events]:
     e = dataclasses.replace(ev, seq_surfaces=(ev.seq_surfaces + n_pad).astype(np.int32),
                             meas_surfaces=(ev.meas_surfaces + n_pad).astype(np.int32))
     return d, e
+
+
+def shuffled(ev: Events, seed: int) -> Events:
+    """The events with each track's measurements in a random order."""
+    rng = np.random.default_rng(seed)
+    mo = ev.meas_offsets
+    perm = np.concatenate([mo[t] + rng.permutation(int(mo[t + 1] - mo[t]))
+                           for t in range(ev.n_tracks)]).astype(np.int64)
+    return dataclasses.replace(
+        ev, meas_surfaces=np.ascontiguousarray(ev.meas_surfaces[perm]),
+        meas_values=np.ascontiguousarray(ev.meas_values[perm]),
+        meas_cov=np.ascontiguousarray(ev.meas_cov[perm]),
+        truth_meas=np.ascontiguousarray(ev.truth_meas[perm]),
+        meas_is_outlier=np.ascontiguousarray(ev.meas_is_outlier[perm]))

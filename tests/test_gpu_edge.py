@@ -250,3 +250,65 @@ def test_long_sequences(oracle_mod, smoother):
     print(rep, flush=True)
     assert rep["ok"], rep
     assert np.array_equal(got.n_steps, ref.n_steps)
+
+
+@pytest.mark.parametrize("stepper", [capi.STEPPER_EIGEN, capi.STEPPER_SYMPY])
+def test_zero_field(oracle_mod, stepper):
+    """Straight tracks (B = 0, the reference's own KalmanFitter test
+    setting) through barrel and endcap discs, against the oracle; the mixed
+    precision fit agrees on every status and state count."""
+    print("zero field", stepper, flush=True)
+    det = A.barrel_endcap_detector()
+    field = A.ConstantField(bz=0.0)
+    ev = A.generate(det, field, A.gen_config(3000, 66, pt=(1.0, 10.0), eta=(-2.5, 2.5),
+                                             hole_probability=0.05))
+    out = {}
+    for p in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+        f = A.KalmanFitter(det, field, A.KalmanFitterOptions(stepper=stepper, precision=p))
+        try:
+            out[p] = f.fit(ev, outputs=ALL_STATE_OUTPUTS)
+        finally:
+            f.close()
+    got = out[capi.PRECISION_FP64]
+    ref, _ = oracle_mod.fit(det, field, A.KalmanFitterOptions(stepper=stepper), ev, n_threads=0,
+                            outputs=ALL_STATE_OUTPUTS)
+    rep = compare(got, ref, ev, 1e-10, rel_jacobian=1e-8 if stepper == capi.STEPPER_SYMPY else None)
+    print(rep, flush=True)
+    assert rep["n_ok"] >= 0.99 * ev.n_tracks
+    assert rep["ok"], rep
+    assert np.array_equal(got.n_steps, ref.n_steps)
+    for name in ("status", "n_states", "n_measurements", "n_holes"):
+        assert np.array_equal(getattr(out[capi.PRECISION_MIXED], name), getattr(got, name)), name
+
+
+@pytest.mark.parametrize("precision", [capi.PRECISION_FP64, capi.PRECISION_MIXED])
+def test_shuffled_measurements(precision):
+    """Each track's measurements in a random order (the reference's
+    ZeroFieldShuffled case): the fit is bit for bit the ordered one, with
+    every state pointing at the same measurement."""
+    from edge_cases import shuffled
+    print("shuffled", precision, flush=True)
+    det, field, ev = A.workload("ragged", n_tracks=3000, seed=71)
+    sev = shuffled(ev, 5)
+    out = []
+    for e in (ev, sev):
+        f = A.KalmanFitter(det, field, A.workload_options("ragged", precision=precision))
+        try:
+            out.append(f.fit(e, outputs=ALL_STATE_OUTPUTS))
+        finally:
+            f.close()
+    a, b = out
+    for name in ("status", "n_states", "chi2", "n_outliers", "n_holes", "n_steps", "smoothed",
+                 "smoothed_cov", "filtered_cov", "state_flags"):
+        assert np.array_equal(getattr(a, name), getattr(b, name)), name
+    # the measurement each state uses, as an index into the ordered input
+    mo = ev.meas_offsets
+    rng = np.random.default_rng(5)
+    perm = np.concatenate([mo[t] + rng.permutation(int(mo[t + 1] - mo[t]))
+                           for t in range(ev.n_tracks)])
+    so = ev.seq_offsets
+    idx = np.concatenate([np.arange(so[t], so[t] + a.n_states[t]) for t in np.nonzero(a.status == 0)[0]])
+    m = b.state_measurement[idx]
+    has = m >= 0
+    assert np.count_nonzero(has) > 0
+    assert np.array_equal(np.where(has, perm[np.where(has, m, 0)], -1), a.state_measurement[idx])

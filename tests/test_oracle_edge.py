@@ -128,3 +128,19 @@ def test_padded_surface_indices(oracle_mod):
     so = eb.seq_offsets
     idx = np.concatenate([np.arange(so[t], so[t] + r0.n_states[t]) for t in np.nonzero(ok)[0]])
     assert np.array_equal(r1.state_surface[idx], r0.state_surface[idx] + 300)
+
+
+def test_shuffled_measurements(oracle_mod):
+    """The order of a track's measurements does not matter: the fitter
+    looks them up by surface (KalmanFitter.hpp:750-755; the reference's
+    ZeroFieldShuffled case, FitterTestsCommon.hpp)."""
+    from edge_cases import shuffled
+    det, field, ev = A.workload("ragged", n_tracks=500, seed=71)
+    opts = A.workload_options("ragged")
+    r0, _ = oracle_mod.fit(det, field, opts, ev, n_threads=0, outputs=ALL_STATE_OUTPUTS)
+    r1, _ = oracle_mod.fit(det, field, opts, shuffled(ev, 5), n_threads=0,
+                           outputs=ALL_STATE_OUTPUTS)
+    assert np.count_nonzero(r0.n_outliers) > 0 and np.count_nonzero(r0.n_holes) > 0
+    for name in ("status", "n_states", "chi2", "n_outliers", "n_holes", "n_steps", "smoothed",
+                 "smoothed_cov", "state_flags"):
+        assert np.array_equal(getattr(r0, name), getattr(r1, name)), name
