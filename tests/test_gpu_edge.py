@@ -312,3 +312,35 @@ def test_shuffled_measurements(precision):
     has = m >= 0
     assert np.count_nonzero(has) > 0
     assert np.array_equal(np.where(has, perm[np.where(has, m, 0)], -1), a.state_measurement[idx])
+
+
+def test_surface_index_out_of_range():
+    """A sequence that names a surface outside the geometry fails its track
+    (PropagationFailure) before any geometry is read; the other tracks of
+    the batch are fitted as without it."""
+    import dataclasses
+    print("surface index out of range", flush=True)
+    det, field, ev = A.workload("endcap", n_tracks=500, seed=81)
+    seq = ev.seq_surfaces.copy()
+    so = ev.seq_offsets
+    bad = {3: len(det.surfaces), 7: -1, 11: 1 << 20}
+    for t, v in bad.items():
+        seq[so[t] + 1] = v
+    bev = dataclasses.replace(ev, seq_surfaces=seq)
+    out = []
+    for e in (ev, bev):
+        f = A.KalmanFitter(det, field, A.KalmanFitterOptions())
+        try:
+            out.append(f.fit(e, outputs=ALL_STATE_OUTPUTS))
+        finally:
+            f.close()
+    good, got = out
+    rows = np.array(sorted(bad))
+    assert np.all(got.status[rows] == capi.STATUS_PROPAGATION_FAILURE)
+    assert np.all(got.n_states[rows] == 0)
+    keep = np.setdiff1d(np.arange(ev.n_tracks), rows)
+    for name in ("status", "n_states", "chi2", "n_steps"):
+        assert np.array_equal(getattr(got, name)[keep], getattr(good, name)[keep]), name
+    slots = np.concatenate([np.arange(so[t], so[t + 1]) for t in keep])
+    for name in ("smoothed", "smoothed_cov"):
+        assert np.array_equal(getattr(got, name)[slots], getattr(good, name)[slots]), name
