@@ -1137,9 +1137,21 @@ void initializeStepper(Stepper& st, const Surface& s, const Vec6& bp,
 }
 
 // ===========================================================================
-// PointwiseMaterialInteraction.hpp/.cpp (forward direction only: +1)
+// PointwiseMaterialInteraction.hpp/.cpp
 // ===========================================================================
 enum MatMode { NoUpdate = 0, PreUpdate = 1, PostUpdate = 2, FullUpdate = 3 };
+
+// ORACLE_PLANT selects a deliberately wrong material application, compiled
+// only into the test libraries liboracle_kf_plant<N>.so, so that the tests
+// can show that their closed-form checks detect it:
+//   1: material applied before the update on measurement surfaces (forward
+//      PreUpdate factor 1, PostUpdate factor 0),
+//   2: phi variance theta0^2 without the 1/sin^2(theta) factor,
+//   3: no 10 MeV momentum floor after the energy loss.
+// The oracle proper is built with 0.
+#ifndef ORACLE_PLANT
+#define ORACLE_PLANT 0
+#endif
 
 int performMaterialInteraction(const Surface& s, Stepper& st,
                                const Particle& ph, const acts_kf_options& o,
@@ -1157,9 +1169,9 @@ int performMaterialInteraction(const Surface& s, Stepper& st,
     if (mode == FullUpdate)
       factor = 1.;
     else if (mode == PreUpdate)
-      factor = propDir < 0 ? 1. : 0.;
+      factor = (ORACLE_PLANT == 1) ? 1. : (propDir < 0 ? 1. : 0.);
     else if (mode == PostUpdate)
-      factor = propDir > 0 ? 1. : 0.;
+      factor = (ORACLE_PLANT == 1) ? 0. : (propDir > 0 ? 1. : 0.);
     if (factor == 0.) return 0;  // MaterialSlab::Nothing()
     slab.scaleThickness(static_cast<float>(factor));
   }
@@ -1178,7 +1190,7 @@ int performMaterialInteraction(const Surface& s, Stepper& st,
     if (o.multiple_scattering) {
       const double theta0 = multipleScatteringTheta0(
           slab, ph.absPdg, static_cast<float>(mass), qopF, static_cast<float>(absQ));
-      const double sigmaPhi = theta0 * (norm(dir) / perp(dir));
+      const double sigmaPhi = (ORACLE_PLANT == 2) ? theta0 : theta0 * (norm(dir) / perp(dir));
       varPhi = sigmaPhi * sigmaPhi;
       varTheta = theta0 * theta0;
     }
@@ -1193,7 +1205,7 @@ int performMaterialInteraction(const Surface& s, Stepper& st,
   const double momentum = ph.extractMomentum(st.qop());
   const double nextE = std::sqrt(mass * mass + momentum * momentum) - eLoss * propDir;
   double nextP = (mass < nextE) ? fastCathetus1(nextE, mass) : 0;
-  nextP = std::max(10 * kMeV, nextP);
+  if (ORACLE_PLANT != 3) nextP = std::max(10 * kMeV, nextP);
   const double nextQop =
       ph.qOverP(nextP, static_cast<float>(std::copysign(absQ, qopF)));
   st.pars.a[7][0] = nextQop;

@@ -39,7 +39,29 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         build()
-    lib = C.CDLL(LIB_PATH)
+    _lib = _bind(C.CDLL(LIB_PATH))
+    return _lib
+
+
+_plants = {}
+
+
+def load_plant(n: int):
+    """A test-only build of the oracle with planted mistake number `n` in the
+    material application (kf_oracle.cpp, ORACLE_PLANT): tests show that their
+    checks fail on it."""
+    if n in _plants:
+        return _plants[n]
+    name = f"liboracle_kf_plant{int(n)}.so"
+    path = os.path.join(ORACLE_DIR, name)
+    res = subprocess.run(["make", "-C", ORACLE_DIR, name], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("oracle plant build failed:\n" + res.stdout + res.stderr)
+    _plants[n] = _bind(C.CDLL(path))
+    return _plants[n]
+
+
+def _bind(lib):
     dp = C.POINTER(C.c_double)
     lib.acts_oracle_fit.restype = C.c_int
     lib.acts_oracle_fit.argtypes = [C.POINTER(capi.Surface), C.c_int32,
@@ -79,7 +101,6 @@ def load():
                                               C.c_int64, dp, dp]
     lib.acts_oracle_elliptic.restype = None
     lib.acts_oracle_elliptic.argtypes = [C.c_double, dp, dp]
-    _lib = lib
     return lib
 
 
@@ -120,9 +141,10 @@ def _dp(a: np.ndarray):
 
 
 def fit(detector, field, options, events, n_threads: int = 0,
-        outputs=ALL_STATE_OUTPUTS):
-    """Fit `events` on the CPU; returns (FitResult, wall seconds)."""
-    lib = load()
+        outputs=ALL_STATE_OUTPUTS, lib=None):
+    """Fit `events` on the CPU; returns (FitResult, wall seconds).  `lib`
+    selects another build of the oracle (load_plant)."""
+    lib = lib or load()
     res = FitResult(events.n_tracks, int(events.seq_offsets[-1]), outputs)
     surf = detector.surface_table()
     mats = detector.material_table()
