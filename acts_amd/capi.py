@@ -158,6 +158,8 @@ class BatchIn(C.Structure):
         ("meas_cov", _P),
         ("max_sequence_length", C.c_int32),
         ("reserved", C.c_int32),
+        ("meas_dims", _P),
+        ("meas_indices", _P),
     ]
 
 

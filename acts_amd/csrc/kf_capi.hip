@@ -157,12 +157,14 @@ template <bool Mixed, bool Sympy = false>
 __global__ __launch_bounds__(kBlock) void referenceKernel(DevOut out, DevScratch sc,
                                                           const DevSurface* __restrict__ geo,
                                                           DevField F, DevOpts o, LineRef L,
-                                                          int32_t strategy, int64_t n) {
+                                                          int32_t strategy, int64_t n,
+                                                          const int64_t* __restrict__ seqOff) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= n || sc.track[t] != 0) return;
   double p[6], C[36];
   const int st = extrapolateTrack<Mixed, Sympy>(t, sc, sc.ref, geo, F, o, L, strategy, p, C);
   if (st) {
+    for (int64_t slot = seqOff[t]; slot < seqOff[t + 1]; ++slot) clearSlot(out, slot);
     if (out.status) out.status[t] = st;
     if (out.nStates) out.nStates[t] = 0;
     if (out.chi2) out.chi2[t] = 0.f;
@@ -193,6 +195,9 @@ __global__ void zeroKernel(int32_t* p, int32_t count) {
 // queue: results do not depend on it.
 constexpr double kSeamRisk = 0.05;
 struct QueueKeyArgs {
+  const double* startParams;
+  int32_t byDirection;  // 1: direction bins instead of the sequence length
+  int32_t pad;
   const int64_t* seqOff;
   const int64_t* measOff;
   const int32_t* measSurf;
@@ -201,15 +206,30 @@ struct QueueKeyArgs {
   int32_t nSurfaces;
   int32_t maxLen;
 };
+// With byDirection the key is instead a bin of the start direction and
+// charge (16 eta bins over [-4, 4] x 32 phi bins x 2 charges, keys 0..1023,
+// seam tracks 1024): neighbouring lanes then follow neighbouring paths, so
+// that they reach their surfaces after similar numbers of steps and read
+// the same cells of a field map.
+constexpr int kDirectionKeys = 1024;
+__device__ inline int32_t directionKey(const QueueKeyArgs& k, int64_t t) {
+  const double* bp = k.startParams + 6 * t;
+  const double theta = fmin(fmax(bp[3], 1e-6), kPi - 1e-6);
+  const double eta = -log(tan(0.5 * theta));
+  const int eb = min(max(static_cast<int>(floor((eta + 4.0) * 2.0)), 0), 15);
+  const int pb = min(max(static_cast<int>(floor((bp[2] + kPi) * (16.0 / kPi))), 0), 31);
+  const int qb = bp[4] < 0 ? 0 : 1;
+  return (eb * 32 + pb) * 2 + qb;
+}
 __device__ inline int32_t queueKey(const QueueKeyArgs& k, int64_t t) {
-  const int32_t len = min(static_cast<int32_t>(k.seqOff[t + 1] - k.seqOff[t]), k.maxLen);
   for (int64_t m = k.measOff[t]; m < k.measOff[t + 1]; ++m) {
     const int32_t s = k.measSurf[m];
     if (s < 0 || s >= k.nSurfaces || k.geo[s].type != ACTS_KF_SURFACE_CYLINDER) continue;
     const double phi = k.measVal[2 * m] / k.geo[s].b0;
-    if (fabs(phi) > kPi - kSeamRisk) return k.maxLen + 1;
+    if (fabs(phi) > kPi - kSeamRisk) return k.byDirection ? kDirectionKeys : k.maxLen + 1;
   }
-  return len;
+  if (k.byDirection) return directionKey(k, t);
+  return min(static_cast<int32_t>(k.seqOff[t + 1] - k.seqOff[t]), k.maxLen);
 }
 // Counting sort of the tracks by queue key, largest key first, in three
 // passes without global atomics: per-block histograms (LDS atomics) stored
@@ -360,6 +380,10 @@ struct acts_kf_ctx {
   bool timing = false;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   bool timed = false;
+  // recorded after the last work of every call on whichever stream it used:
+  // acts_kf_sync waits for it and for nothing else on the device
+  hipEvent_t done = nullptr;
+  bool pending = false;
   // captured launch sequence of the last fit, replayed while its key holds
   bool useGraph = false;
   hipGraphExec_t graphExec = nullptr;
@@ -367,7 +391,9 @@ struct acts_kf_ctx {
   DevOpts opts{};
   PhysConst pc{};
   DevBuf scratchD, scratchI, scratchT, lenBuf, permBuf, histBuf, queueBuf;
-  bool lengthSorted = true;
+  // track queue order: 0 input order, 1 longest sequence first, 2 by start
+  // direction (kf: directionKey)
+  int32_t queueOrder = 1;
   // persistent forward kernel: resident wavefronts and scheduling knobs
   int64_t residentWaves = 0;
   int32_t innerSteps = 64;
@@ -553,6 +579,8 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   };
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(ACTS_KF_ERR_DEVICE, "hipStreamCreate failed");
+  if (hipEventCreateWithFlags(&c->done, hipEventDisableTiming) != hipSuccess)
+    return cleanup(ACTS_KF_ERR_DEVICE, "hipEventCreate failed");
   if (hipMalloc(&c->geo, sizeof(DevSurface) * nSurfaces) != hipSuccess ||
       hipMalloc(&c->slabs, sizeof(DevSlab) * hm.size()) != hipSuccess)
     return cleanup(ACTS_KF_ERR_OUT_OF_MEMORY, "hipMalloc geometry failed");
@@ -640,7 +668,9 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
     e.maxTargetSkipping = d.max_target_skipping;
     e.loopProtection = d.loop_protection;
   }
-  c->lengthSorted = options->length_sorted != 0;
+  if (options->length_sorted < 0 || options->length_sorted > 2)
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: bad length_sorted (queue order)");
+  c->queueOrder = options->length_sorted;
   c->timing = options->kernel_timing != 0;
   c->useGraph = options->use_graph != 0;
   if (c->timing)
@@ -675,6 +705,22 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   return ACTS_KF_OK;
 }
 
+// Records the context's completion event on `s` after the work just queued.
+static int markQueued(acts_kf_ctx* c, hipStream_t s) {
+  KF_HIP(hipEventRecord(c->done, s));
+  c->pending = true;
+  return ACTS_KF_OK;
+}
+
+// Offsets of a host-side batch: both CSR offset arrays start at 0 and never
+// decrease (anything else would index outside the input and output arrays).
+static bool offsetsValid(const int64_t* off, int64_t n) {
+  if (off[0] != 0) return false;
+  for (int64_t t = 0; t < n; ++t)
+    if (off[t + 1] < off[t]) return false;
+  return true;
+}
+
 static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
                      const acts_kf_batch_out* out, hipStream_t stream,
                      int32_t maxLen) {
@@ -692,8 +738,9 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   if (c->scratchD.ensure(needD) || c->scratchI.ensure(needI) || c->scratchT.ensure(needT) ||
       c->queueBuf.ensure(sizeof(int32_t)))
     return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "scratch allocation failed");
-  const bool sorted = c->lengthSorted && n > kBlock;
-  const int32_t nKeys = std::min(states + 2, kMaxKeys);
+  const bool sorted = c->queueOrder != 0 && n > kBlock;
+  const bool byDirection = c->queueOrder == 2;
+  const int32_t nKeys = byDirection ? kDirectionKeys + 1 : std::min(states + 2, kMaxKeys);
   const int64_t sortBlocks = (n + kSortBlock - 1) / kSortBlock;
   if (sorted && (c->permBuf.ensure(sizeof(int32_t) * n) ||
                  c->histBuf.ensure(sizeof(int32_t) * nKeys * sortBlocks)))
@@ -734,6 +781,8 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   b.measSurf = in->meas_surfaces;
   b.measVal = in->meas_values;
   b.measCov = in->meas_cov;
+  b.measDims = in->meas_dims;
+  b.measIdx = in->meas_indices;
   DevOut d;
   d.status = out->status;
   d.nStates = out->n_states;
@@ -771,6 +820,9 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
     perm = static_cast<int32_t*>(c->permBuf.p);
     int32_t* hist = static_cast<int32_t*>(c->histBuf.p);
     QueueKeyArgs qk;
+    qk.startParams = in->start_params;
+    qk.byDirection = byDirection ? 1 : 0;
+    qk.pad = 0;
     qk.seqOff = in->seq_offsets;
     qk.measOff = in->meas_offsets;
     qk.measSurf = in->meas_surfaces;
@@ -814,19 +866,19 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
     if (c->sympy && c->mixed)
       hipLaunchKernelGGL((referenceKernel<true, true>), dim3(static_cast<unsigned>(blocks)),
                          dim3(kBlock), 0, stream, d, sc, c->geo, c->field, c->extOpts,
-                         c->refLine, c->refStrategy, n);
+                         c->refLine, c->refStrategy, n, in->seq_offsets);
     else if (c->sympy)
       hipLaunchKernelGGL((referenceKernel<false, true>), dim3(static_cast<unsigned>(blocks)),
                          dim3(kBlock), 0, stream, d, sc, c->geo, c->field, c->extOpts,
-                         c->refLine, c->refStrategy, n);
+                         c->refLine, c->refStrategy, n, in->seq_offsets);
     else if (c->mixed)
       hipLaunchKernelGGL(referenceKernel<true>, dim3(static_cast<unsigned>(blocks)), dim3(kBlock),
                          0, stream, d, sc, c->geo, c->field, c->extOpts, c->refLine,
-                         c->refStrategy, n);
+                         c->refStrategy, n, in->seq_offsets);
     else
       hipLaunchKernelGGL(referenceKernel<false>, dim3(static_cast<unsigned>(blocks)),
                          dim3(kBlock), 0, stream, d, sc, c->geo, c->field, c->extOpts,
-                         c->refLine, c->refStrategy, n);
+                         c->refLine, c->refStrategy, n, in->seq_offsets);
   }
   const hipError_t launchErr = hipGetLastError();
   if (c->useGraph) {
@@ -880,7 +932,9 @@ int acts_kf_fit_batch_device(acts_kf_ctx* c, const acts_kf_batch_in* in,
     KF_HIP(hipMemcpyAsync(&maxLen, c->lenBuf.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     KF_HIP(hipStreamSynchronize(s));
   }
-  return launchFit(c, in, out, s, maxLen);
+  const int r = launchFit(c, in, out, s, maxLen);
+  if (r) return r;
+  return in->n_tracks > 0 ? markQueued(c, s) : ACTS_KF_OK;
 }
 
 int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
@@ -893,10 +947,25 @@ int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
       !in->seq_surfaces || !in->meas_offsets || !in->meas_surfaces || !in->meas_values ||
       !in->meas_cov)
     return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "null input array");
-  KF_HIP(hipSetDevice(c->device));
-  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  if (!offsetsValid(in->seq_offsets, n) || !offsetsValid(in->meas_offsets, n))
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT,
+                "acts_kf_fit_batch: seq_offsets and meas_offsets must start at 0 and not decrease");
   const int64_t nSeq = in->seq_offsets[n];
   const int64_t nMeas = in->meas_offsets[n];
+  if (in->meas_dims || in->meas_indices) {
+    for (int64_t m = 0; m < nMeas; ++m) {
+      const int d = in->meas_dims ? in->meas_dims[m] : 2;
+      const int a = in->meas_indices ? in->meas_indices[2 * m] : 0;
+      const int b = in->meas_indices ? in->meas_indices[2 * m + 1] : 1;
+      if ((d != 1 && d != 2) || a > 5 || (d == 2 && (b > 5 || b == a)))
+        return fail(ACTS_KF_ERR_INVALID_ARGUMENT,
+                    "acts_kf_fit_batch: measurement " + std::to_string(m) +
+                        " has an unsupported subspace (dimension 1 or 2 on distinct "
+                        "bound parameters 0..5)");
+    }
+  }
+  KF_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
   int32_t maxLen = 0;
   for (int64_t t = 0; t < n; ++t)
     maxLen = std::max<int32_t>(maxLen, static_cast<int32_t>(in->seq_offsets[t + 1] -
@@ -911,15 +980,18 @@ int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
       {in->start_plane, sizeof(double) * 12 * n},  {in->seq_offsets, sizeof(int64_t) * (n + 1)},
       {in->seq_surfaces, sizeof(int32_t) * nSeq},  {in->meas_offsets, sizeof(int64_t) * (n + 1)},
       {in->meas_surfaces, sizeof(int32_t) * nMeas}, {in->meas_values, sizeof(double) * 2 * nMeas},
-      {in->meas_cov, sizeof(double) * 4 * nMeas}};
+      {in->meas_cov, sizeof(double) * 4 * nMeas},
+      {in->meas_dims, in->meas_dims ? sizeof(uint8_t) * nMeas : 0},
+      {in->meas_indices, in->meas_indices ? sizeof(uint8_t) * 2 * nMeas : 0}};
+  constexpr int nIn = sizeof(ins) / sizeof(ins[0]);
   auto al = [](size_t x) { return (x + 255) / 256 * 256; };
   size_t total = 0;
   for (const Sec& q : ins) total += al(q.bytes);
   if (c->inBuf.ensure(total)) return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "input staging alloc");
   char* base = static_cast<char*>(c->inBuf.p);
-  void* dptr[9];
+  void* dptr[nIn];
   size_t off = 0;
-  for (int i = 0; i < 9; ++i) {
+  for (int i = 0; i < nIn; ++i) {
     dptr[i] = base + off;
     if (ins[i].bytes)
       KF_HIP(hipMemcpyAsync(dptr[i], ins[i].src, ins[i].bytes, hipMemcpyHostToDevice, s));
@@ -935,6 +1007,8 @@ int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
   din.meas_surfaces = static_cast<const int32_t*>(dptr[6]);
   din.meas_values = static_cast<const double*>(dptr[7]);
   din.meas_cov = static_cast<const double*>(dptr[8]);
+  din.meas_dims = in->meas_dims ? static_cast<const uint8_t*>(dptr[9]) : nullptr;
+  din.meas_indices = in->meas_indices ? static_cast<const uint8_t*>(dptr[10]) : nullptr;
   din.max_sequence_length = maxLen;
 
   // output staging: the same arrays on the device, copied back afterwards
@@ -998,7 +1072,7 @@ int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
   for (int i = 0; i < nOut; ++i)
     if (outs[i].host && outs[i].bytes)
       KF_HIP(hipMemcpyAsync(outs[i].host, optr[i], outs[i].bytes, hipMemcpyDeviceToHost, s));
-  return ACTS_KF_OK;
+  return markQueued(c, s);
 }
 
 int acts_kf_track_states_device(acts_kf_ctx* c, const acts_kf_batch_in* in,
@@ -1051,7 +1125,7 @@ int acts_kf_track_states_device(acts_kf_ctx* c, const acts_kf_batch_in* in,
   hipLaunchKernelGGL(trackStatesKernel, dim3(static_cast<unsigned>(blocks)), dim3(kTsBlock), 0, s,
                      a);
   KF_HIP(hipGetLastError());
-  return ACTS_KF_OK;
+  return markQueued(c, s);
 }
 
 int acts_kf_kernel_times(acts_kf_ctx* c, double* forwardMs, double* smootherMs) {
@@ -1070,8 +1144,10 @@ int acts_kf_kernel_times(acts_kf_ctx* c, double* forwardMs, double* smootherMs) 
 
 int acts_kf_sync(acts_kf_ctx* c) {
   if (!c) return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "null context");
+  if (!c->pending) return ACTS_KF_OK;
   KF_HIP(hipSetDevice(c->device));
-  KF_HIP(hipDeviceSynchronize());
+  KF_HIP(hipEventSynchronize(c->done));
+  c->pending = false;
   return ACTS_KF_OK;
 }
 
@@ -1097,6 +1173,8 @@ void acts_kf_destroy(acts_kf_ctx* c) {
   if (c->graphExec) (void)hipGraphExecDestroy(c->graphExec);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  if (c->pending) (void)hipEventSynchronize(c->done);
+  if (c->done) (void)hipEventDestroy(c->done);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }

@@ -33,6 +33,8 @@ struct DevBatch {
   const int32_t* measSurf;
   const double* measVal;
   const double* measCov;
+  const uint8_t* measDims;  // per measurement (null: all 2D)
+  const uint8_t* measIdx;   // 2 per measurement (null: loc0, loc1)
 };
 
 // Scratch of the forward pass (states of every track, read back by the

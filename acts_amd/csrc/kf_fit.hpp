@@ -329,6 +329,230 @@ __device__ inline double predictedChi2(const double x[6], const double P[21],
   return row0 * r0 + row1 * r1;
 }
 
+// A measurement of dimension n (1 or 2) on the bound parameters a (and b)
+// with values v and the n x n row-major covariance V.
+struct MeasDev {
+  int n, a, b;
+  double v[2];
+  double V[4];
+};
+
+// The measurement m of a batch (2D on loc0, loc1 without subspace arrays);
+// false if its subspace is not one this fitter handles.
+__device__ inline bool loadMeas(const DevBatch& in, int64_t m, MeasDev& M) {
+  M.n = in.measDims ? in.measDims[m] : 2;
+  M.a = in.measIdx ? in.measIdx[2 * m] : 0;
+  M.b = in.measIdx ? in.measIdx[2 * m + 1] : 1;
+  M.v[0] = in.measVal[2 * m];
+  M.v[1] = in.measVal[2 * m + 1];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) M.V[i] = in.measCov[4 * m + i];
+  if (M.n == 1) M.b = 0;
+  return (M.n == 1 || M.n == 2) && M.a >= 0 && M.a < 6 &&
+         (M.n == 1 || (M.b >= 0 && M.b < 6 && M.b != M.a));
+}
+
+// The (0, 1) subspace of a 2D measurement: the fast specialised update
+__device__ inline bool isLoc01(const MeasDev& M) { return M.n == 2 && M.a == 0 && M.b == 1; }
+
+// Row i of a packed symmetric matrix
+__device__ inline void symRow(const double P[21], int i, double r[6]) {
+#pragma unroll
+  for (int k = 0; k < 6; ++k) r[k] = P[sym(i, k)];
+}
+
+// Entry (i, k) of I - K H for a measurement on the parameters a (, b):
+// 1 on the diagonal, minus the gain column of the measured parameter
+__device__ inline double ikh(int i, int k, const MeasDev& M, double K0, double K1) {
+  const bool isA = k == M.a, isB = M.n == 2 && k == M.b;
+  return (i == k ? 1.0 : 0.0) - (isA ? K0 : (isB ? K1 : 0.0));
+}
+__device__ inline bool ikhNonZero(int i, int k, const MeasDev& M) {
+  return k == i || k == M.a || (M.n == 2 && k == M.b);
+}
+
+// GainMatrixUpdaterImpl.hpp:21-91 for a measurement of dimension N on any
+// subspace.  The products with the projector H are the reference's dense
+// products with their zero terms left out: each sum adds its non-zero
+// terms in the order of the summation index, which gives the dense sum's
+// value exactly (adding a zero product leaves a sum unchanged).
+template <int N>
+__device__ inline int gainUpdateSub(const double x[6], const double P[21], const MeasDev& M,
+                                    bool joseph, double f[6], double C[21], double& chi2) {
+  // P H^T (column j = column a or b of P), S = H P H^T + V
+  double Pa[6], Pb[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double va = 0.0, vb = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      va = (k == M.a) ? P[sym(i, k)] : va;
+      vb = (k == M.b) ? P[sym(i, k)] : vb;
+    }
+    Pa[i] = va;
+    Pb[i] = vb;
+  }
+  double xa = 0.0, xb = 0.0, Saa = 0.0, Sab = 0.0, Sba = 0.0, Sbb = 0.0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    xa = (k == M.a) ? x[k] : xa;
+    xb = (k == M.b) ? x[k] : xb;
+    Saa = (k == M.a) ? Pa[k] : Saa;
+    Sab = (k == M.a) ? Pb[k] : Sab;
+    Sba = (k == M.b) ? Pa[k] : Sba;
+    Sbb = (k == M.b) ? Pb[k] : Sbb;
+  }
+  double Si00, Si01 = 0.0, Si10 = 0.0, Si11 = 0.0;
+  if constexpr (N == 1) {
+    Si00 = 1.0 / (Saa + M.V[0]);
+  } else {
+    double o[4];
+    inverse2(Saa + M.V[0], Sab + M.V[1], Sba + M.V[2], Sbb + M.V[3], o);
+    Si00 = o[0];
+    Si01 = o[1];
+    Si10 = o[2];
+    Si11 = o[3];
+  }
+  // K = (P H^T) S^-1
+  double K0[6], K1[6];
+  bool nan = false;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    if constexpr (N == 1) {
+      K0[i] = Pa[i] * Si00;
+      K1[i] = 0.0;
+    } else {
+      K0[i] = Pa[i] * Si00 + Pb[i] * Si10;
+      K1[i] = Pa[i] * Si01 + Pb[i] * Si11;
+    }
+    nan = nan || isnan(K0[i]) || isnan(K1[i]);
+  }
+  if (nan) return ACTS_KF_STATUS_UPDATE_FAILED;
+  // filtered = x + K (m - H x), normalised
+  const double r0 = M.v[0] - xa, r1 = M.v[1] - xb;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) f[i] = x[i] + (N == 1 ? K0[i] * r0 : K0[i] * r0 + K1[i] * r1);
+  normalizePhiTheta(f[2], f[3]);
+  // C = (I - K H) P, or the Joseph form (I - K H) P (I - K H)^T + K V K^T
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double TP[6];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      double acc = 0.0;
+      bool started = false;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) {
+        if (ikhNonZero(i, k, M)) {
+          const double term = ikh(i, k, M, K0[i], K1[i]) * P[sym(k, j)];
+          acc = started ? acc + term : term;
+          started = true;
+        }
+      }
+      TP[j] = acc;
+    }
+#pragma unroll
+    for (int j = i; j < 6; ++j) {
+      if (!joseph) {
+        C[sym(i, j)] = TP[j];
+      } else {
+        double acc = 0.0;
+        bool started = false;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+          if (ikhNonZero(j, k, M)) {
+            const double term = TP[k] * ikh(j, k, M, K0[j], K1[j]);
+            acc = started ? acc + term : term;
+            started = true;
+          }
+        }
+        // (K V) K^T
+        double b;
+        if constexpr (N == 1) {
+          b = (K0[i] * M.V[0]) * K0[j];
+        } else {
+          const double kv0 = K0[i] * M.V[0] + K1[i] * M.V[2];
+          const double kv1 = K0[i] * M.V[1] + K1[i] * M.V[3];
+          b = kv0 * K0[j] + kv1 * K1[j];
+        }
+        C[sym(i, j)] = acc + b;
+      }
+    }
+  }
+  // chi2 = res^T ((I - H K) V)^-1 res on the filtered residual
+  double fa = 0.0, fb = 0.0, Ka0 = 0.0, Ka1 = 0.0, Kb0 = 0.0, Kb1 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    fa = (k == M.a) ? f[k] : fa;
+    fb = (k == M.b) ? f[k] : fb;
+    Ka0 = (k == M.a) ? K0[k] : Ka0;
+    Ka1 = (k == M.a) ? K1[k] : Ka1;
+    Kb0 = (k == M.b) ? K0[k] : Kb0;
+    Kb1 = (k == M.b) ? K1[k] : Kb1;
+  }
+  const double e0 = M.v[0] - fa, e1 = M.v[1] - fb;
+  if constexpr (N == 1) {
+    const double m00 = (1.0 - Ka0) * M.V[0];
+    const double mi = 1.0 / m00;
+    chi2 = (e0 * mi) * e0;
+  } else {
+    const double i00 = 1.0 - Ka0, i01 = 0.0 - Ka1, i10 = 0.0 - Kb0, i11 = 1.0 - Kb1;
+    double mi[4];
+    inverse2(i00 * M.V[0] + i01 * M.V[2], i00 * M.V[1] + i01 * M.V[3],
+             i10 * M.V[0] + i11 * M.V[2], i10 * M.V[1] + i11 * M.V[3], mi);
+    const double row0 = e0 * mi[0] + e1 * mi[2];
+    const double row1 = e0 * mi[1] + e1 * mi[3];
+    chi2 = row0 * e0 + row1 * e1;
+  }
+  return 0;
+}
+
+// GainMatrixUpdater::visitMeasurement (GainMatrixUpdater.cpp:19-29): the
+// specialised 2D (loc0, loc1) update, or the dense one of the measurement's
+// dimension.
+__device__ inline int gainUpdateMeas(const double x[6], const double P[21], const MeasDev& M,
+                                     bool joseph, double f[6], double C[21], double& chi2) {
+  if (isLoc01(M)) return gainUpdate(x, P, M.v, M.V, joseph, f, C, chi2);
+  if (M.n == 1) return gainUpdateSub<1>(x, P, M, joseph, f, C, chi2);
+  return gainUpdateSub<2>(x, P, M, joseph, f, C, chi2);
+}
+
+// calculatePredictedChi2 (TrackHelpers.hpp:676-694) for any measurement
+__device__ inline double predictedChi2Meas(const double x[6], const double P[21],
+                                           const MeasDev& M) {
+  if (isLoc01(M)) return predictedChi2(x, P, M.v, M.V);
+  double ra[6], rb[6];
+  symRow(P, 0, ra);
+  double xa = x[0], xb = x[0], paa = ra[0], pab = ra[0], pba = ra[0], pbb = ra[0];
+#pragma unroll
+  for (int k = 1; k < 6; ++k) {
+    xa = (M.a == k) ? x[k] : xa;
+    xb = (M.b == k) ? x[k] : xb;
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    symRow(P, i, rb);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (M.a == i && M.a == k) paa = rb[k];
+      if (M.a == i && M.b == k) pab = rb[k];
+      if (M.b == i && M.a == k) pba = rb[k];
+      if (M.b == i && M.b == k) pbb = rb[k];
+    }
+  }
+  if (M.n == 1) {
+    const double r0 = M.v[0] - xa;
+    const double i00 = 1.0 / (paa + M.V[0]);
+    return (r0 * i00) * r0;
+  }
+  const double r0 = M.v[0] - xa, r1 = M.v[1] - xb;
+  double Si[4];
+  inverse2(paa + M.V[0], pab + M.V[1], pba + M.V[2], pbb + M.V[3], Si);
+  const double row0 = r0 * Si[0] + r1 * Si[2];
+  const double row1 = r0 * Si[1] + r1 * Si[3];
+  return row0 * r0 + row1 * r1;
+}
+
 // 6x6 inverse by partial-pivot LU with all indices static (row swaps done
 // with selects so that the matrix stays in registers).
 __device__ inline bool inverse6(const double* A, double* X) {

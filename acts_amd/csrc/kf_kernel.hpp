@@ -122,6 +122,13 @@ __device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a,
     const int sid = L.seq(i);
     badIndex = badIndex || sid < 0 || sid >= a.nSurfaces;
   }
+  // measurement subspaces the updater does not handle fail the track too
+  if (in.measDims || in.measIdx) {
+    for (int i = 0; i < L.nMeas; ++i) {
+      MeasDev M;
+      badIndex = badIndex || !loadMeas(in, L.m0 + i, M);
+    }
+  }
   if (badIndex) {
     L.status = ACTS_KF_STATUS_PROPAGATION_FAILURE;
     return kPhaseFinished;
@@ -333,15 +340,13 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
         bool measured = false;
         if (mi >= 0) {
           flags |= ACTS_KF_STATE_HAS_MEASUREMENT;
-          const double* mv = in.measVal + 2 * (L.m0 + mi);
-          const double* mc = in.measCov + 4 * (L.m0 + mi);
-          const double mvv[2] = {mv[0], mv[1]};
-          const double mcc[4] = {mc[0], mc[1], mc[2], mc[3]};
+          MeasDev M;
+          loadMeas(in, L.m0 + mi, M);  // its subspace was checked by laneInit
           bool outlier = false;
-          if (o.chi2Cut > 0) outlier = predictedChi2(x, T.cov, mvv, mcc) > o.chi2Cut;
+          if (o.chi2Cut > 0) outlier = predictedChi2Meas(x, T.cov, M) > o.chi2Cut;
           if (!outlier) {
             double f[6], C[21], chi2 = 0;
-            const int us = gainUpdate(x, T.cov, mvv, mcc, o.useJoseph != 0, f, C, chi2);
+            const int us = gainUpdateMeas(x, T.cov, M, o.useJoseph != 0, f, C, chi2);
             if (us) {
               L.status = us;
             } else {
@@ -412,6 +417,30 @@ __device__ inline void laneFinish(Lane& L, const FwdArgs& a) {
   a.sc.track[a.sc.stride + t] = L.tip;
   a.sc.track[2 * a.sc.stride + t] = L.firstMeas;
   a.sc.track[3 * a.sc.stride + t] = L.lastMeas;
+}
+
+// An output slot that holds no state: surface and measurement -1, flags 0,
+// every number 0.
+__device__ inline void clearSlot(const DevOut& out, int64_t slot) {
+  if (out.smoothed)
+    for (int i = 0; i < 6; ++i) out.smoothed[6 * slot + i] = 0.0;
+  if (out.smoothedCov)
+    for (int i = 0; i < 36; ++i) out.smoothedCov[36 * slot + i] = 0.0;
+  if (out.filtered)
+    for (int i = 0; i < 6; ++i) out.filtered[6 * slot + i] = 0.0;
+  if (out.filteredCov)
+    for (int i = 0; i < 36; ++i) out.filteredCov[36 * slot + i] = 0.0;
+  if (out.predicted)
+    for (int i = 0; i < 6; ++i) out.predicted[6 * slot + i] = 0.0;
+  if (out.predictedCov)
+    for (int i = 0; i < 36; ++i) out.predictedCov[36 * slot + i] = 0.0;
+  if (out.jacobian)
+    for (int i = 0; i < 36; ++i) out.jacobian[36 * slot + i] = 0.0;
+  if (out.stateSurface) out.stateSurface[slot] = -1;
+  if (out.stateFlags) out.stateFlags[slot] = 0u;
+  if (out.statePath) out.statePath[slot] = 0.0;
+  if (out.stateChi2) out.stateChi2[slot] = 0.f;
+  if (out.stateMeas) out.stateMeas[slot] = -1;
 }
 
 // Expand a packed symmetric matrix into a row-major 6x6 output.
@@ -493,7 +522,101 @@ __device__ inline void mbfUpdate(int64_t t, int k, uint32_t flags, const DevBatc
   for (int i = 0; i < 36; ++i) F[i] = sd(sc, k, kJac + i, t);
   const bool isMeasurement =
       (flags & ACTS_KF_STATE_HAS_MEASUREMENT) && !(flags & ACTS_KF_STATE_IS_OUTLIER);
-  if (isMeasurement) {
+  MeasDev M;
+  if (isMeasurement) loadMeas(in, in.measOff[t] + si(sc, k, kIMeas, t), M);
+  if (isMeasurement && !isLoc01(M)) {
+    // a measurement of another subspace: the same recursion with the
+    // projector's indices a (, b) in place of loc0 (, loc1)
+    double P[21], xp[6];
+#pragma unroll
+    for (int i = 0; i < 21; ++i) P[i] = sd(sc, k, kPCov + i, t);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) xp[i] = sd(sc, k, kPred + i, t);
+    const bool two = M.n == 2;
+    double Paa = 0, Pab = 0, Pba = 0, Pbb = 0, xa = 0, xb = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      xa = (M.a == i) ? xp[i] : xa;
+      xb = (M.b == i) ? xp[i] : xb;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const double pij = P[sym(i, j)];
+        Paa = (M.a == i && M.a == j) ? pij : Paa;
+        Pab = (M.a == i && M.b == j) ? pij : Pab;
+        Pba = (M.b == i && M.a == j) ? pij : Pba;
+        Pbb = (M.b == i && M.b == j) ? pij : Pbb;
+      }
+    }
+    double Si[2][2] = {{0, 0}, {0, 0}};
+    if (!two) {
+      Si[0][0] = 1.0 / (Paa + M.V[0]);
+    } else {
+      double o[4];
+      inverse2(Paa + M.V[0], Pab + M.V[1], Pba + M.V[2], Pbb + M.V[3], o);
+      Si[0][0] = o[0];
+      Si[0][1] = o[1];
+      Si[1][0] = o[2];
+      Si[1][1] = o[3];
+    }
+    // K = (P H^T) S^-1, CHat = I - K H
+    double CH[36];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double pia = 0, pib = 0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        pia = (M.a == j) ? P[sym(i, j)] : pia;
+        pib = (M.b == j) ? P[sym(i, j)] : pib;
+      }
+      double k0 = pia * Si[0][0], k1 = pia * Si[0][1];
+      if (two) {
+        k0 = k0 + pib * Si[1][0];
+        k1 = k1 + pib * Si[1][1];
+      }
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const double id = (i == j) ? 1.0 : 0.0;
+        CH[6 * i + j] = (M.a == j) ? id - k0 : ((two && M.b == j) ? id - k1 : id);
+      }
+    }
+    const double y0 = M.v[0] - xa, y1 = two ? M.v[1] - xb : 0.0;
+    // Lambda~ = (H^T S^-1) H + (CHat^T Lambda) CHat
+    mbfTransport(CH, L);
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int j = 0; j < 6; ++j) {
+        const int r = (M.a == i) ? 0 : ((two && M.b == i) ? 1 : -1);
+        const int c = (M.a == j) ? 0 : ((two && M.b == j) ? 1 : -1);
+        if (r >= 0 && c >= 0) {
+          const double sij = (r == 0) ? (c == 0 ? Si[0][0] : Si[0][1])
+                                      : (c == 0 ? Si[1][0] : Si[1][1]);
+          L[6 * i + j] = sij + L[6 * i + j];
+        }
+      }
+    // lambda~ = (-H^T S^-1) y + CHat^T lambda
+    double lt[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double acc = CH[i] * l[0];
+#pragma unroll
+      for (int m = 1; m < 6; ++m) acc = acc + CH[6 * m + i] * l[m];
+      lt[i] = acc;
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      const int r = (M.a == i) ? 0 : ((two && M.b == i) ? 1 : -1);
+      if (r >= 0) {
+        const double s0 = (r == 0) ? Si[0][0] : Si[1][0];
+        const double s1 = (r == 0) ? Si[0][1] : Si[1][1];
+        double acc = -s0 * y0;
+        if (two) acc = acc + -s1 * y1;
+        lt[i] = acc + lt[i];
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) l[i] = lt[i];
+  } else if (isMeasurement) {
     double P[21];
 #pragma unroll
     for (int i = 0; i < 21; ++i) P[i] = sd(sc, k, kPCov + i, t);
@@ -631,7 +754,7 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
     } else if (flags & ACTS_KF_STATE_HAS_MEASUREMENT) {
       ++nMeasT;
       chi2sum += c2;
-      ndf += 2;
+      ndf += in.measDims ? in.measDims[in.measOff[t] + si(sc, k, kIMeas, t)] : 2;
     }
     const int64_t slot = base + k;
     if (out.smoothed)
@@ -673,6 +796,10 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
     }
   }
   if (!valid) return;
+  // slots past the track's states (all of them for a failed track) get the
+  // values of an empty slot, so that every output slot is defined
+  const int seqLen = static_cast<int>(in.seqOff[t + 1] - base);
+  for (int k = status ? 0 : tip + 1; k < seqLen; ++k) clearSlot(out, base + k);
   // the status after smoothing, for the extrapolation kernel
   sc.track[t] = status;
   if (out.status) out.status[t] = status;

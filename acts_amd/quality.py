@@ -79,3 +79,61 @@ def relative_error(got: dict, ref: dict, max_chi2_ndf: float = 100.0) -> dict:
             "p999": float(np.quantile(ec, 0.999)) if len(ec) else None,
             "max": float(ec.max()) if len(ec) else None,
             "max_incl_not_converged": float(ea.max()) if len(ea) else None}
+
+
+def parity_counts(got, ref, seq_offsets, rel: float = 1e-10, rel_cov: float = 5e-8,
+                  chunk: int = 1 << 20) -> dict:
+    """Agreement of two fits of one batch (objects with the per-track
+    summaries and the per-state arrays state_surface, state_flags, smoothed
+    and optionally smoothed_cov), e.g. the GPU fit against the CPU fit:
+    tracks with another status or state count, slots of the compared tracks
+    with another (surface, flags), and the largest relative errors of the
+    smoothed parameters (|a - b| <= rel * max(|b|, floor)) and covariances
+    (|a_ij - b_ij| <= rel_cov * sqrt(b_ii b_jj)) with the number of tracks
+    beyond each bar.  Vectorised over slots in chunks, for whole batches."""
+    n = len(ref.status)
+    same = (got.status == ref.status) & (got.n_states == ref.n_states)
+    good = same & (ref.status == 0)
+    so = np.asarray(seq_offsets)
+    lens = np.where(good, ref.n_states, 0).astype(np.int64)
+    owner = np.repeat(np.arange(n), lens)
+    starts = np.repeat(so[:-1], lens)
+    within = np.arange(len(owner)) - np.repeat(np.cumsum(lens) - lens, lens)
+    idx = starts + within
+    out = {"n_tracks": int(n), "n_compared_tracks": int(np.count_nonzero(good)),
+           "n_compared_states": int(len(idx)),
+           "status_or_state_count_mismatch": int(np.count_nonzero(~same)),
+           "rel_bar": rel, "rel_cov_bar": rel_cov}
+    seq_bad = np.zeros(n, bool)
+    p_bad = np.zeros(n, bool)
+    c_bad = np.zeros(n, bool)
+    p_max = 0.0
+    c_max = 0.0
+    has_cov = getattr(got, "smoothed_cov", None) is not None and \
+        getattr(ref, "smoothed_cov", None) is not None
+    for a in range(0, len(idx), chunk):
+        ii, oo = idx[a:a + chunk], owner[a:a + chunk]
+        bad = (got.state_surface[ii] != ref.state_surface[ii]) | \
+              (got.state_flags[ii] != ref.state_flags[ii])
+        seq_bad[oo[bad]] = True
+        pe = (np.abs(got.smoothed[ii] - ref.smoothed[ii]) /
+              np.maximum(np.abs(ref.smoothed[ii]), PARAM_FLOORS)).max(axis=1)
+        p_max = max(p_max, float(pe.max()) if len(pe) else 0.0)
+        p_bad[oo[pe > rel]] = True
+        if has_cov:
+            g = got.smoothed_cov[ii].reshape(-1, 6, 6)
+            r = ref.smoothed_cov[ii].reshape(-1, 6, 6)
+            d = np.sqrt(np.abs(np.einsum("nii->ni", r)))
+            sc = d[:, :, None] * d[:, None, :]
+            ce = (np.abs(g - r) / np.where(sc > 0, sc, 1.0)).reshape(len(ii), -1).max(axis=1)
+            c_max = max(c_max, float(ce.max()) if len(ce) else 0.0)
+            c_bad[oo[ce > rel_cov]] = True
+    out["state_sequence_mismatch_tracks"] = int(np.count_nonzero(seq_bad))
+    out["smoothed_max_rel"] = p_max
+    out["tracks_beyond_rel_bar"] = int(np.count_nonzero(p_bad))
+    if has_cov:
+        out["smoothed_cov_max_rel"] = c_max
+        out["tracks_beyond_rel_cov_bar"] = int(np.count_nonzero(c_bad))
+    chi2_bad = good & (np.abs(got.chi2 - ref.chi2) > 1e-5 * np.maximum(np.abs(ref.chi2), 1.0))
+    out["chi2_mismatch_tracks"] = int(np.count_nonzero(chi2_bad))
+    return out

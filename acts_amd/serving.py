@@ -13,6 +13,12 @@ stream), leaving the results (surface, flags and smoothed parameters of
 every state, track summaries) in pinned memory, and stamps every event
 of the batch with its completion time.  Latency of an event = completion -
 arrival.
+
+Admission: a batch is admitted when a slot is free and its device memory
+(inputs, per-state scratch, outputs: `batch_bytes`) fits, together with the
+batches already in flight, under the HBM budget; as batches retire their
+bytes are returned and waiting events are admitted.  An event larger than
+the whole budget runs alone.
 """
 from __future__ import annotations
 
@@ -42,10 +48,11 @@ class ServingConfig:
     sigma: float = 0.7               # ... and width of log(multiplicity)
     min_tracks: int = 500
     max_tracks: int = 20000
-    slots: int = 3                   # batches in flight
-    hbm_budget_bytes: float = 4e9    # device memory one batch may use
+    slots: int = 3                   # fitter contexts: most batches in flight
+    hbm_budget_bytes: float = 16e9   # device memory of all batches in flight
     max_batch_tracks: int = 60000    # latency bound: tracks per batch
     seed: int = 1
+    keep_results: bool = False       # keep every event's results (tests)
 
 
 def event_stream(cfg: ServingConfig, pool_tracks: int):
@@ -128,6 +135,17 @@ class _Slot:
         self.out["state_flags"] = pinned(cap_seq, 1, np.int32)
         self.out["smoothed"] = pinned(cap_seq, 6, np.float64)
 
+    def results(self, t0: int, cnt: int, s0: int, ns: int) -> dict:
+        """Copies of the results of the batch's tracks t0..t0+cnt-1, whose
+        state slots are s0..s0+ns-1."""
+        out = {name: self.out[name][t0:t0 + cnt].copy()
+               for name in ("status", "n_states", "chi2", "ndf", "n_measurements", "n_holes",
+                            "n_outliers")}
+        for name in SERVING_OUTPUTS:
+            out[name] = self.out[name][s0:s0 + ns].copy()
+        out["state_flags"] = out["state_flags"].view(np.uint32)
+        return out
+
     def fit(self, pool: Events, ranges) -> int:
         """Copy the tracks pool[off:off+m] of `ranges` into the pinned inputs,
         fit them, and leave the results in the pinned outputs."""
@@ -168,7 +186,9 @@ class _Slot:
 
 def serve(detector, field, pool: Events, cfg: ServingConfig,
           options: KalmanFitterOptions | None = None, device: int = 0) -> dict:
-    """Run the serving loop; returns throughput and latency figures."""
+    """Run the serving loop; returns throughput and latency figures (and,
+    with cfg.keep_results, each event's results under "results" and its
+    tracks as pool ranges under "event_ranges")."""
     options = options or KalmanFitterOptions()
     arrivals, mult, offs = event_stream(cfg, pool.n_tracks)
     n_ev = len(arrivals)
@@ -188,19 +208,35 @@ def serve(detector, field, pool: Events, cfg: ServingConfig,
     for i in range(cfg.slots):
         free.put(i)
     work = [queue.Queue() for _ in range(cfg.slots)]
-    stats = {"batches": 0, "batch_tracks": []}
+    stats = {"batches": 0, "batch_tracks": [], "busy": 0, "max_busy": 0,
+             "inflight": 0.0, "max_inflight": 0.0}
+    results = {}
+    cond = threading.Condition()
     t0 = [0.0]
+    so, mo = pool.seq_offsets, pool.meas_offsets
 
     def worker(i):
         sl = slots[i]
         while True:
-            ids = work[i].get()
-            if ids is None:
+            item = work[i].get()
+            if item is None:
                 return
+            ids, nbytes = item
             n = sl.fit(pool, [(int(offs[e]), int(mult[e])) for e in ids])
             done_at[ids] = time.perf_counter() - t0[0]
-            stats["batches"] += 1
-            stats["batch_tracks"].append(n)
+            if cfg.keep_results:
+                t = s = 0
+                for e in ids:
+                    lo, cnt = int(offs[e]), int(mult[e])
+                    ns = int(so[lo + cnt] - so[lo])
+                    results[int(e)] = sl.results(t, cnt, s, ns)
+                    t, s = t + cnt, s + ns
+            with cond:
+                stats["batches"] += 1
+                stats["batch_tracks"].append(n)
+                stats["busy"] -= 1
+                stats["inflight"] -= nbytes
+                cond.notify_all()
             free.put(i)
 
     threads = [threading.Thread(target=worker, args=(i,), daemon=True)
@@ -215,14 +251,31 @@ def serve(detector, field, pool: Events, cfg: ServingConfig,
             time.sleep(min(arrivals[nxt] - now, 0.001))
             continue
         slot = free.get()  # blocks while every slot is busy
-        now = time.perf_counter() - t0[0]
-        ids, tracks = [], 0
-        while nxt < n_ev and arrivals[nxt] <= now and (
-                not ids or tracks + mult[nxt] <= cap_tracks):
-            ids.append(nxt)
-            tracks += int(mult[nxt])
-            nxt += 1
-        work[slot].put(np.array(ids))
+        with cond:
+            while True:
+                now = time.perf_counter() - t0[0]
+                ids, tracks, nbytes = [], 0, 0.0
+                while nxt + len(ids) < n_ev and arrivals[nxt + len(ids)] <= now:
+                    e = nxt + len(ids)
+                    b = batch_bytes(tracks + int(mult[e]), max_len)
+                    if ids and tracks + mult[e] > cap_tracks:
+                        break
+                    # admitted if it fits the budget left, or if nothing
+                    # else is in flight (an event above the whole budget)
+                    if stats["inflight"] + b > cfg.hbm_budget_bytes and (ids or stats["busy"]):
+                        break
+                    ids.append(e)
+                    tracks += int(mult[e])
+                    nbytes = b
+                if ids:
+                    break
+                cond.wait()  # memory frees up as batches retire
+            nxt += len(ids)
+            stats["busy"] += 1
+            stats["inflight"] += nbytes
+            stats["max_busy"] = max(stats["max_busy"], stats["busy"])
+            stats["max_inflight"] = max(stats["max_inflight"], stats["inflight"])
+        work[slot].put((np.array(ids), nbytes))
     for q in work:
         q.put(None)
     for th in threads:
@@ -231,12 +284,19 @@ def serve(detector, field, pool: Events, cfg: ServingConfig,
     for sl in slots:
         sl.fitter.close()
     lat = (done_at - arrivals) * 1e3
-    return {"events": int(n_ev), "tracks": int(mult.sum()),
-            "throughput_tracks_per_s": float(mult.sum() / total),
-            "offered_tracks_per_s": float(mult.sum() / cfg.duration_s),
-            "latency_ms_p50": float(np.percentile(lat, 50)),
-            "latency_ms_p99": float(np.percentile(lat, 99)),
-            "latency_ms_max": float(np.max(lat)),
-            "batches": stats["batches"],
-            "mean_batch_tracks": float(np.mean(stats["batch_tracks"])),
-            "wall_s": float(total)}
+    rep = {"events": int(n_ev), "tracks": int(mult.sum()),
+           "throughput_tracks_per_s": float(mult.sum() / total),
+           "offered_tracks_per_s": float(mult.sum() / cfg.duration_s),
+           "latency_ms_p50": float(np.percentile(lat, 50)),
+           "latency_ms_p99": float(np.percentile(lat, 99)),
+           "latency_ms_max": float(np.max(lat)),
+           "batches": stats["batches"],
+           "mean_batch_tracks": float(np.mean(stats["batch_tracks"])),
+           "max_batches_in_flight": stats["max_busy"],
+           "max_inflight_bytes": stats["max_inflight"],
+           "hbm_budget_bytes": cfg.hbm_budget_bytes,
+           "wall_s": float(total)}
+    if cfg.keep_results:
+        rep["results"] = results
+        rep["event_ranges"] = [(int(o), int(m)) for o, m in zip(offs, mult)]
+    return rep

@@ -197,8 +197,12 @@ typedef struct acts_kf_options {
   int32_t reference_strategy;
   double reference_position[3];
   /* scheduling (no reference counterpart; results do not depend on it):
-   * length_sorted     1 = the smoother visits tracks in order of decreasing
-   *                   sequence length (default), 0 = input order;
+   * length_sorted     order in which the kernels take the tracks: 1 =
+   *                   decreasing sequence length (default), 0 = input
+   *                   order, 2 = by start direction and charge (bins of
+   *                   eta, phi and the sign of q/p, so that neighbouring
+   *                   lanes follow neighbouring paths); tracks with a
+   *                   measurement near the +-pi seam of a cylinder first;
    * sched_inner_steps most RK steps the forward kernel takes before the
    *                   lanes waiting at a surface are updated (0 = 64);
    * sched_wait_64ths  the surface update runs early once this many 64ths of
@@ -249,13 +253,29 @@ typedef struct acts_kf_batch_in {
    * 0 = let the library compute it (one extra device->host copy)            */
   int32_t max_sequence_length;
   int32_t reserved;
+  /* Measurement subspaces, optional (NULL: every measurement is 2D on
+   * (loc0, loc1)).  meas_dims[m] = N in {1, 2}; meas_indices[m][0..N-1] =
+   * the bound parameters measured (0..5, distinct); the values and the N x N
+   * row-major covariance fill the first N and N*N entries of meas_values[m]
+   * and meas_cov[m] (a strip measurement of loc1: dims 1, indices {1, -},
+   * meas_values[m][0] = loc1, meas_cov[m][0] = its variance).  The
+   * reference's GainMatrixUpdater dispatches on N = 1..6 with the
+   * projector of the subspace indices (Core/src/TrackFitting/
+   * GainMatrixUpdater.cpp:19-29, detail/GainMatrixUpdaterImpl.hpp:21-91);
+   * N above 2 is not supported here.  acts_kf_fit_batch rejects other
+   * values with ACTS_KF_ERR_INVALID_ARGUMENT; on the device path a track
+   * with such a measurement fails with ACTS_KF_STATUS_PROPAGATION_FAILURE. */
+  const uint8_t* meas_dims;     /* [m] or NULL */
+  const uint8_t* meas_indices;  /* [m][2] or NULL */
 } acts_kf_batch_in;
 
 /* Output buffers.  Per-state arrays use the sequence layout: the states of
  * track i occupy slots seq_offsets[i] .. seq_offsets[i] + n_states[i] - 1 in
  * propagation order (first state first), up to and including the last
- * measurement state (the track tip, KalmanFitter.hpp:831).  Any per-state
- * pointer may be NULL to skip that output.                                  */
+ * measurement state (the track tip, KalmanFitter.hpp:831).  The remaining
+ * slots of the track (all of them when its status is not 0) are written as
+ * empty slots: state_surface and state_measurement -1, state_flags 0 and
+ * every number 0.  Any per-state pointer may be NULL to skip that output.  */
 typedef struct acts_kf_batch_out {
   int32_t* status;          /* [n] ACTS_KF_STATUS_* */
   int32_t* n_states;        /* [n] */
@@ -306,7 +326,8 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t n_surfaces,
 
 /* Fit a batch whose buffers (in and out) are host memory.  Asynchronous on
  * `stream` (a hipStream_t, NULL = the context's own stream); results are in
- * the output buffers after acts_kf_sync.  Replaces one call of
+ * the output buffers after acts_kf_sync.  seq_offsets and meas_offsets must
+ * start at 0 and never decrease (ACTS_KF_ERR_INVALID_ARGUMENT otherwise).  Replaces one call of
  * KalmanFitter::fit (KalmanFitter.hpp:721-731, the DirectNavigator overload
  * with the surface sequence) per track of the batch, as issued by
  * KalmanFitterFunctionImpl (Examples/Algorithms/TrackFitting/src/
@@ -316,11 +337,17 @@ int acts_kf_fit_batch(acts_kf_ctx* ctx, const acts_kf_batch_in* in,
 
 /* Fit a batch whose buffers are all device memory on the context's device
  * (inputs already resident in HBM).  Asynchronous on `stream`.  Same
- * reference call as acts_kf_fit_batch (KalmanFitter.hpp:721-731).          */
+ * reference call as acts_kf_fit_batch (KalmanFitter.hpp:721-731).
+ * Precondition (not checked: the offsets are device memory): seq_offsets
+ * and meas_offsets start at 0, never decrease, and their last entries are
+ * the lengths of seq_surfaces and of the measurement arrays, which the
+ * output slot arrays match.                                                 */
 int acts_kf_fit_batch_device(acts_kf_ctx* ctx, const acts_kf_batch_in* in,
                              const acts_kf_batch_out* out, void* stream);
 
-/* Wait for all work the context queued.  The reference's fit() returns
+/* Wait for the work of the context's last call (fit_batch, fit_batch_device
+ * or track_states_device), on whichever stream it was queued; work of other
+ * contexts on the device is not waited for.  The reference's fit() returns
  * with its result (KalmanFitter.hpp:721-731); a batched, asynchronous fit
  * makes that completion point explicit.                                    */
 int acts_kf_sync(acts_kf_ctx* ctx);

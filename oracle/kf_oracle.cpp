@@ -1220,39 +1220,51 @@ int performMaterialInteraction(const Surface& s, Stepper& st,
 // ===========================================================================
 // GainMatrixUpdaterImpl.hpp:21-91 (kMeasurementSize = 2, subspace {0, 1})
 // ===========================================================================
-int gainMatrixUpdate(const Vec6& x, const M6& P, const double meas[2],
-                     const double V[4], bool useJoseph, Vec6& filtered,
-                     M6& filteredCov, double& chi2) {
-  Mat<2, 6> H = Mat<2, 6>::zero();
-  H.a[0][0] = 1;
-  H.a[1][1] = 1;
-  Mat<2, 2> Vm;
-  Vm.a[0][0] = V[0];
-  Vm.a[0][1] = V[1];
-  Vm.a[1][0] = V[2];
-  Vm.a[1][1] = V[3];
-  Mat<6, 2> PHt = mulT(P, H);
-  Mat<2, 6> HP = mul(H, P);
-  Mat<2, 2> S = mulT(HP, H);
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) S.a[i][j] = S.a[i][j] + Vm.a[i][j];
-  // Eigen compute_inverse_size2
+// The projector H (N x 6) of a measurement's subspace
+// (FixedBoundSubspaceHelper::projector)
+template <int N>
+Mat<N, 6> projector(const Meas& m) {
+  Mat<N, 6> H = Mat<N, 6>::zero();
+  for (int i = 0; i < N; ++i) H.a[i][m.idx[i]] = 1;
+  return H;
+}
+
+// Eigen's closed-form inverses of a 1x1 and a 2x2 matrix
+inline Mat<1, 1> inverseSmall(const Mat<1, 1>& S) {
+  Mat<1, 1> r;
+  r.a[0][0] = 1.0 / S.a[0][0];
+  return r;
+}
+inline Mat<2, 2> inverseSmall(const Mat<2, 2>& S) {
   Mat<2, 2> Si;
-  {
-    const double det = S.a[0][0] * S.a[1][1] - S.a[1][0] * S.a[0][1];
-    const double invdet = 1.0 / det;
-    Si.a[0][0] = S.a[1][1] * invdet;
-    Si.a[1][0] = -S.a[1][0] * invdet;
-    Si.a[0][1] = -S.a[0][1] * invdet;
-    Si.a[1][1] = S.a[0][0] * invdet;
-  }
-  Mat<6, 2> K = mul(PHt, Si);
+  const double det = S.a[0][0] * S.a[1][1] - S.a[1][0] * S.a[0][1];
+  const double invdet = 1.0 / det;
+  Si.a[0][0] = S.a[1][1] * invdet;
+  Si.a[1][0] = -S.a[1][0] * invdet;
+  Si.a[0][1] = -S.a[0][1] * invdet;
+  Si.a[1][1] = S.a[0][0] * invdet;
+  return Si;
+}
+
+template <int N>
+int gainMatrixUpdateN(const Vec6& x, const M6& P, const Meas& ms, bool useJoseph,
+                      Vec6& filtered, M6& filteredCov, double& chi2) {
+  const Mat<N, 6> H = projector<N>(ms);
+  Mat<N, N> Vm;
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) Vm.a[i][j] = ms.V[N * i + j];
+  Mat<6, N> PHt = mulT(P, H);
+  Mat<N, 6> HP = mul(H, P);
+  Mat<N, N> S = mulT(HP, H);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) S.a[i][j] = S.a[i][j] + Vm.a[i][j];
+  const Mat<N, N> Si = inverseSmall(S);
+  Mat<6, N> K = mul(PHt, Si);
   if (hasNaN(K)) return ACTS_KF_STATUS_UPDATE_FAILED;
 
-  Mat<2, 1> Hx = mul(H, x);
-  Mat<2, 1> r;
-  r.a[0][0] = meas[0] - Hx.a[0][0];
-  r.a[1][0] = meas[1] - Hx.a[1][0];
+  Mat<N, 1> Hx = mul(H, x);
+  Mat<N, 1> r;
+  for (int i = 0; i < N; ++i) r.a[i][0] = ms.v[i] - Hx.a[i][0];
   Mat<6, 1> Kr = mul(K, r);
   for (int i = 0; i < 6; ++i) filtered.a[i][0] = x.a[i][0] + Kr.a[i][0];
   normalizePhiTheta(filtered.a[2][0], filtered.a[3][0]);
@@ -1265,44 +1277,59 @@ int gainMatrixUpdate(const Vec6& x, const M6& P, const double meas[2],
     filteredCov = mul(tmp, P);
   } else {
     M6 a = mulT(mul(tmp, P), tmp);
-    Mat<6, 2> KV = mul(K, Vm);
+    Mat<6, N> KV = mul(K, Vm);
     M6 b = mulT(KV, K);
     for (int i = 0; i < 6; ++i)
       for (int j = 0; j < 6; ++j) filteredCov.a[i][j] = a.a[i][j] + b.a[i][j];
   }
 
-  Mat<2, 1> Hf = mul(H, filtered);
-  Mat<2, 1> res;
-  res.a[0][0] = meas[0] - Hf.a[0][0];
-  res.a[1][0] = meas[1] - Hf.a[1][0];
-  Mat<2, 2> HK = mul(H, K);
-  Mat<2, 2> IHK;
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) IHK.a[i][j] = (i == j ? 1.0 : 0.0) - HK.a[i][j];
-  Mat<2, 2> m = mul(IHK, Vm);
-  Mat<2, 2> mi;
-  {
-    const double det = m.a[0][0] * m.a[1][1] - m.a[1][0] * m.a[0][1];
-    const double invdet = 1.0 / det;
-    mi.a[0][0] = m.a[1][1] * invdet;
-    mi.a[1][0] = -m.a[1][0] * invdet;
-    mi.a[0][1] = -m.a[0][1] * invdet;
-    mi.a[1][1] = m.a[0][0] * invdet;
-  }
+  Mat<N, 1> Hf = mul(H, filtered);
+  Mat<N, 1> res;
+  for (int i = 0; i < N; ++i) res.a[i][0] = ms.v[i] - Hf.a[i][0];
+  Mat<N, N> HK = mul(H, K);
+  Mat<N, N> IHK;
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) IHK.a[i][j] = (i == j ? 1.0 : 0.0) - HK.a[i][j];
+  const Mat<N, N> mi = inverseSmall(mul(IHK, Vm));
   // (r^T * m^-1) * r
-  double row0 = res.a[0][0] * mi.a[0][0] + res.a[1][0] * mi.a[1][0];
-  double row1 = res.a[0][0] * mi.a[0][1] + res.a[1][0] * mi.a[1][1];
-  chi2 = row0 * res.a[0][0] + row1 * res.a[1][0];
+  chi2 = 0;
+  for (int j = 0; j < N; ++j) {
+    double row = res.a[0][0] * mi.a[0][j];
+    for (int i = 1; i < N; ++i) row = row + res.a[i][0] * mi.a[i][j];
+    chi2 = (j == 0) ? row * res.a[0][0] : chi2 + row * res.a[j][0];
+  }
   return 0;
 }
 
-// TrackHelpers.hpp:676-694 calculatePredictedChi2 for a 2D measurement
-static double predictedChi2(const Vec6& x, const M6& P, const double meas[2],
-                            const double V[4]) {
-  double r0 = meas[0] - x.a[0][0];
-  double r1 = meas[1] - x.a[1][0];
-  double S00 = P.a[0][0] + V[0], S01 = P.a[0][1] + V[1];
-  double S10 = P.a[1][0] + V[2], S11 = P.a[1][1] + V[3];
+int gainMatrixUpdate(const Vec6& x, const M6& P, const Meas& m, bool useJoseph,
+                     Vec6& filtered, M6& filteredCov, double& chi2) {
+  if (m.dim == 1) return gainMatrixUpdateN<1>(x, P, m, useJoseph, filtered, filteredCov, chi2);
+  return gainMatrixUpdateN<2>(x, P, m, useJoseph, filtered, filteredCov, chi2);
+}
+
+int gainMatrixUpdate(const Vec6& x, const M6& P, const double meas[2],
+                     const double V[4], bool useJoseph, Vec6& filtered,
+                     M6& filteredCov, double& chi2) {
+  Meas m;
+  for (int k = 0; k < 2; ++k) m.v[k] = meas[k];
+  for (int k = 0; k < 4; ++k) m.V[k] = V[k];
+  return gainMatrixUpdateN<2>(x, P, m, useJoseph, filtered, filteredCov, chi2);
+}
+
+// TrackHelpers.hpp:676-694 calculatePredictedChi2 (residual of the
+// prediction, its covariance H P H^T + V)
+static double predictedChi2(const Vec6& x, const M6& P, const Meas& m) {
+  if (m.dim == 1) {
+    const double r0 = m.v[0] - x.a[m.idx[0]][0];
+    const double S00 = P.a[m.idx[0]][m.idx[0]] + m.V[0];
+    const double i00 = 1.0 / S00;
+    return (r0 * i00) * r0;
+  }
+  const int a = m.idx[0], b = m.idx[1];
+  double r0 = m.v[0] - x.a[a][0];
+  double r1 = m.v[1] - x.a[b][0];
+  double S00 = P.a[a][a] + m.V[0], S01 = P.a[a][b] + m.V[1];
+  double S10 = P.a[b][a] + m.V[2], S11 = P.a[b][b] + m.V[3];
   const double det = S00 * S11 - S10 * S01;
   const double invdet = 1.0 / det;
   double i00 = S11 * invdet, i10 = -S10 * invdet, i01 = -S01 * invdet,
@@ -1530,15 +1557,14 @@ int kfFilter(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
     ks.s.path = st.pathAccumulated;
     ks.s.flags = ACTS_KF_STATE_HAS_PARAMETERS;
     if (hasMat) ks.s.flags |= ACTS_KF_STATE_HAS_MATERIAL;
-    const double* mv = &in.measValue[2 * mi];
-    const double* mc = &in.measCov[4 * mi];
+    const Meas ms = in.meas(mi);
     bool outlier = false;
     if (o.outlier_chi2_cut > 0) {
-      outlier = predictedChi2(bp, st.cov, mv, mc) > o.outlier_chi2_cut;
+      outlier = predictedChi2(bp, st.cov, ms) > o.outlier_chi2_cut;
     }
     if (!outlier) {
       double chi2 = 0;
-      r = gainMatrixUpdate(ks.s.predicted, ks.s.predictedCov, mv, mc,
+      r = gainMatrixUpdate(ks.s.predicted, ks.s.predictedCov, ms,
                            o.use_joseph != 0, ks.s.filtered, ks.s.filteredCov,
                            chi2);
       if (r) return r;
@@ -1617,30 +1643,48 @@ void mbfSmooth(const TrackIn& in, std::vector<KFState>& S, int tip) {
         (s.flags & ACTS_KF_STATE_HAS_MEASUREMENT) && !(s.flags & ACTS_KF_STATE_IS_OUTLIER);
     if (isMeasurement) {
       const M6& P = s.predictedCov;
-      const double* m = &in.measValue[2 * s.meas];
-      const double* V = &in.measCov[4 * s.meas];
-      // S = H P H^T + V and its inverse (Eigen's closed-form 2x2 inverse)
-      const double S00 = P.a[0][0] + V[0], S01 = P.a[0][1] + V[1];
-      const double S10 = P.a[1][0] + V[2], S11 = P.a[1][1] + V[3];
-      const double det = S00 * S11 - S10 * S01;
-      const double invdet = 1.0 / det;
-      const double Si[2][2] = {{S11 * invdet, -S01 * invdet}, {-S10 * invdet, S00 * invdet}};
+      const Meas ms = in.meas(s.meas);
+      const int N = ms.dim;
+      const int* ix = ms.idx;
+      // S = H P H^T + V and its inverse (Eigen's closed-form 1x1 / 2x2
+      // inverse)
+      double Si[2][2] = {{0, 0}, {0, 0}};
+      if (N == 1) {
+        Si[0][0] = 1.0 / (P.a[ix[0]][ix[0]] + ms.V[0]);
+      } else {
+        const double S00 = P.a[ix[0]][ix[0]] + ms.V[0], S01 = P.a[ix[0]][ix[1]] + ms.V[1];
+        const double S10 = P.a[ix[1]][ix[0]] + ms.V[2], S11 = P.a[ix[1]][ix[1]] + ms.V[3];
+        const double det = S00 * S11 - S10 * S01;
+        const double invdet = 1.0 / det;
+        Si[0][0] = S11 * invdet;
+        Si[0][1] = -S01 * invdet;
+        Si[1][0] = -S10 * invdet;
+        Si[1][1] = S00 * invdet;
+      }
       // K = (P H^T) S^-1, CHat = I - K H, y = m - H x_pred
       double K[6][2];
       for (int i = 0; i < 6; ++i)
-        for (int j = 0; j < 2; ++j) K[i][j] = P.a[i][0] * Si[0][j] + P.a[i][1] * Si[1][j];
+        for (int j = 0; j < N; ++j) {
+          double acc = P.a[i][ix[0]] * Si[0][j];
+          if (N == 2) acc = acc + P.a[i][ix[1]] * Si[1][j];
+          K[i][j] = acc;
+        }
       M6 CHat = M6::identity();
       for (int i = 0; i < 6; ++i)
-        for (int j = 0; j < 2; ++j) CHat.a[i][j] = CHat.a[i][j] - K[i][j];
-      const double y0 = m[0] - s.predicted.a[0][0], y1 = m[1] - s.predicted.a[1][0];
+        for (int j = 0; j < N; ++j) CHat.a[i][ix[j]] = CHat.a[i][ix[j]] - K[i][j];
+      double y[2] = {0, 0};
+      for (int j = 0; j < N; ++j) y[j] = ms.v[j] - s.predicted.a[ix[j]][0];
       // Lambda~ = (H^T S^-1) H + (CHat^T Lambda) CHat
       M6 bigLT = mul(mul(transpose(CHat), bigL), CHat);
-      for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) bigLT.a[i][j] = Si[i][j] + bigLT.a[i][j];
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) bigLT.a[ix[i]][ix[j]] = Si[i][j] + bigLT.a[ix[i]][ix[j]];
       // lambda~ = (-H^T S^-1) y + CHat^T lambda
       Vec6 smallLT = mul(transpose(CHat), smallL);
-      for (int i = 0; i < 2; ++i)
-        smallLT.a[i][0] = (-Si[i][0] * y0 + -Si[i][1] * y1) + smallLT.a[i][0];
+      for (int i = 0; i < N; ++i) {
+        double acc = -Si[i][0] * y[0];
+        if (N == 2) acc = acc + -Si[i][1] * y[1];
+        smallLT.a[ix[i]][0] = acc + smallLT.a[ix[i]][0];
+      }
       bigL = mul(mul(Ft, bigLT), F);
       smallL = mul(Ft, smallLT);
     } else {
@@ -1800,7 +1844,7 @@ void fitTrack(const Geometry& geo, const TrackIn& in, TrackOut& out) {
     } else if (f & ACTS_KF_STATE_HAS_MEASUREMENT) {
       ++out.nMeas;
       chi2 += S[k].s.chi2;
-      out.ndf += 2;
+      out.ndf += in.meas(S[k].s.meas).dim;
     }
   }
   out.chi2 = chi2;

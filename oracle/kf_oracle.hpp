@@ -358,6 +358,15 @@ struct TrackOut {
   int nSteps = 0, nTrials = 0;
 };
 
+// One measurement: N = dim in {1, 2} bound parameters idx[0..N-1] with
+// the values v[0..N-1] and the N x N row-major covariance V.
+struct Meas {
+  int dim = 2;
+  int idx[2] = {0, 1};
+  double v[2] = {0, 0};
+  double V[4] = {0, 0, 0, 0};
+};
+
 struct TrackIn {
   Vec6 startParams;
   M6 startCov;
@@ -366,6 +375,19 @@ struct TrackIn {
   std::vector<int> measSurface;
   std::vector<double> measValue;  // 2 per measurement
   std::vector<double> measCov;    // 4 per measurement
+  std::vector<uint8_t> measDim;   // per measurement (empty: all 2)
+  std::vector<uint8_t> measIdx;   // 2 per measurement (empty: 0, 1)
+  Meas meas(int i) const {
+    Meas m;
+    m.dim = measDim.empty() ? 2 : measDim[i];
+    if (!measIdx.empty()) {
+      m.idx[0] = measIdx[2 * i];
+      m.idx[1] = measIdx[2 * i + 1];
+    }
+    for (int k = 0; k < 2; ++k) m.v[k] = measValue[2 * i + k];
+    for (int k = 0; k < 4; ++k) m.V[k] = measCov[4 * i + k];
+    return m;
+  }
 };
 
 // KalmanFitter::fit with DirectNavigator + GainMatrixUpdater +
@@ -377,6 +399,9 @@ int gainMatrixUpdate(const Vec6& predicted, const M6& predictedCov,
                      const double meas[2], const double measCov[4],
                      bool useJoseph, Vec6& filtered, M6& filteredCov,
                      double& chi2);
+// the same for a measurement of dimension 1 or 2 on any subspace
+int gainMatrixUpdate(const Vec6& predicted, const M6& predictedCov, const Meas& m,
+                     bool useJoseph, Vec6& filtered, M6& filteredCov, double& chi2);
 
 // GainMatrixSmoother.cpp:18-97 for one pair of states
 int gainMatrixSmoothStep(const Vec6& filtered, const M6& filteredCov,

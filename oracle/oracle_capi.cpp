@@ -1,6 +1,7 @@
 // oracle_capi.cpp -- C entry points of the CPU oracle.
 // TEST INFRASTRUCTURE ONLY (see kf_oracle.hpp): loaded by tests/, by
 // __graft_entry__.smoke() as the checker and by bench.py's cpu_baseline leg.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstring>
@@ -87,6 +88,10 @@ void loadTrack(const acts_kf_batch_in* in, int64_t t, TrackIn& tr) {
   tr.measSurface.assign(in->meas_surfaces + m0, in->meas_surfaces + m1);
   tr.measValue.assign(in->meas_values + 2 * m0, in->meas_values + 2 * m1);
   tr.measCov.assign(in->meas_cov + 4 * m0, in->meas_cov + 4 * m1);
+  tr.measDim.clear();
+  tr.measIdx.clear();
+  if (in->meas_dims) tr.measDim.assign(in->meas_dims + m0, in->meas_dims + m1);
+  if (in->meas_indices) tr.measIdx.assign(in->meas_indices + 2 * m0, in->meas_indices + 2 * m1);
 }
 
 template <int R, int C>
@@ -108,7 +113,24 @@ void storeTrack(const acts_kf_batch_in* in, const acts_kf_batch_out* out,
   if (out->n_step_trials) out->n_step_trials[t] = tr.nTrials;
   const int64_t base = in->seq_offsets[t];
   const int64_t cap = in->seq_offsets[t + 1] - base;
-  for (size_t k = 0; k < tr.states.size() && static_cast<int64_t>(k) < cap; ++k) {
+  // slots without a state (past the tip, all of a failed track's) hold an
+  // empty slot: surface and measurement -1, flags 0, numbers 0
+  const int64_t nDef = tr.status ? 0 : std::min<int64_t>(cap, static_cast<int64_t>(tr.states.size()));
+  for (int64_t k = nDef; k < cap; ++k) {
+    const int64_t slot = base + k;
+    if (out->state_surface) out->state_surface[slot] = -1;
+    if (out->state_flags) out->state_flags[slot] = 0u;
+    if (out->state_path) out->state_path[slot] = 0.0;
+    if (out->state_chi2) out->state_chi2[slot] = 0.f;
+    double* arr6[] = {out->predicted, out->filtered, out->smoothed};
+    for (double* a : arr6)
+      if (a) std::fill(a + 6 * slot, a + 6 * slot + 6, 0.0);
+    double* arr36[] = {out->predicted_cov, out->filtered_cov, out->smoothed_cov, out->jacobian};
+    for (double* a : arr36)
+      if (a) std::fill(a + 36 * slot, a + 36 * slot + 36, 0.0);
+    if (out->state_measurement) out->state_measurement[slot] = -1;
+  }
+  for (size_t k = 0; static_cast<int64_t>(k) < nDef; ++k) {
     const StateOut& s = tr.states[k];
     const int64_t slot = base + static_cast<int64_t>(k);
     if (out->state_surface) out->state_surface[slot] = s.surface;

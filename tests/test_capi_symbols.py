@@ -66,7 +66,7 @@ def test_structs_match_header_sizes():
     assert C.sizeof(capi.Surface) == 4 * 4 + 8 * 14
     assert C.sizeof(capi.MaterialSlab) == 6 * 4
     assert C.sizeof(capi.BatchOut) == 8 * len(capi.BATCH_OUT_FIELDS)
-    assert C.sizeof(capi.BatchIn) == 8 * 10 + 8
+    assert C.sizeof(capi.BatchIn) == 8 * 10 + 8 + 2 * 8
 
 
 @pytest.mark.parametrize("field,value", [("smoother", 7), ("stepper", 2), ("precision", 5),

@@ -414,3 +414,76 @@ def test_long_sequences_track_states(oracle_mod):
     ulp = np.spacing(np.maximum(np.abs(g[fin]), np.abs(r[fin])).astype(np.float32))
     assert np.all(np.abs(g[fin].astype(np.float64) - r[fin]) <= 2 * ulp)
     assert np.count_nonzero(got.column("stateType") >= 0) == int(fr.n_states[fr.status == 0].sum())
+
+
+def _empty_slot_violations(res, ev):
+    """Slots past n_states (all slots of a failed track) that are not empty
+    (surface and measurement -1, flags 0, numbers 0)."""
+    so = ev.seq_offsets
+    bad = []
+    for t in range(ev.n_tracks):
+        n = 0 if res.status[t] else int(res.n_states[t])
+        sl = np.arange(so[t] + n, so[t + 1])
+        if len(sl) == 0:
+            continue
+        ok = (np.all(res.state_surface[sl] == -1) and np.all(res.state_measurement[sl] == -1)
+              and np.all(res.state_flags[sl] == 0) and np.all(res.state_path[sl] == 0)
+              and np.all(res.state_chi2[sl] == 0))
+        for name in ("predicted", "predicted_cov", "filtered", "filtered_cov", "smoothed",
+                     "smoothed_cov", "jacobian"):
+            ok = ok and np.all(getattr(res, name)[sl] == 0)
+        if not ok:
+            bad.append(t)
+    return bad
+
+
+@pytest.mark.parametrize("reference", [False, True])
+def test_empty_slots_defined(oracle_mod, reference):
+    """Every output slot is defined: after a fit of another batch has left
+    the context's staging buffers full of results, the edge batch's slots
+    past each track's states (and all slots of failed tracks) read as empty
+    slots, and the whole arrays equal the oracle's, which follows the same
+    convention."""
+    print("empty slots defined, reference", reference, flush=True)
+    det, field, ev = A.workload("endcap", n_tracks=60, seed=19)
+    eb, kinds = edge_batch(ev, n_ordinary=40)
+    kw = dict(reference_surface=capi.REFERENCE_PERIGEE) if reference else {}
+    opts = A.workload_options("endcap", **kw)
+    f = A.KalmanFitter(det, field, opts)
+    try:
+        _, _, other = A.workload("endcap", n_tracks=400, seed=5)
+        f.fit(other, outputs=ALL_STATE_OUTPUTS)
+        got = f.fit(eb, outputs=ALL_STATE_OUTPUTS)
+    finally:
+        f.close()
+    assert np.any(got.status != 0), "the edge batch has failed tracks"
+    assert _empty_slot_violations(got, eb) == []
+    ref, _ = oracle_mod.fit(det, field, opts, eb, n_threads=0, outputs=ALL_STATE_OUTPUTS)
+    assert _empty_slot_violations(ref, eb) == []
+    for name in ("status", "n_states"):
+        assert np.array_equal(getattr(got, name), getattr(ref, name)), name
+    for name in ("state_surface", "state_flags", "state_measurement"):
+        assert np.array_equal(getattr(got, name), getattr(ref, name)), name
+
+
+def test_bad_offsets_rejected():
+    """acts_kf_fit_batch rejects CSR offsets that do not start at 0 or that
+    decrease, before any device work."""
+    import dataclasses
+    det, field, ev = A.workload("barrel", n_tracks=20, seed=9)
+    f = A.KalmanFitter(det, field, A.KalmanFitterOptions())
+    try:
+        for name in ("seq_offsets", "meas_offsets"):
+            for edit in ("start", "decrease"):
+                off = getattr(ev, name).copy()
+                if edit == "start":
+                    off += 1
+                else:
+                    off[5], off[6] = off[6], off[5]
+                bad = dataclasses.replace(ev, **{name: off})
+                with pytest.raises(RuntimeError, match="offsets"):
+                    f.fit(bad)
+        res = f.fit(ev)
+        assert np.all(res.status == 0)
+    finally:
+        f.close()

@@ -1,6 +1,6 @@
 """Scheduling knobs change only the speed of a fit: results are bit for bit
 the same with quarantine, other inner step counts and wait fractions, in
-input order, and with the jacobian transport skipped after any number of
+input order, in start-direction order, and with the jacobian transport skipped after any number of
 steps since the last surface (a track that reaches a surface after all
 goes back to its checkpoint and repeats the steps with the transport; 1
 and 2 steps make nearly every track do so)."""
@@ -18,6 +18,8 @@ VARIANTS = [
     dict(sched_quarantine_steps=20, sched_inner_steps=64, sched_wait_64ths=64),
     dict(sched_inner_steps=4, sched_wait_64ths=16),
     dict(length_sorted=False),
+    dict(length_sorted=2),
+    dict(length_sorted=2, sched_wait_64ths=32),
     dict(sched_transport_skip_steps=0),
     dict(sched_transport_skip_steps=1),
     dict(sched_transport_skip_steps=2, sched_quarantine_steps=20),
@@ -47,19 +49,19 @@ def test_scheduling_does_not_change_results():
 @pytest.mark.parametrize("precision", [capi.PRECISION_FP64, capi.PRECISION_MIXED])
 def test_transport_skip_field_map(precision):
     """The same in the field map, in both precisions, with the perigee
-    extrapolation."""
+    extrapolation (and the start-direction queue order)."""
     det, field, ev = A.workload("solenoid", n_tracks=3000, seed=52)
     out = {}
-    for skip in (0, 1, 64):
+    for skip in (0, 1, 64, "direction"):
         print("skip", skip, precision, flush=True)
+        kw = dict(length_sorted=2) if skip == "direction" else dict(sched_transport_skip_steps=skip)
         f = A.KalmanFitter(det, field, A.KalmanFitterOptions(
-            precision=precision, sched_transport_skip_steps=skip,
-            reference_surface=capi.REFERENCE_PERIGEE))
+            precision=precision, reference_surface=capi.REFERENCE_PERIGEE, **kw))
         try:
             out[skip] = f.fit(ev, outputs=ALL_STATE_OUTPUTS + ("reference_params", "reference_cov"))
         finally:
             f.close()
-    for skip in (1, 64):
+    for skip in (1, 64, "direction"):
         for name in FIELDS + ("reference_params", "reference_cov"):
             assert np.array_equal(getattr(out[skip], name), getattr(out[0], name),
                                   equal_nan=True), (skip, name)
