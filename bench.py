@@ -48,6 +48,14 @@ OUTPUTS = ("state_surface", "state_flags", "state_path", "state_chi2", "smoothed
 CPU_THREADS_PER_GPU = 16
 
 
+_T0 = time.perf_counter()
+
+
+def log(msg: str) -> None:
+    """Progress on stderr (the JSON line is the only stdout output)."""
+    print(f"bench [{time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +158,7 @@ def serving(args, det, field, pool):
         sat_cfg = dataclasses.replace(base, rate_hz=4e4, duration_s=0.05)
         sat = serve(det, field, pool, sat_cfg, A.KalmanFitterOptions(precision=prec))
         cap[name] = sat["throughput_tracks_per_s"]
+        log(f"serving {name}: capacity {cap[name]:.3g} tracks/s")
         out[f"saturated_{name}"] = sat
     out["capacity_tracks_per_s"] = cap
     out["mixed_over_fp64_capacity"] = cap["mixed"] / cap["fp64"]
@@ -424,6 +433,7 @@ def ragged_section(args):
     import acts_amd as A
     from acts_amd import capi
     from acts_amd.autotune import autotune
+    log(f"ragged: generating {args.ragged_tracks} tracks")
     det, field, events = A.workload("ragged", n_tracks=args.ragged_tracks, seed=args.seed)
     opts = A.workload_options("ragged")
     lens = np.diff(events.meas_offsets)
@@ -438,6 +448,7 @@ def ragged_section(args):
             continue
         o = dataclasses.replace(opts, precision=prec)
         tuned, _ = autotune(det, field, sample, o, cache_path="")
+        log(f"ragged {name}: tuned {tuned}")
         b = Bench(args, det, field, events, o, dict(tuned, use_graph=True))
         ms, _, _, _ = b.run(prec, streams=2, steps=max(3, args.steps // 4))
         ms1 = b.single[(prec, ())]
@@ -463,8 +474,10 @@ def main() -> int:
     from acts_amd.flops import batch_flops, forward_flops
 
     t_start = time.perf_counter()
+    log(f"generating the {args.workload} workload")
     det, field, events = A.workload(args.workload, n_tracks=args.n_tracks, seed=args.seed)
     n = events.n_tracks
+    log(f"{n} tracks generated")
     wl_opts = A.workload_options(args.workload)
     tuned = {}
     if args.autotune:
@@ -476,6 +489,7 @@ def main() -> int:
 
     ms64, wall64, kt64, out64 = b.run(capi.PRECISION_FP64)
     torch.cuda.synchronize()
+    log(f"FP64: {ms64:.3f} ms per fit in flight")
 
     if args.copy_calibration_mib > 0:
         nbytes = args.copy_calibration_mib << 20
@@ -553,7 +567,9 @@ def main() -> int:
             "pulls": quality.pulls(firstm, events),
             "first_state_smoothed_rel_err_vs_fp64": quality.relative_error(firstm, first64)}
         del outm
+        log(f"mixed: {msm:.3f} ms per fit in flight")
     line["with_host_copies"] = host_copies(det, field, wl_opts, events, 5)
+    log("host copies done")
 
     if not args.no_variants:
         msp, _, _, outp = b.run(capi.PRECISION_FP64,
@@ -601,14 +617,17 @@ def main() -> int:
             "config": "SympyStepper + MbfSmoother + perigee (first measurement)",
             "fit_failures": int(np.count_nonzero(oute.host("status")))}
         del oute
+        log("variants done")
 
     if tuned:
         line["config"]["scheduling"] = tuned
 
     if not args.no_endcap and args.workload != "endcap":
         line["endcap"] = endcap_section(args)
+        log("endcap done")
     if args.ragged_tracks > 0 and args.workload != "ragged":
         line["ragged"] = ragged_section(args)
+        log("ragged done")
 
     if args.serving_seconds > 0:
         if args.workload == "solenoid" and args.n_tracks is None:
@@ -617,10 +636,13 @@ def main() -> int:
         else:
             sdet, sfield, pool = A.workload("solenoid", n_tracks=200_000, seed=args.seed + 1)
         line["serving"] = serving(args, sdet, sfield, pool)
+        log("serving done")
 
     if not args.no_cpu_baseline:
         threads = max(1, min(CPU_THREADS_PER_GPU, host_cpu()["affinity_cpus"]))
+        log(f"CPU baseline on {threads} threads")
         line["cpu_baseline"], ref = cpu_baseline(det, field, wl_opts, events, threads)
+        log("CPU baseline done")
         got = type("GpuFit", (), {})()
         for name in ("status", "n_states", "chi2", "state_surface", "state_flags", "smoothed",
                      "smoothed_cov"):
