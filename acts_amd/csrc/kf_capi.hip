@@ -132,9 +132,10 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
 }
 
 // Track-state table: one thread per row
+template <bool Sub>
 __global__ __launch_bounds__(kTsBlock) void trackStatesKernel(TsArgs a) {
   const int64_t slot = static_cast<int64_t>(blockIdx.x) * kTsBlock + threadIdx.x;
-  if (slot < a.nSlots) trackStateRow(a, slot);
+  if (slot < a.nSlots) trackStateRow<Sub>(a, slot);
 }
 
 // Smoother (RTS, or MBF with Mbf) + track summary + outputs: one lane per
@@ -1104,6 +1105,8 @@ int acts_kf_track_states_device(acts_kf_ctx* c, const acts_kf_batch_in* in,
   a.measOff = in->meas_offsets;
   a.measVal = in->meas_values;
   a.measCov = in->meas_cov;
+  a.measDims = in->meas_dims;
+  a.measIdx = in->meas_indices;
   a.status = f->status;
   a.nStates = f->n_states;
   a.stateSurf = f->state_surface;
@@ -1122,8 +1125,8 @@ int acts_kf_track_states_device(acts_kf_ctx* c, const acts_kf_batch_in* in,
   a.cols = out->columns;
   a.ints = out->ints;
   const int64_t blocks = (out->n_slots + kTsBlock - 1) / kTsBlock;
-  hipLaunchKernelGGL(trackStatesKernel, dim3(static_cast<unsigned>(blocks)), dim3(kTsBlock), 0, s,
-                     a);
+  hipLaunchKernelGGL((a.measDims || a.measIdx) ? trackStatesKernel<true> : trackStatesKernel<false>,
+                     dim3(static_cast<unsigned>(blocks)), dim3(kTsBlock), 0, s, a);
   KF_HIP(hipGetLastError());
   return markQueued(c, s);
 }

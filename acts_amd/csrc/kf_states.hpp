@@ -22,6 +22,8 @@ struct TsArgs {
   const int64_t* measOff;
   const double* measVal;
   const double* measCov;
+  const uint8_t* measDims;  // null: all 2D on (loc0, loc1)
+  const uint8_t* measIdx;
   const int32_t* status;
   const int32_t* nStates;
   const int32_t* stateSurf;
@@ -65,24 +67,49 @@ KF_HD double etaOf(const V3& v) {
   return asinh(v.z / perp(v));
 }
 
-// calculateUnbiasedParametersCovariance for a 2D measurement on (loc0,
-// loc1) (TrackHelpersUnbiased.cpp:37-52): K = C H^T (H C H^T - V)^-1,
-// s + K (m - H s), C - K H C, with the 2x2 inverse in Eigen's closed form.
-// The table needs the parameters and the diagonal of the covariance; C is
-// read at rows 0-1, columns 0-1 and the diagonal (row-major 6x6).
-KF_HD void unbiasedState(const double* s, const double* C, const double* m, const double* V,
-                         double* su, double* du) {
-  const double S00 = C[0] - V[0], S01 = C[1] - V[1];
-  const double S10 = C[6] - V[2], S11 = C[7] - V[3];
-  const double det = S00 * S11 - S10 * S01;
-  const double invdet = 1.0 / det;
-  const double Si[2][2] = {{S11 * invdet, -S01 * invdet}, {-S10 * invdet, S00 * invdet}};
-  const double r0 = m[0] - s[0], r1 = m[1] - s[1];
+// v[k] for a run-time k in 0..5, by selects (a register array indexed at
+// run time would be moved to scratch memory)
+KF_HD double pick6(const double* v, int k) {
+  double r = v[0];
+#pragma unroll
+  for (int i = 1; i < 6; ++i) r = (k == i) ? v[i] : r;
+  return r;
+}
+
+// calculateUnbiasedParametersCovariance (TrackHelpersUnbiased.cpp:37-52)
+// for a measurement of dimension n on the bound parameters a (, b):
+// K = C H^T (H C H^T - V)^-1, s + K (m - H s), C - K H C, with Eigen's
+// closed-form 1x1 / 2x2 inverse.  The table needs the parameters and the
+// diagonal of the covariance; C (row-major 6x6) is read in the measured
+// rows and columns and on the diagonal.
+KF_HD void unbiasedState(const double* s, const double* C, int n, int a, int b,
+                         const double* m, const double* V, double* su, double* du) {
+  const bool two = n == 2;
+  const double S00 = C[7 * a] - V[0];
+  const double S01 = two ? C[6 * a + b] - V[1] : 0.0;
+  const double S10 = two ? C[6 * b + a] - V[2] : 0.0;
+  const double S11 = two ? C[7 * b] - V[3] : 1.0;
+  double Si[2][2];
+  if (two) {
+    const double det = S00 * S11 - S10 * S01;
+    const double invdet = 1.0 / det;
+    Si[0][0] = S11 * invdet;
+    Si[0][1] = -S01 * invdet;
+    Si[1][0] = -S10 * invdet;
+    Si[1][1] = S00 * invdet;
+  } else {
+    Si[0][0] = 1.0 / S00;
+    Si[0][1] = Si[1][0] = Si[1][1] = 0.0;
+  }
+  const double r0 = m[0] - pick6(s, a), r1 = two ? m[1] - pick6(s, b) : 0.0;
   for (int i = 0; i < 6; ++i) {
-    const double K0 = C[6 * i] * Si[0][0] + C[6 * i + 1] * Si[1][0];
-    const double K1 = C[6 * i] * Si[0][1] + C[6 * i + 1] * Si[1][1];
-    su[i] = s[i] + (K0 * r0 + K1 * r1);
-    du[i] = C[7 * i] - (K0 * C[i] + K1 * C[6 + i]);
+    double K0 = C[6 * i + a] * Si[0][0], K1 = 0.0;
+    if (two) {
+      K0 = K0 + C[6 * i + b] * Si[1][0];
+      K1 = C[6 * i + a] * Si[0][1] + C[6 * i + b] * Si[1][1];
+    }
+    su[i] = s[i] + (two ? K0 * r0 + K1 * r1 : K0 * r0);
+    du[i] = C[7 * i] - (two ? K0 * C[6 * a + i] + K1 * C[6 * b + i] : K0 * C[6 * a + i]);
   }
 }
 
@@ -135,6 +162,10 @@ constexpr int kTsBlock = 256;
 // stored as float: the cost is the FP64 transcendental and division work
 // of four parameter sets (~5 k lane instructions per state), not the
 // 1.8 KB it moves.
+// Sub: the batch has measurement subspaces (dimension and indices read per
+// measurement); otherwise every measurement is 2D on (loc0, loc1), known at
+// compile time.
+template <bool Sub>
 __device__ inline void trackStateRow(const TsArgs& a, int64_t slot) {
   const int64_t nSeq = a.seqOff[a.nTracks];
   const bool inTable = slot < a.nSlots;
@@ -168,6 +199,15 @@ __device__ inline void trackStateRow(const TsArgs& a, int64_t slot) {
   const DevSurface& s = a.geo[ok ? sid : 0];
   const double* truth = (link && a.truth) ? a.truth + 6 * static_cast<int64_t>(mi) : nullptr;
   double m[2] = {0, 0}, V[4] = {0, 0, 0, 0};
+  int mdim = 2, ma = 0, mb = 1;
+  if constexpr (Sub) {
+    if (link && a.measDims) mdim = a.measDims[mi];
+    if (link && a.measIdx) {
+      ma = a.measIdx[2 * static_cast<int64_t>(mi)];
+      mb = a.measIdx[2 * static_cast<int64_t>(mi) + 1];
+    }
+    if (mdim == 1) mb = ma;
+  }
   if (ok) {
     const uint32_t flags = a.stateFlags[slot];
     // RootTrackStatesWriter::getStateType (RootTrackStatesWriter.cpp:293-308)
@@ -177,7 +217,7 @@ __device__ inline void trackStateRow(const TsArgs& a, int64_t slot) {
                          : (flags & ACTS_KF_STATE_HAS_MATERIAL)    ? 3
                                                                    : 4;
     puti(ACTS_KF_TS_STATE_TYPE, type);
-    puti(ACTS_KF_TS_DIM_HIT, link ? 2 : 0);
+    puti(ACTS_KF_TS_DIM_HIT, link ? mdim : 0);
     puti(ACTS_KF_TS_HAS_PARAMS, link ? 0xF : 0x7);
     put(ACTS_KF_TS_CHI2, a.stateChi2[slot]);
     put(ACTS_KF_TS_PATH_LENGTH, static_cast<float>(a.statePath[slot]));
@@ -204,9 +244,13 @@ __device__ inline void trackStateRow(const TsArgs& a, int64_t slot) {
     if (link) {
       for (int i = 0; i < 2; ++i) m[i] = a.measVal[2 * static_cast<int64_t>(mi) + i];
       for (int i = 0; i < 4; ++i) V[i] = a.measCov[4 * static_cast<int64_t>(mi) + i];
-      const V3 g = localToGlobal(s, m[0], m[1]);
-      put(ACTS_KF_TS_L_X_HIT + 0, clampF(m[0]));
-      put(ACTS_KF_TS_L_X_HIT + 1, clampF(m[1]));
+      // the measurement expanded into the bound space: its loc0 and loc1
+      // (0 where not measured)
+      const double l0 = (ma == 0) ? m[0] : ((mdim == 2 && mb == 0) ? m[1] : 0.0);
+      const double l1 = (ma == 1) ? m[0] : ((mdim == 2 && mb == 1) ? m[1] : 0.0);
+      const V3 g = localToGlobal(s, l0, l1);
+      put(ACTS_KF_TS_L_X_HIT + 0, clampF(l0));
+      put(ACTS_KF_TS_L_X_HIT + 1, clampF(l1));
       put(ACTS_KF_TS_L_X_HIT + 2, clampF(g.x));
       put(ACTS_KF_TS_L_X_HIT + 3, clampF(g.y));
       put(ACTS_KF_TS_L_X_HIT + 4, clampF(g.z));
@@ -226,17 +270,25 @@ __device__ inline void trackStateRow(const TsArgs& a, int64_t slot) {
     const double* xp = a.par[p] + 6 * slot;
     for (int i = 0; i < 6; ++i) x[i] = xp[i];
     for (int i = 0; i < 6; ++i) d[i] = C[7 * i];
-    if (p == ACTS_KF_TS_SMOOTHED && link) unbiasedState(x, C, m, V, xu, du);
+    if (p == ACTS_KF_TS_SMOOTHED && link) unbiasedState(x, C, mdim, ma, mb, m, V, xu, du);
     paramColumns(s, x, d, truth, put, ACTS_KF_TS_PARAMS(p));
     if (p == ACTS_KF_TS_PREDICTED) {
       // hit residual against the prediction: res = m - H x, resCov = V + H C H^T
       if (link) {
-        const double resX = m[0] - x[0], resY = m[1] - x[1];
-        const double errX = V[0] >= 0 ? sqrt(V[0]) : static_cast<double>(NAN);
-        const double errY = V[3] >= 0 ? sqrt(V[3]) : static_cast<double>(NAN);
-        const double rcX = V[0] + d[0], rcY = V[3] + d[1];
-        const double pullX = rcX > 0 ? resX / sqrt(rcX) : static_cast<double>(NAN);
-        const double pullY = rcY > 0 ? resY / sqrt(rcY) : static_cast<double>(NAN);
+        // the x columns take the first measured component, the y columns
+        // the second (NaN for a 1D measurement)
+        const double nan = static_cast<double>(NAN);
+        const double resX = m[0] - pick6(x, ma);
+        const double errX = V[0] >= 0 ? sqrt(V[0]) : nan;
+        const double rcX = V[0] + pick6(d, ma);
+        const double pullX = rcX > 0 ? resX / sqrt(rcX) : nan;
+        double resY = nan, errY = nan, pullY = nan;
+        if (mdim == 2) {
+          resY = m[1] - pick6(x, mb);
+          errY = V[3] >= 0 ? sqrt(V[3]) : nan;
+          const double rcY = V[3] + pick6(d, mb);
+          pullY = rcY > 0 ? resY / sqrt(rcY) : nan;
+        }
         put(ACTS_KF_TS_RES_X_HIT + 0, clampF(resX));
         put(ACTS_KF_TS_RES_X_HIT + 1, clampF(resY));
         put(ACTS_KF_TS_RES_X_HIT + 2, clampF(errX));

@@ -46,6 +46,12 @@ def silicon() -> Material:
                     molar_rho=(2.329 / 28.0855) * 1.0 / (CM ** 3))
 
 
+def beryllium() -> Material:
+    """Beryllium as Tests/CommonHelpers/.../PredefinedMaterials.hpp:17-21."""
+    return Material(x0=35.28 * CM, l0=42.10 * CM, ar=9.012, z=4.0,
+                    molar_rho=(1.848 / 9.012) * 1.0 / (CM ** 3))
+
+
 @dataclass
 class SurfaceDesc:
     type: int
@@ -61,6 +67,9 @@ class Detector:
     surfaces: list[SurfaceDesc] = field(default_factory=list)
     materials: list[tuple[Material, float]] = field(default_factory=list)
     name: str = ""
+    # per surface (dimension, bound indices, sigmas) of its measurements;
+    # None: 2D (loc0, loc1) with the generator's resolutions
+    kinds: tuple | None = None
 
     def add_material(self, mat: Material, thickness: float) -> int:
         self.materials.append((mat, thickness))
@@ -145,6 +154,37 @@ def double_layer_detector(radii=BARREL_RADII, half_length=3000.0, disc_z=None,
             det.surfaces.append(SurfaceDesc(capi.SURFACE_DISC, np.eye(3),
                                             np.array([0.0, 0.0, side * z]),
                                             (disc_rmin, disc_rmax), mi, True))
+    return det
+
+
+# measurement kinds of a telescope plane: (dimension, bound indices, sigmas)
+PIXEL = (2, (0, 1), (25 * 1e-3, 50 * 1e-3))
+STRIP0 = (1, (0, 0), (100 * 1e-3,))
+STRIP1 = (1, (1, 1), (150 * 1e-3,))
+
+
+def telescope_detector(shift=3000.0) -> Detector:
+    """The reference's cubic test telescope (Tests/CommonHelpers/src/
+    CubicTrackingGeometry.cpp:32-112): six planes normal to x at x = -2,
+    -1, 1 -+ 0.001, 2 -+ 0.001 m, each 0.5 mm of beryllium, the last four
+    rotated by +-0.026 rad about their normal (stereo strips); local x is
+    global z and local y global y (rotation by 90 degrees about y).  The
+    planes are shifted by `shift` along x, so that tracks from the origin
+    start where the reference's start at x = -3 m does.  `kinds` gives the
+    measurement of each plane as in FitterTestsCommon.hpp:145-155: 2D pixels
+    (25 / 50 um) on the first two, 1D strips on loc0 (100 um) and loc1
+    (150 um) alternating on the others."""
+    det = Detector(name="telescope")
+    mi = det.add_material(beryllium(), 0.5)
+    xs = (-2000.0, -1000.0, 999.0, 1001.0, 1999.0, 2001.0)
+    angles = (0.0, 0.0, 0.026, -0.026, 0.026, -0.026)
+    ry = np.array([[0.0, 0.0, -1.0], [0.0, 1.0, 0.0], [1.0, 0.0, 0.0]])
+    for x, a in zip(xs, angles):
+        rz = np.array([[math.cos(a), -math.sin(a), 0.0], [math.sin(a), math.cos(a), 0.0],
+                       [0.0, 0.0, 1.0]])
+        det.surfaces.append(SurfaceDesc(capi.SURFACE_PLANE, ry @ rz,
+                                        np.array([x + shift, 0.0, 0.0]), (0.0, 0.0), mi, True))
+    det.kinds = (PIXEL, PIXEL, STRIP0, STRIP1, STRIP0, STRIP1)
     return det
 
 

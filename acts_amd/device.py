@@ -33,12 +33,17 @@ class DeviceEvents:
         self.max_sequence_length = events.max_sequence_length
         self.n_slots = int(events.seq_offsets[-1])
         self.t = {name: to_device(getattr(events, name), device) for name in self.NAMES}
+        for name in ("meas_dims", "meas_indices"):
+            arr = getattr(events, name, None)
+            if arr is not None:
+                self.t[name] = torch.from_numpy(np.ascontiguousarray(arr)).to(device)
 
     def batch_in(self) -> capi.BatchIn:
         b = capi.BatchIn()
         b.n_tracks = self.n_tracks
-        for name in self.NAMES:
-            setattr(b, name, self.t[name].data_ptr())
+        for name in self.NAMES + ("meas_dims", "meas_indices"):
+            if name in self.t:
+                setattr(b, name, self.t[name].data_ptr())
         b.max_sequence_length = self.max_sequence_length
         return b
 

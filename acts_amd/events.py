@@ -18,7 +18,8 @@ import numpy as np
 
 from . import capi
 from .detector import (ConstantField, Detector, FieldMap, MUON_MASS, NS, UM,
-                       barrel_detector, barrel_endcap_detector, double_layer_detector)
+                       barrel_detector, barrel_endcap_detector, double_layer_detector,
+                       telescope_detector)
 
 
 @dataclass
@@ -36,6 +37,9 @@ class Events:
     truth_start: np.ndarray    # [n,6]
     truth_meas: np.ndarray     # [m,6]
     meas_is_outlier: np.ndarray  # [m] int32
+    # measurement subspaces (None: every measurement 2D on loc0, loc1)
+    meas_dims: np.ndarray | None = None      # [m] uint8
+    meas_indices: np.ndarray | None = None   # [m,2] uint8
 
     @property
     def max_sequence_length(self) -> int:
@@ -54,6 +58,11 @@ class Events:
             assert arr.flags["C_CONTIGUOUS"]
             setattr(b, name, arr.ctypes.data)
         b.max_sequence_length = self.max_sequence_length
+        for name in ("meas_dims", "meas_indices"):
+            arr = getattr(self, name)
+            if arr is not None:
+                assert arr.flags["C_CONTIGUOUS"] and arr.dtype == np.uint8
+                setattr(b, name, arr.ctypes.data)
         return b
 
     def subset(self, idx) -> "Events":
@@ -81,7 +90,11 @@ class Events:
             meas_cov=np.ascontiguousarray(self.meas_cov[mid]),
             truth_start=np.ascontiguousarray(self.truth_start[idx]),
             truth_meas=np.ascontiguousarray(self.truth_meas[mid]),
-            meas_is_outlier=np.ascontiguousarray(self.meas_is_outlier[mid]))
+            meas_is_outlier=np.ascontiguousarray(self.meas_is_outlier[mid]),
+            meas_dims=None if self.meas_dims is None
+            else np.ascontiguousarray(self.meas_dims[mid]),
+            meas_indices=None if self.meas_indices is None
+            else np.ascontiguousarray(self.meas_indices[mid]))
 
 
 def particle_options(mass: float = MUON_MASS) -> capi.Options:
@@ -93,7 +106,7 @@ def particle_options(mass: float = MUON_MASS) -> capi.Options:
 
 
 def gen_config(n_tracks: int, seed: int, *, pt=(1.0, 10.0), pt_log=False,
-               eta=(-1.0, 1.0), sigma_loc=(15 * UM, 50 * UM),
+               eta=(-1.0, 1.0), phi=(-np.pi, np.pi), sigma_loc=(15 * UM, 50 * UM),
                hole_probability=0.0, outlier_fraction=0.0,
                outlier_count=(1, 3), outlier_distance=20.0,
                min_measurements=1, n_threads=0) -> capi.GenConfig:
@@ -104,7 +117,7 @@ def gen_config(n_tracks: int, seed: int, *, pt=(1.0, 10.0), pt_log=False,
     c.pt_log_uniform = 1 if pt_log else 0
     c.n_threads = n_threads
     c.eta_min, c.eta_max = eta
-    c.phi_min, c.phi_max = -np.pi, np.pi
+    c.phi_min, c.phi_max = phi
     c.vertex_sigma_z = 50.0
     c.sigma_loc0, c.sigma_loc1 = sigma_loc
     start = (10 * sigma_loc[0], 10 * sigma_loc[1], 1e-3, 1e-3, 0.1, 1.0 * NS)
@@ -154,6 +167,33 @@ def generate(detector: Detector, field: ConstantField, cfg: capi.GenConfig,
     return ev
 
 
+def apply_measurement_kinds(events: Events, detector: Detector, seed: int) -> Events:
+    """Measurements of the kinds the detector's surfaces give
+    (Detector.kinds: dimension, bound indices, sigmas per surface), smeared
+    from the generator's truth with those sigmas, as the reference's
+    MeasurementsCreator does for a resolution map (Tests/CommonHelpers/
+    .../MeasurementsCreator.hpp:51-174, FitterTestsCommon.hpp:145-155).  A
+    1D measurement keeps its value and variance in the first entries of
+    meas_values and meas_cov."""
+    rng = np.random.default_rng(seed)
+    nm = len(events.meas_surfaces)
+    dims = np.zeros(nm, np.uint8)
+    idx = np.zeros((nm, 2), np.uint8)
+    vals = np.zeros((nm, 2))
+    cov = np.zeros((nm, 4))
+    noise = rng.normal(size=(nm, 2))
+    for m in range(nm):
+        d, ix, sig = detector.kinds[events.meas_surfaces[m]]
+        dims[m] = d
+        idx[m] = ix
+        for i in range(d):
+            vals[m, i] = events.truth_meas[m, ix[i]] + sig[i] * noise[m, i]
+            cov[m, 3 * i] = sig[i] ** 2
+    import dataclasses
+    return dataclasses.replace(events, meas_values=vals, meas_cov=cov, meas_dims=dims,
+                               meas_indices=idx)
+
+
 # ---- benchmark workloads ---------------------------------------------------
 WORKLOADS = {
     # 1,000 muons, barrel only, |eta| < 1, uniform Bz = 2 T
@@ -174,6 +214,11 @@ WORKLOADS = {
     # counts all its measurements, so with the direct navigator it only ends
     # at the loop-protection path limit (half a helix turn); in a finite field
     # map it would end with FIELD_OUT_OF_BOUNDS instead, as in the reference.
+    # the reference's own fitter-test geometry: 10,000 muons along x through
+    # six beryllium planes (two 2D pixel planes, four stereo 1D strip
+    # planes), pT 0.5-10 GeV, |eta| < 0.1, |phi| < 0.1, no field
+    "telescope": dict(detector=telescope_detector, n_tracks=10000, eta=(-0.1, 0.1),
+                      phi=(-0.1, 0.1), pt=(0.5, 10.0), field="zero"),
     "ragged": dict(detector=double_layer_detector, n_tracks=2_000_000, eta=(-2.5, 2.5),
                    pt=(0.5, 100.0), pt_log=True, hole_probability=0.05,
                    outlier_fraction=0.1, min_measurements=6,
@@ -190,14 +235,20 @@ def workload(name: str, n_tracks: int | None = None, seed: int = 42,
     det = w["detector"]()
     if w.get("field") == "solenoid":
         field = FieldMap.solenoid(n=field_map_points or FIELD_MAP_POINTS, n_threads=n_threads)
+    elif w.get("field") == "zero":
+        field = ConstantField(0.0, 0.0, 0.0)
     else:
         field = ConstantField()
     n = w["n_tracks"] if n_tracks is None else n_tracks
     cfg = gen_config(n, seed, pt=w["pt"], pt_log=w.get("pt_log", False), eta=w["eta"],
+                     phi=w.get("phi", (-np.pi, np.pi)),
                      hole_probability=w.get("hole_probability", 0.0),
                      outlier_fraction=w.get("outlier_fraction", 0.0),
                      min_measurements=w.get("min_measurements", 1), n_threads=n_threads)
-    return det, field, generate(det, field, cfg)
+    ev = generate(det, field, cfg)
+    if det.kinds is not None:
+        ev = apply_measurement_kinds(ev, det, seed)
+    return det, field, ev
 
 
 def workload_options(name: str, **overrides):

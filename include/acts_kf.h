@@ -33,7 +33,9 @@
 extern "C" {
 #endif
 
-#define ACTS_KF_API_VERSION 1
+/* 2: acts_kf_batch_in gained meas_dims and meas_indices (measurement
+ * subspaces); callers that value-initialise the struct are unaffected */
+#define ACTS_KF_API_VERSION 2
 
 /* ---- API return codes (the call itself) -------------------------------- */
 #define ACTS_KF_OK 0

@@ -77,6 +77,9 @@ def _bind(lib):
                                              dp, C.POINTER(capi.TrackStatesOut)]
     lib.acts_oracle_gain_update.restype = C.c_int
     lib.acts_oracle_gain_update.argtypes = [dp, dp, dp, dp, C.c_int32, dp, dp, dp]
+    lib.acts_oracle_gain_update_sub.restype = C.c_int
+    lib.acts_oracle_gain_update_sub.argtypes = [dp, dp, C.c_int32, C.POINTER(C.c_int32), dp, dp,
+                                                C.c_int32, dp, dp, dp]
     lib.acts_oracle_smooth.restype = C.c_int
     lib.acts_oracle_smooth.argtypes = [C.c_int32, dp, dp, dp, dp, dp, dp, dp]
     lib.acts_oracle_interaction.restype = C.c_float
@@ -172,6 +175,24 @@ def gain_update(pred, pred_cov, meas, meas_cov, joseph=False):
     chi2 = np.zeros(1)
     rc = lib.acts_oracle_gain_update(_dp(pred), _dp(pred_cov), _dp(meas), _dp(meas_cov),
                                      int(joseph), _dp(filt), _dp(fcov), _dp(chi2))
+    return rc, filt, fcov.reshape(6, 6), float(chi2[0])
+
+
+def gain_update_sub(pred, pred_cov, idx, meas, meas_cov, joseph=False):
+    """The updater for a measurement of dimension len(idx) on the bound
+    parameters idx (meas_cov: len(idx) x len(idx))."""
+    lib = load()
+    dim = len(idx)
+    pred = np.ascontiguousarray(pred, np.float64)
+    pred_cov = np.ascontiguousarray(pred_cov, np.float64).reshape(36)
+    ix = (C.c_int32 * 2)(*(list(idx) + [0] * (2 - dim)))
+    m = np.ascontiguousarray(meas, np.float64).reshape(dim)
+    V = np.ascontiguousarray(meas_cov, np.float64).reshape(dim * dim)
+    filt = np.zeros(6)
+    fcov = np.zeros(36)
+    chi2 = np.zeros(1)
+    rc = lib.acts_oracle_gain_update_sub(_dp(pred), _dp(pred_cov), dim, ix, _dp(m), _dp(V),
+                                         int(joseph), _dp(filt), _dp(fcov), _dp(chi2))
     return rc, filt, fcov.reshape(6, 6), float(chi2[0])
 
 

@@ -230,6 +230,35 @@ int acts_oracle_gain_update(const double* predicted, const double* predicted_cov
   return 0;
 }
 
+// The same for a measurement of dimension dim (1 or 2) on the bound
+// parameters idx[0..dim-1]: values meas[0..dim-1], covariance meas_cov
+// (dim x dim row-major in its first entries).
+int acts_oracle_gain_update_sub(const double* predicted, const double* predicted_cov,
+                                int32_t dim, const int32_t* idx, const double* meas,
+                                const double* meas_cov, int32_t use_joseph,
+                                double* filtered, double* filtered_cov, double* chi2) {
+  if (dim != 1 && dim != 2) return ACTS_KF_ERR_INVALID_ARGUMENT;
+  Vec6 x, f;
+  M6 P, C;
+  for (int i = 0; i < 6; ++i) x.a[i][0] = predicted[i];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) P.a[i][j] = predicted_cov[6 * i + j];
+  Meas m;
+  m.dim = dim;
+  for (int i = 0; i < dim; ++i) {
+    m.idx[i] = idx[i];
+    m.v[i] = meas[i];
+  }
+  for (int i = 0; i < dim * dim; ++i) m.V[i] = meas_cov[i];
+  double c2 = 0;
+  int r = gainMatrixUpdate(x, P, m, use_joseph != 0, f, C, c2);
+  if (r) return r;
+  store(filtered, f);
+  store(filtered_cov, C);
+  *chi2 = c2;
+  return 0;
+}
+
 // GainMatrixSmoother over n states (arrays of n entries, index 0 first);
 // the last state is the tip (smoothed = filtered).
 int acts_oracle_smooth(int32_t n, const double* predicted, const double* predicted_cov,

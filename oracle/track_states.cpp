@@ -76,11 +76,11 @@ Mat<R, C> sub(const Mat<R, C>& A, const Mat<R, C>& B) {
   return o;
 }
 
-// H: the 2x6 projector of a (loc0, loc1) measurement
-Mat<2, 6> projector() {
-  Mat<2, 6> H = Mat<2, 6>::zero();
-  H.a[0][0] = 1;
-  H.a[1][1] = 1;
+// H: the N x 6 projector of a measurement on the bound parameters idx
+template <int N>
+Mat<N, 6> projector(const int idx[2]) {
+  Mat<N, 6> H = Mat<N, 6>::zero();
+  for (int i = 0; i < N; ++i) H.a[i][idx[i]] = 1;
   return H;
 }
 
@@ -97,14 +97,68 @@ Mat<2, 2> inverse2(const Mat<2, 2>& m) {
 }
 
 // TrackHelpersUnbiased.cpp:37-52 for a 2D measurement
-void unbiased(const Vec6& s, const M6& C, const Mat<2, 1>& m, const Mat<2, 2>& V, Vec6& su,
-              M6& Cu) {
-  const Mat<2, 6> H = projector();
-  const Mat<6, 2> Ht = transpose(H);
+inline Mat<1, 1> inverseN(const Mat<1, 1>& m) {
+  Mat<1, 1> r;
+  r.a[0][0] = 1.0 / m.a[0][0];
+  return r;
+}
+inline Mat<2, 2> inverseN(const Mat<2, 2>& m) { return inverse2(m); }
+
+// calculateUnbiasedParametersCovariance (TrackHelpersUnbiased.cpp:37-52)
+template <int N>
+void unbiased(const Vec6& s, const M6& C, const int idx[2], const Mat<N, 1>& m,
+              const Mat<N, N>& V, Vec6& su, M6& Cu) {
+  const Mat<N, 6> H = projector<N>(idx);
+  const Mat<6, N> Ht = transpose(H);
   // the products keep the projector's zero terms, as Eigen's do
-  const Mat<6, 2> K = mul(mul(C, Ht), inverse2(sub(mul(mul(H, C), Ht), V)));
+  const Mat<6, N> K = mul(mul(C, Ht), inverseN(sub(mul(mul(H, C), Ht), V)));
   su = add(s, mul(K, sub(m, mul(H, s))));
   Cu = sub(C, mul(mul(K, H), C));
+}
+
+// the measurement of a state (dimension N = ms.dim): values and covariance
+template <int N>
+void measMats(const Meas& ms, Mat<N, 1>& m, Mat<N, N>& V) {
+  for (int i = 0; i < N; ++i) m.a[i][0] = ms.v[i];
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) V.a[i][j] = ms.V[N * i + j];
+}
+
+// local hit residual against the prediction (RootTrackStatesWriter.cpp:
+// 677-722): res = m - H x, resCov = V + H C H^T; the x columns take the
+// first measured component, the y columns the second (NaN for N = 1)
+template <int N>
+void hitResiduals(const Row& row, const Meas& ms, const Vec6& x, const M6& C) {
+  Mat<N, 1> m;
+  Mat<N, N> V;
+  measMats<N>(ms, m, V);
+  const Mat<N, 6> H = projector<N>(ms.idx);
+  const Mat<N, N> resCov = add(V, mul(mul(H, C), transpose(H)));
+  const Mat<N, 1> res = sub(m, mul(H, x));
+  row.put(ACTS_KF_TS_RES_X_HIT + 0, res.a[0][0]);
+  row.put(ACTS_KF_TS_RES_X_HIT + 2, V.a[0][0] >= 0 ? std::sqrt(V.a[0][0]) : kNaN);
+  row.put(ACTS_KF_TS_RES_X_HIT + 4,
+          resCov.a[0][0] > 0 ? res.a[0][0] / std::sqrt(resCov.a[0][0]) : kNaN);
+  if (N >= 2) {
+    row.put(ACTS_KF_TS_RES_X_HIT + 1, res.a[N - 1][0]);
+    row.put(ACTS_KF_TS_RES_X_HIT + 3,
+            V.a[N - 1][N - 1] >= 0 ? std::sqrt(V.a[N - 1][N - 1]) : kNaN);
+    row.put(ACTS_KF_TS_RES_X_HIT + 5, resCov.a[N - 1][N - 1] > 0
+                                          ? res.a[N - 1][0] / std::sqrt(resCov.a[N - 1][N - 1])
+                                          : kNaN);
+  } else {
+    row.put(ACTS_KF_TS_RES_X_HIT + 1, kNaN);
+    row.put(ACTS_KF_TS_RES_X_HIT + 3, kNaN);
+    row.put(ACTS_KF_TS_RES_X_HIT + 5, kNaN);
+  }
+}
+
+template <int N>
+void unbiasedOf(const Meas& ms, const Vec6& s, const M6& C, Vec6& su, M6& Cu) {
+  Mat<N, 1> m;
+  Mat<N, N> V;
+  measMats<N>(ms, m, V);
+  unbiased<N>(s, C, ms.idx, m, V, su, Cu);
 }
 
 // the values, errors, global position and momentum, residuals and pulls of
@@ -208,8 +262,7 @@ void trackStatesTable(const Geometry& geo, const acts_kf_batch_in& in,
     // particle's bound state on the surface stands for the averaged sim hits
     Vec6 truth;
     const bool haveTruth = hasSourceLink && truthMeas != nullptr;
-    Mat<2, 1> meas = Mat<2, 1>::zero();
-    Mat<2, 2> measCov = Mat<2, 2>::zero();
+    Meas ms;
     if (haveTruth) {
       truth = loadVec(truthMeas + 6 * static_cast<int64_t>(mi));
       const Vec8 tf = boundToFree(surface, truth);
@@ -227,14 +280,22 @@ void trackStatesTable(const Geometry& geo, const acts_kf_batch_in& in,
     }
     if (hasSourceLink) {
       const int64_t g = mi;
-      meas.a[0][0] = in.meas_values[2 * g];
-      meas.a[1][0] = in.meas_values[2 * g + 1];
-      for (int i = 0; i < 2; ++i)
-        for (int j = 0; j < 2; ++j) measCov.a[i][j] = in.meas_cov[4 * g + 2 * i + j];
+      ms.dim = in.meas_dims ? in.meas_dims[g] : 2;
+      if (in.meas_indices) {
+        ms.idx[0] = in.meas_indices[2 * g];
+        ms.idx[1] = in.meas_indices[2 * g + 1];
+      }
+      for (int i = 0; i < 2; ++i) ms.v[i] = in.meas_values[2 * g + i];
+      for (int i = 0; i < 4; ++i) ms.V[i] = in.meas_cov[4 * g + i];
+      // the measurement expanded into the bound space: its loc0 and loc1
+      // (0 where not measured)
+      double loc[2] = {0.0, 0.0};
+      for (int i = 0; i < ms.dim; ++i)
+        if (ms.idx[i] < 2) loc[ms.idx[i]] = ms.v[i];
       // the plane, disc and cylinder localToGlobal ignore the direction
-      const V3 global = surface.localToGlobal(meas.a[0][0], meas.a[1][0]);
-      row.put(ACTS_KF_TS_L_X_HIT + 0, meas.a[0][0]);
-      row.put(ACTS_KF_TS_L_X_HIT + 1, meas.a[1][0]);
+      const V3 global = surface.localToGlobal(loc[0], loc[1]);
+      row.put(ACTS_KF_TS_L_X_HIT + 0, loc[0]);
+      row.put(ACTS_KF_TS_L_X_HIT + 1, loc[1]);
       row.put(ACTS_KF_TS_L_X_HIT + 2, global[0]);
       row.put(ACTS_KF_TS_L_X_HIT + 3, global[1]);
       row.put(ACTS_KF_TS_L_X_HIT + 4, global[2]);
@@ -253,21 +314,10 @@ void trackStatesTable(const Geometry& geo, const acts_kf_batch_in& in,
       if (p == ACTS_KF_TS_PREDICTED) {
         // local hit residual (RootTrackStatesWriter.cpp:677-722)
         if (hasSourceLink) {
-          const Mat<2, 6> H = projector();
-          const Mat<2, 2> resCov = add(measCov, mul(mul(H, C), transpose(H)));
-          const Mat<2, 1> res = sub(meas, mul(H, x));
-          const double errX = measCov.a[0][0] >= 0 ? std::sqrt(measCov.a[0][0]) : kNaN;
-          const double pullX =
-              resCov.a[0][0] > 0 ? res.a[0][0] / std::sqrt(resCov.a[0][0]) : kNaN;
-          const double errY = measCov.a[1][1] >= 0 ? std::sqrt(measCov.a[1][1]) : kNaN;
-          const double pullY =
-              resCov.a[1][1] > 0 ? res.a[1][0] / std::sqrt(resCov.a[1][1]) : kNaN;
-          row.put(ACTS_KF_TS_RES_X_HIT + 0, res.a[0][0]);
-          row.put(ACTS_KF_TS_RES_X_HIT + 1, res.a[1][0]);
-          row.put(ACTS_KF_TS_RES_X_HIT + 2, errX);
-          row.put(ACTS_KF_TS_RES_X_HIT + 3, errY);
-          row.put(ACTS_KF_TS_RES_X_HIT + 4, pullX);
-          row.put(ACTS_KF_TS_RES_X_HIT + 5, pullY);
+          if (ms.dim == 1)
+            hitResiduals<1>(row, ms, x, C);
+          else
+            hitResiduals<2>(row, ms, x, C);
         } else {
           for (int c = ACTS_KF_TS_RES_X_HIT; c < ACTS_KF_TS_PARAMS(0); ++c) row.put(c, kNaN);
         }
@@ -281,14 +331,17 @@ void trackStatesTable(const Geometry& geo, const acts_kf_batch_in& in,
     if (hasSourceLink) {
       Vec6 su;
       M6 Cu;
-      unbiased(smoothed, smoothedCov, meas, measCov, su, Cu);
+      if (ms.dim == 1)
+        unbiasedOf<1>(ms, smoothed, smoothedCov, su, Cu);
+      else
+        unbiasedOf<2>(ms, smoothed, smoothedCov, su, Cu);
       fillParams(row, ACTS_KF_TS_PARAMS(ACTS_KF_TS_UNBIASED), surface, su, Cu,
                  haveTruth ? &truth : nullptr);
     } else {
       for (int c = ACTS_KF_TS_PARAMS(ACTS_KF_TS_UNBIASED); c < ACTS_KF_TS_NFLOAT; ++c)
         row.put(c, kNaN);
     }
-    row.puti(ACTS_KF_TS_DIM_HIT, hasSourceLink ? 2 : 0);
+    row.puti(ACTS_KF_TS_DIM_HIT, hasSourceLink ? ms.dim : 0);
     row.puti(ACTS_KF_TS_HAS_PARAMS, hasSourceLink ? 0xF : 0x7);
   }
 }

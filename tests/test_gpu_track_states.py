@@ -8,8 +8,8 @@ within 2 float ulps (the columns are doubles rounded to float; the two
 math libraries' sin/cos/asinh/hypot may differ in the last double bit,
 which moves a float by at most one ulp, and only where the double lies
 next to a float rounding boundary).  Workloads: the endcap (uniform
-field), the solenoid field map and the ragged detector with holes and
-outliers; without truth the truth-dependent columns are NaN and the
+field), the solenoid field map, the ragged detector with holes and
+outliers, and the plane telescope with 1D strip measurements; without truth the truth-dependent columns are NaN and the
 others unchanged.
 """
 import numpy as np
@@ -59,7 +59,7 @@ def _compare(got: TS.TrackStatesTable, ref: TS.TrackStatesTable) -> dict:
 
 
 @pytest.mark.parametrize("workload,n,seed", [("endcap", 4000, 61), ("solenoid", 2000, 62),
-                                             ("ragged", 3000, 63)])
+                                             ("ragged", 3000, 63), ("telescope", 3000, 65)])
 def test_track_states_parity(oracle_mod, workload, n, seed):
     print("track states", workload, n, seed, flush=True)
     det, field, opts, ev, fitter, din, dres = _device_fit(workload, n, seed)

@@ -45,6 +45,11 @@ def ragged(oracle_mod):
     return _fit(oracle_mod, "ragged", 600, 72)
 
 
+@pytest.fixture(scope="module")
+def telescope(oracle_mod):
+    return _fit(oracle_mod, "telescope", 800, 73)
+
+
 def _close(a, b, rel=FLOAT_REL, abs_=0.0):
     return np.abs(a - b) <= rel * np.maximum(np.abs(a), np.abs(b)) + abs_
 
@@ -74,8 +79,10 @@ def test_state_types_count_the_fit(ragged):
     assert np.all(np.isnan(tab.column("eLOC0_ubs")[valid & ~link]))
 
 
-def test_unbiased_at_the_tip_is_the_prediction(barrel):
-    _, ev, r, tab = barrel
+@pytest.mark.parametrize("which", ["barrel", "telescope"])
+def test_unbiased_at_the_tip_is_the_prediction(request, which):
+    """On the telescope the tip is a 1D strip measurement of loc1."""
+    _, ev, r, tab = request.getfixturevalue(which)
     ok = np.nonzero(r.status == 0)[0]
     tip = ev.seq_offsets[ok] + r.n_states[ok] - 1
     assert np.all(tab.column("stateType")[tip] == TS.MEASUREMENT)
@@ -237,3 +244,41 @@ def test_parquet_round_trip(barrel, tmp_path):
     first = t.column("eLOC0_smt")[0].as_py()
     assert len(first) == r.n_states[np.nonzero(r.status == 0)[0][0]]
     assert math.isclose(first[0], float(tr["eLOC0_smt"][0][0]))
+
+
+def test_1d_strips_in_the_table(telescope):
+    """Strip rows of the telescope: the hit dimension is 1; the hit's local
+    position is the measured component with 0 for the other; the x hit
+    columns describe the measured component (the writer's res[0]), the y
+    columns are NaN; the unbiased variance and estimate of the measured
+    parameter are the 1D information difference of the smoothed state and
+    the measurement."""
+    _, ev, r, tab = telescope
+    rows = np.nonzero(tab.column("dim_hit") == 1)[0]
+    assert len(rows) == 4 * int(np.count_nonzero(r.status == 0))
+    mi = r.state_measurement[rows]
+    assert np.all(ev.meas_dims[mi] == 1)
+    a = ev.meas_indices[mi, 0].astype(int)
+    m = ev.meas_values[mi, 0]
+    V = ev.meas_cov[mi, 0]
+    lx, ly = tab.column("l_x_hit")[rows], tab.column("l_y_hit")[rows]
+    assert np.all(np.where(a == 0, lx, ly) == m.astype(np.float32))
+    assert np.all(np.where(a == 0, ly, lx) == 0)
+    for c in ("res_y_hit", "err_y_hit", "pull_y_hit"):
+        assert np.all(np.isnan(tab.column(c)[rows])), c
+    xp = r.predicted[rows, a]
+    assert np.all(_close(tab.column("res_x_hit")[rows], m - xp, 1e-6, 1e-9))
+    assert np.all(_close(tab.column("err_x_hit")[rows], np.sqrt(V), 1e-6))
+    Cs = r.smoothed_cov[rows].reshape(-1, 6, 6)[np.arange(len(rows)), a, a]
+    xs = r.smoothed[rows, a]
+    Cu = 1.0 / (1.0 / Cs - 1.0 / V)
+    xu = Cu * (xs / Cs - m / V)
+    q = np.array(["eLOC0", "eLOC1"])[a]
+    for i in range(2):
+        sel = a == i
+        name = ("eLOC0", "eLOC1")[i]
+        got_err = tab.column(f"err_{name}_ubs")[rows[sel]].astype(np.float64)
+        assert np.all(np.abs(got_err - np.sqrt(Cu[sel])) <= 1e-5 * np.sqrt(Cu[sel])), name
+        got = tab.column(f"{name}_ubs")[rows[sel]].astype(np.float64)
+        assert np.all(np.abs(got - xu[sel]) <= 1e-5 * np.sqrt(Cu[sel]) + 1e-6 * np.abs(xu[sel]))
+    assert len(q) == len(rows)
