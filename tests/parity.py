@@ -132,6 +132,25 @@ def float_qop_flips(got, ref, events) -> np.ndarray:
     return flips
 
 
+def smoothing_spread(rts, mbf, events) -> np.ndarray:
+    """Per track, the largest relative difference (floors as in `compare`)
+    between the smoothed parameters of the oracle's RTS and MBF smoothers on
+    the same forward pass.  The two recursions are equal in exact arithmetic,
+    so the difference measures how well the track's smoothed parameters are
+    determined in double precision at all: ~1e-12 for a converged fit, O(1)
+    for a fit that did not converge (chi2/ndf of 1e6 and more after a
+    residual across the +-pi wrap of loc0 = R phi), whose smoothing inverts
+    predicted covariances the residuals disagree with by orders of
+    magnitude.  Tracks that either fit failed get 0."""
+    so = events.seq_offsets
+    spread = np.zeros(events.n_tracks)
+    both = (rts.status == 0) & (mbf.status == 0) & (rts.n_states == mbf.n_states)
+    for t in np.nonzero(both)[0]:
+        s = slice(so[t], so[t] + rts.n_states[t])
+        spread[t] = float(param_rel_err(mbf.smoothed[s], rts.smoothed[s]).max())
+    return spread
+
+
 def error_report(got, ref, events, max_chi2_ndf: float | None = None) -> dict:
     """Errors of a reduced-precision fit against the FP64 reference fit.
 
