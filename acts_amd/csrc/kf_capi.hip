@@ -373,8 +373,12 @@ struct acts_kf_ctx {
   DevSurface* geo = nullptr;
   DevSlab* slabs = nullptr;
   DevField field{};
+  // field map: the point arrays (freed once the cell copies exist) and the
+  // cell-major copies the kernels read
   double* fieldValues = nullptr;
   float* fieldValuesF = nullptr;
+  double* fieldCells = nullptr;
+  float* fieldCellsF = nullptr;
   bool mixed = false;
   // kernel timing: events before the forward kernel, between the two
   // kernels and after the smoother
@@ -418,6 +422,74 @@ struct acts_kf_ctx {
 };
 
 namespace {
+
+// Cell-major copy of a point-array field map (DevField::cells): one thread
+// per cell gathers its 8 corners (a z pair for each (x, y) corner) and
+// writes them as one record of `Stride` values, the tail zero.
+template <typename T, int Stride>
+__global__ void fieldCellsKernel(const T* __restrict__ v, T* __restrict__ cells, int32_t n0,
+                                 int32_t n1, int32_t n2) {
+  const int64_t cell = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (cell >= static_cast<int64_t>(n0) * n1 * n2) return;
+  const int64_t iz = cell % n2;
+  const int64_t ixy = cell / n2;
+  const int64_t iy = ixy % n1;
+  const int64_t ix = ixy / n1;
+  const int64_t sy = n2 + 1;
+  const int64_t sx = static_cast<int64_t>(n1 + 1) * (n2 + 1);
+  T* out = cells + static_cast<int64_t>(Stride) * cell;
+  for (int i = 0; i < 4; ++i)
+    for (int dz = 0; dz < 2; ++dz)
+      for (int k = 0; k < 3; ++k)
+        out[6 * i + 3 * dz + k] = v[3 * ((ix + (i >> 1)) * sx + (iy + (i & 1)) * sy + iz + dz) + k];
+  for (int j = kCellValues; j < Stride; ++j) out[j] = T(0);
+}
+
+// Field maps above this size stay in the point layout on the device.
+constexpr size_t kMaxCellBytes = size_t{32} << 30;
+
+// Uploads the padded point array of a field map (double, or float for the
+// mixed precision) and expands it into the cell-major layout the kernels
+// read; if the cell copy does not fit, the kernels read the point array.
+template <typename T, int Stride>
+int uploadPoints(acts_kf_ctx* c, const std::vector<double>& padded, T** pts, T** cells) {
+  std::vector<T> hv(padded.begin(), padded.end());
+  if (hipMalloc(pts, sizeof(T) * hv.size()) != hipSuccess)
+    return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "hipMalloc field map failed");
+  if (hipMemcpy(*pts, hv.data(), sizeof(T) * hv.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(ACTS_KF_ERR_DEVICE, "hipMemcpy field map failed");
+  const DevField& F = c->field;
+  const int64_t nCells = static_cast<int64_t>(F.nb[0]) * F.nb[1] * F.nb[2];
+  const size_t cellBytes = sizeof(T) * Stride * static_cast<size_t>(nCells);
+  if (cellBytes > kMaxCellBytes) return ACTS_KF_OK;
+  if (hipMalloc(cells, cellBytes) != hipSuccess) {
+    (void)hipGetLastError();  // not fatal: the point layout is used
+    *cells = nullptr;
+    return ACTS_KF_OK;
+  }
+  const unsigned blocks = static_cast<unsigned>((nCells + 255) / 256);
+  hipLaunchKernelGGL((fieldCellsKernel<T, Stride>), dim3(blocks), dim3(256), 0, c->stream, *pts,
+                     *cells, F.nb[0], F.nb[1], F.nb[2]);
+  KF_HIP(hipGetLastError());
+  KF_HIP(hipStreamSynchronize(c->stream));
+  (void)hipFree(*pts);
+  *pts = nullptr;
+  return ACTS_KF_OK;
+}
+
+int uploadFieldMap(acts_kf_ctx* c, const std::vector<double>& padded) {
+  if (!c->mixed) {
+    const int r = uploadPoints<double, kCellDoubles>(c, padded, &c->fieldValues, &c->fieldCells);
+    c->field.v = c->fieldValues;
+    c->field.cells = c->fieldCells;
+    return r;
+  }
+  // the mixed-precision fit reads an FP32 copy of the map
+  const int r = uploadPoints<float, kCellFloats>(c, padded, &c->fieldValuesF, &c->fieldCellsF);
+  c->field.vf = c->fieldValuesF;
+  c->field.cellsF = c->fieldCellsF;
+  return r;
+}
 
 using ForwardFn = void (*)(FwdArgs, int32_t*, int32_t, int32_t, int32_t, const int32_t*);
 
@@ -598,27 +670,17 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
     std::vector<double> padded;
     if (!buildGridField(*field, c->field, padded))
       return cleanup(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: bad field grid");
-    if (!c->mixed) {
-      if (hipMalloc(&c->fieldValues, sizeof(double) * padded.size()) != hipSuccess)
-        return cleanup(ACTS_KF_ERR_OUT_OF_MEMORY, "hipMalloc field map failed");
-      if (hipMemcpy(c->fieldValues, padded.data(), sizeof(double) * padded.size(),
-                    hipMemcpyHostToDevice) != hipSuccess)
-        return cleanup(ACTS_KF_ERR_DEVICE, "hipMemcpy field map failed");
-      c->field.v = c->fieldValues;
-    } else {
-      // the mixed-precision fit reads an FP32 copy of the map
-      std::vector<float> pf(padded.begin(), padded.end());
+    if (c->mixed) {
       for (int a = 0; a < 3; ++a) {
         c->field.minF[a] = static_cast<float>(c->field.min[a]);
         c->field.widthF[a] = static_cast<float>(c->field.width[a]);
         c->field.upperF[a] = static_cast<float>(c->field.upper[a]);
       }
-      if (hipMalloc(&c->fieldValuesF, sizeof(float) * pf.size()) != hipSuccess)
-        return cleanup(ACTS_KF_ERR_OUT_OF_MEMORY, "hipMalloc field map failed");
-      if (hipMemcpy(c->fieldValuesF, pf.data(), sizeof(float) * pf.size(),
-                    hipMemcpyHostToDevice) != hipSuccess)
-        return cleanup(ACTS_KF_ERR_DEVICE, "hipMemcpy field map failed");
-      c->field.vf = c->fieldValuesF;
+    }
+    const int r = uploadFieldMap(c, padded);
+    if (r) {
+      acts_kf_destroy(c);
+      return r;
     }
   }
   DevOpts& o = c->opts;
@@ -1173,6 +1235,8 @@ void acts_kf_destroy(acts_kf_ctx* c) {
   if (c->slabs) (void)hipFree(c->slabs);
   if (c->fieldValues) (void)hipFree(c->fieldValues);
   if (c->fieldValuesF) (void)hipFree(c->fieldValuesF);
+  if (c->fieldCells) (void)hipFree(c->fieldCells);
+  if (c->fieldCellsF) (void)hipFree(c->fieldCellsF);
   if (c->graphExec) (void)hipGraphExecDestroy(c->graphExec);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);

@@ -239,6 +239,18 @@ struct DevField {
   float minF[3], widthF[3], upperF[3];
   int32_t pad3;
   const float* __restrict__ vf;
+  // Cell-major copies of the map on the device (null: use the point arrays
+  // above): cell (ix, iy, iz), ix < nb0, iy < nb1, iz < nb2, at
+  // (ix * nb1 + iy) * nb2 + iz holds the values of its 8 corner points in
+  // the order the interpolation reads them: corner (dx, dy) = (i >> 1, i & 1)
+  // for i = 0..3, then dz = 0, 1, then the 3 components.  A lookup reads one
+  // contiguous record (kCellDoubles doubles, kCellFloats floats with 8 of
+  // padding) instead of four point pairs in different cache lines.
+  const double* __restrict__ cells;
+  const float* __restrict__ cellsF;
 };
+constexpr int kCellValues = 24;
+constexpr int kCellDoubles = 24;  // 192 B: two 128-byte cache lines
+constexpr int kCellFloats = 32;   // 128 B: one cache line
 
 }  // namespace kf

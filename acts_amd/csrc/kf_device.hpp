@@ -294,10 +294,9 @@ __device__ inline void resetTransport(Track& T) {
 // Grid.hpp:470-494, Interpolation.hpp:96-120): the cell of the point's bin,
 // the last axis collapsed first with (1 - f) * a + f * b.  Returns 0 or
 // FIELD_OUT_OF_BOUNDS.
-KF_HD int gridField(const DevField& F, const V3& p, V3& out) {
+// The cell of p and its interpolation weights; 0 or FIELD_OUT_OF_BOUNDS.
+KF_HD int gridLocate(const DevField& F, const V3& p, int lo[3], double f[3]) {
   const double x[3] = {p.x, p.y, p.z};
-  int lo[3];
-  double f[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     if (!(x[a] >= F.min[a]) || x[a] >= F.upper[a]) return ACTS_KF_STATUS_FIELD_OUT_OF_BOUNDS;
@@ -309,16 +308,34 @@ KF_HD int gridField(const DevField& F, const V3& p, V3& out) {
     const double ur = F.min[a] + static_cast<double>(bin) * F.width[a];
     f[a] = (x[a] - ll) / (ur - ll);
   }
-  const int64_t sy = F.nb[2] + 1;
-  const int64_t sx = static_cast<int64_t>(F.nb[1] + 1) * (F.nb[2] + 1);
-  const double* base = F.v + 3 * (lo[0] * sx + lo[1] * sy + lo[2]);
-  // corners (x, y) in canonical order, each with its z pair contiguous
+  return 0;
+}
+
+KF_HD int gridField(const DevField& F, const V3& p, V3& out) {
+  int lo[3];
+  double f[3];
+  if (const int st = gridLocate(F, p, lo, f)) return st;
+  // corners (x, y) in canonical order, each with its z pair: 6 values from
+  // the cell record, or from two neighbouring points of the point array
   double c[4][3];
+  if (F.cells) {
+    const int64_t cell = (static_cast<int64_t>(lo[0]) * F.nb[1] + lo[1]) * F.nb[2] + lo[2];
+    const double* q = static_cast<const double*>(
+        __builtin_assume_aligned(F.cells + kCellDoubles * cell, 64));
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const double* q = base + 3 * (((i >> 1) & 1) * sx + (i & 1) * sy);
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int k = 0; k < 3; ++k) c[i][k] = (1 - f[2]) * q[k] + f[2] * q[3 + k];
+      for (int k = 0; k < 3; ++k) c[i][k] = (1 - f[2]) * q[6 * i + k] + f[2] * q[6 * i + 3 + k];
+  } else {
+    const int64_t sy = F.nb[2] + 1;
+    const int64_t sx = static_cast<int64_t>(F.nb[1] + 1) * (F.nb[2] + 1);
+    const double* base = F.v + 3 * (lo[0] * sx + lo[1] * sy + lo[2]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const double* q = base + 3 * (((i >> 1) & 1) * sx + (i & 1) * sy);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) c[i][k] = (1 - f[2]) * q[k] + f[2] * q[3 + k];
+    }
   }
   double d[2][3];
 #pragma unroll

@@ -46,15 +46,27 @@ __device__ inline int gridFieldF(const DevField& F, const F3& p, F3& out) {
     const float ll = mn + static_cast<float>(bin - 1) * w;
     f[a] = (x[a] - ll) / w;
   }
-  const int64_t sy = F.nb[2] + 1;
-  const int64_t sx = static_cast<int64_t>(F.nb[1] + 1) * (F.nb[2] + 1);
-  const float* base = F.vf + 3 * (lo[0] * sx + lo[1] * sy + lo[2]);
   float c[4][3];
+  if (F.cellsF) {
+    // the cell's record: one 128-byte line
+    const int64_t cell = (static_cast<int64_t>(lo[0]) * F.nb[1] + lo[1]) * F.nb[2] + lo[2];
+    const float* q = static_cast<const float*>(
+        __builtin_assume_aligned(F.cellsF + kCellFloats * cell, 128));
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const float* q = base + 3 * (((i >> 1) & 1) * sx + (i & 1) * sy);
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int k = 0; k < 3; ++k) c[i][k] = (1.f - f[2]) * q[k] + f[2] * q[3 + k];
+      for (int k = 0; k < 3; ++k)
+        c[i][k] = (1.f - f[2]) * q[6 * i + k] + f[2] * q[6 * i + 3 + k];
+  } else {
+    const int64_t sy = F.nb[2] + 1;
+    const int64_t sx = static_cast<int64_t>(F.nb[1] + 1) * (F.nb[2] + 1);
+    const float* base = F.vf + 3 * (lo[0] * sx + lo[1] * sy + lo[2]);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float* q = base + 3 * (((i >> 1) & 1) * sx + (i & 1) * sy);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) c[i][k] = (1.f - f[2]) * q[k] + f[2] * q[3 + k];
+    }
   }
   float d[2][3];
 #pragma unroll
