@@ -226,6 +226,8 @@ SIGNATURES = {
                                     C.c_void_p]),
     "acts_kf_fit_batch_device": (C.c_int, [C.c_void_p, C.POINTER(BatchIn),
                                            C.POINTER(BatchOut), C.c_void_p]),
+    "acts_kf_fit_batch_device_noalloc": (C.c_int, [C.c_void_p, C.POINTER(BatchIn),
+                                                   C.POINTER(BatchOut), C.c_void_p]),
     "acts_kf_sync": (C.c_int, [C.c_void_p]),
     "acts_kf_kernel_times": (C.c_int, [C.c_void_p, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]),

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -40,6 +41,16 @@ constexpr int kLdsSurfaces = 128;
 constexpr int kLdsSlabs = 32;
 // the track queue counts in int32 and may overshoot n by one per lane
 constexpr int64_t kMaxBatch = (int64_t{1} << 31) - (int64_t{1} << 24);
+
+// Workspace of acts_kf_fit_batch_device_noalloc: a fixed block of device
+// memory that the library owns on every device it runs on, for callers that
+// cannot allocate.  Every call on a device shares that device's copy (calls
+// that overlap on one device race).  Host code never passes the array's name
+// to a kernel: it uses the device address hipGetSymbolAddress returns for
+// the current device, fetched once per device (staticWorkspace).
+constexpr size_t kStaticWorkspaceBytes = size_t{256} << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticWorkspaceBytes];
+constexpr int kMaxDevices = 64;
 
 #ifndef KF_FORWARD_WAVES_PER_EU
 #define KF_FORWARD_WAVES_PER_EU 1
@@ -419,6 +430,8 @@ struct acts_kf_ctx {
   DevBuf refBuf;
   // staging for the host-buffer entry point
   DevBuf inBuf, outBuf;
+  // device address of the static workspace on this context's device
+  void* staticWs = nullptr;
 };
 
 namespace {
@@ -443,6 +456,27 @@ __global__ void fieldCellsKernel(const T* __restrict__ v, T* __restrict__ cells,
       for (int k = 0; k < 3; ++k)
         out[6 * i + 3 * dz + k] = v[3 * ((ix + (i >> 1)) * sx + (iy + (i & 1)) * sy + iz + dz) + k];
   for (int j = kCellValues; j < Stride; ++j) out[j] = T(0);
+}
+
+// Device address of `scratch` on device `device` (current), fetched once
+// per device: each device holds its own copy of the array.
+std::mutex g_staticWsMutex;
+void* g_staticWs[kMaxDevices] = {};
+
+int staticWorkspace(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices)
+    return fail(ACTS_KF_ERR_UNSUPPORTED, "device index beyond the static workspace table");
+  std::lock_guard<std::mutex> lock(g_staticWsMutex);
+  if (!g_staticWs[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(ACTS_KF_ERR_DEVICE,
+                  std::string("hipGetSymbolAddress(static workspace): ") + hipGetErrorString(e));
+    g_staticWs[device] = p;
+  }
+  *out = g_staticWs[device];
+  return ACTS_KF_OK;
 }
 
 // Field maps above this size stay in the point layout on the device.
@@ -654,6 +688,10 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
     return cleanup(ACTS_KF_ERR_DEVICE, "hipStreamCreate failed");
   if (hipEventCreateWithFlags(&c->done, hipEventDisableTiming) != hipSuccess)
     return cleanup(ACTS_KF_ERR_DEVICE, "hipEventCreate failed");
+  if (const int r = staticWorkspace(device, &c->staticWs)) {
+    acts_kf_destroy(c);
+    return r;
+  }
   if (hipMalloc(&c->geo, sizeof(DevSurface) * nSurfaces) != hipSuccess ||
       hipMalloc(&c->slabs, sizeof(DevSlab) * hm.size()) != hipSuccess)
     return cleanup(ACTS_KF_ERR_OUT_OF_MEMORY, "hipMalloc geometry failed");
@@ -784,44 +822,95 @@ static bool offsetsValid(const int64_t* off, int64_t n) {
   return true;
 }
 
+// Device memory a fit works in besides its inputs and outputs.
+struct Workspace {
+  double* d = nullptr;       // per-state records (DevScratch::d)
+  int32_t* i = nullptr;      // their int parts (DevScratch::i)
+  int32_t* track = nullptr;  // per track: status, tip, first/last measurement
+  double* ref = nullptr;     // with a reference surface: first/last smoothed states
+  int32_t* queue = nullptr;  // the forward kernel's track-queue counter
+  int32_t* perm = nullptr;   // queue order (sorted queues only)
+  int32_t* hist = nullptr;   // per-block key histograms of the sort
+  Lane* ckpt = nullptr;      // transport-skip checkpoints (null: no skipping)
+};
+
+// Sizes of the workspace of a fit of n tracks of at most `states` states.
+struct WorkspaceSizes {
+  int32_t states = 1;
+  bool withRef = false, sorted = false, byDirection = false;
+  int32_t nKeys = 0;
+  int64_t sortBlocks = 0;
+  size_t d = 0, i = 0, track = 0, ref = 0, queue = 0, perm = 0, hist = 0;
+};
+
+WorkspaceSizes workspaceSizes(const acts_kf_ctx* c, int64_t n, int32_t maxLen) {
+  WorkspaceSizes w;
+  w.states = std::max(maxLen, 1);
+  w.d = sizeof(double) * static_cast<size_t>(w.states) * kScratchComps * n;
+  w.i = sizeof(int32_t) * static_cast<size_t>(w.states) * kIScratchComps * n;
+  w.track = sizeof(int32_t) * 4 * n;
+  w.withRef = c->refSurface != ACTS_KF_REFERENCE_NONE;
+  w.ref = w.withRef ? sizeof(double) * kRefComps * static_cast<size_t>(n) : 0;
+  w.queue = sizeof(int32_t);
+  w.sorted = c->queueOrder != 0 && n > kBlock;
+  w.byDirection = c->queueOrder == 2;
+  w.nKeys = w.byDirection ? kDirectionKeys + 1 : std::min(w.states + 2, kMaxKeys);
+  w.sortBlocks = (n + kSortBlock - 1) / kSortBlock;
+  w.perm = w.sorted ? sizeof(int32_t) * n : 0;
+  w.hist = w.sorted ? sizeof(int32_t) * w.nKeys * w.sortBlocks : 0;
+  return w;
+}
+
+// The context's own (grow-only) workspace buffers, grown to `w`.
+int contextWorkspace(acts_kf_ctx* c, const WorkspaceSizes& w, Workspace& ws) {
+  if (w.withRef && c->refBuf.ensure(w.ref))
+    return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "reference scratch allocation failed");
+  if (c->scratchD.ensure(w.d) || c->scratchI.ensure(w.i) || c->scratchT.ensure(w.track) ||
+      c->queueBuf.ensure(w.queue))
+    return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "scratch allocation failed");
+  if (w.sorted && (c->permBuf.ensure(w.perm) || c->histBuf.ensure(w.hist)))
+    return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "permutation allocation failed");
+  ws.d = static_cast<double*>(c->scratchD.p);
+  ws.i = static_cast<int32_t*>(c->scratchI.p);
+  ws.track = static_cast<int32_t*>(c->scratchT.p);
+  ws.ref = w.withRef ? static_cast<double*>(c->refBuf.p) : nullptr;
+  ws.queue = static_cast<int32_t*>(c->queueBuf.p);
+  ws.perm = w.sorted ? static_cast<int32_t*>(c->permBuf.p) : nullptr;
+  ws.hist = w.sorted ? static_cast<int32_t*>(c->histBuf.p) : nullptr;
+  ws.ckpt = c->skipAfter > 0 ? static_cast<Lane*>(c->ckptBuf.p) : nullptr;
+  return ACTS_KF_OK;
+}
+
+// Queues the fit of a batch on `stream` in the workspace `ws` (sized by
+// `w`); with `graph` the launch sequence is captured and replayed while
+// nothing it depends on changes.
 static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
                      const acts_kf_batch_out* out, hipStream_t stream,
-                     int32_t maxLen) {
+                     const WorkspaceSizes& w, const Workspace& ws, bool graph) {
   const int64_t n = in->n_tracks;
   if (n == 0) return ACTS_KF_OK;
   if (n > kMaxBatch) return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "batch too large");
   const int64_t stride = n;
-  const int32_t states = std::max(maxLen, 1);
-  const size_t needD = sizeof(double) * static_cast<size_t>(states) * kScratchComps * n;
-  const size_t needI = sizeof(int32_t) * static_cast<size_t>(states) * kIScratchComps * n;
-  const size_t needT = sizeof(int32_t) * 4 * n;
-  const bool withRef = c->refSurface != ACTS_KF_REFERENCE_NONE;
-  if (withRef && c->refBuf.ensure(sizeof(double) * kRefComps * static_cast<size_t>(n)))
-    return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "reference scratch allocation failed");
-  if (c->scratchD.ensure(needD) || c->scratchI.ensure(needI) || c->scratchT.ensure(needT) ||
-      c->queueBuf.ensure(sizeof(int32_t)))
-    return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "scratch allocation failed");
-  const bool sorted = c->queueOrder != 0 && n > kBlock;
-  const bool byDirection = c->queueOrder == 2;
-  const int32_t nKeys = byDirection ? kDirectionKeys + 1 : std::min(states + 2, kMaxKeys);
-  const int64_t sortBlocks = (n + kSortBlock - 1) / kSortBlock;
-  if (sorted && (c->permBuf.ensure(sizeof(int32_t) * n) ||
-                 c->histBuf.ensure(sizeof(int32_t) * nKeys * sortBlocks)))
-    return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "permutation allocation failed");
+  const int32_t states = w.states;
+  const bool withRef = w.withRef;
+  const bool sorted = w.sorted;
+  const bool byDirection = w.byDirection;
+  const int32_t nKeys = w.nKeys;
+  const int64_t sortBlocks = w.sortBlocks;
   // Replay the captured launch sequence when nothing it depends on changed.
   GraphKey key;
-  if (c->useGraph) {
+  if (graph) {
     std::memset(&key, 0, sizeof(key));
     key.in = *in;
     key.out = *out;
     key.states = states;
     key.stream = stream;
-    key.scratch[0] = c->scratchD.p;
-    key.scratch[1] = c->scratchI.p;
-    key.scratch[2] = c->scratchT.p;
-    key.scratch[3] = c->permBuf.p;
-    key.scratch[4] = c->histBuf.p;
-    key.ref = c->refBuf.p;
+    key.scratch[0] = ws.d;
+    key.scratch[1] = ws.i;
+    key.scratch[2] = ws.track;
+    key.scratch[3] = ws.perm;
+    key.scratch[4] = ws.hist;
+    key.ref = ws.ref;
     if (c->graphExec && std::memcmp(&key, &c->graphKey, sizeof(key)) == 0) {
       KF_HIP(hipGraphLaunch(c->graphExec, stream));
       return ACTS_KF_OK;
@@ -832,7 +921,7 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
     }
     KF_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
   }
-  const bool timing = c->timing && !c->useGraph;
+  const bool timing = c->timing && !graph;
   DevBatch b;
   b.n = n;
   b.startParams = in->start_params;
@@ -871,17 +960,17 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   d.referenceCov = out->reference_cov;
   d.stateMeas = out->state_measurement;
   DevScratch sc;
-  sc.ref = withRef ? static_cast<double*>(c->refBuf.p) : nullptr;
-  sc.d = static_cast<double*>(c->scratchD.p);
-  sc.i = static_cast<int32_t*>(c->scratchI.p);
-  sc.track = static_cast<int32_t*>(c->scratchT.p);
+  sc.ref = withRef ? ws.ref : nullptr;
+  sc.d = ws.d;
+  sc.i = ws.i;
+  sc.track = ws.track;
   sc.stride = stride;
   sc.maxStates = states;
   const int64_t blocks = (n + kBlock - 1) / kBlock;
   int32_t* perm = nullptr;
   if (sorted) {
-    perm = static_cast<int32_t*>(c->permBuf.p);
-    int32_t* hist = static_cast<int32_t*>(c->histBuf.p);
+    perm = ws.perm;
+    int32_t* hist = ws.hist;
     QueueKeyArgs qk;
     qk.startParams = in->start_params;
     qk.byDirection = byDirection ? 1 : 0;
@@ -912,10 +1001,10 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   fa.pc = c->pc;
   fa.nSurfaces = c->nSurfaces;
   fa.nSlabs = c->nSlabs;
-  fa.ckpt = static_cast<Lane*>(c->ckptBuf.p);
-  fa.skipAfter = c->ckptBuf.p ? c->skipAfter : 0;
+  fa.ckpt = ws.ckpt;
+  fa.skipAfter = ws.ckpt ? c->skipAfter : 0;
   fa.pad = 0;
-  int32_t* queue = static_cast<int32_t*>(c->queueBuf.p);
+  int32_t* queue = ws.queue;
   hipLaunchKernelGGL(zeroKernel, dim3(1), dim3(64), 0, stream, queue, 1);
   const int64_t waves = std::min(blocks, c->residentWaves);
   if (timing) KF_HIP(hipEventRecord(c->ev[0], stream));
@@ -944,7 +1033,7 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
                          c->refLine, c->refStrategy, n, in->seq_offsets);
   }
   const hipError_t launchErr = hipGetLastError();
-  if (c->useGraph) {
+  if (graph) {
     hipGraph_t graph = nullptr;
     const hipError_t endErr = hipStreamEndCapture(stream, &graph);
     if (launchErr != hipSuccess || endErr != hipSuccess) {
@@ -995,9 +1084,71 @@ int acts_kf_fit_batch_device(acts_kf_ctx* c, const acts_kf_batch_in* in,
     KF_HIP(hipMemcpyAsync(&maxLen, c->lenBuf.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     KF_HIP(hipStreamSynchronize(s));
   }
-  const int r = launchFit(c, in, out, s, maxLen);
-  if (r) return r;
-  return in->n_tracks > 0 ? markQueued(c, s) : ACTS_KF_OK;
+  if (in->n_tracks == 0) return ACTS_KF_OK;
+  const WorkspaceSizes w = workspaceSizes(c, in->n_tracks, maxLen);
+  Workspace ws;
+  if (const int r = contextWorkspace(c, w, ws)) return r;
+  if (const int r = launchFit(c, in, out, s, w, ws, c->useGraph)) return r;
+  return markQueued(c, s);
+}
+
+int acts_kf_fit_batch_device_noalloc(acts_kf_ctx* c, const acts_kf_batch_in* in,
+                                     const acts_kf_batch_out* out, void* stream) {
+  if (!c || !in || !out) return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "null argument");
+  const int64_t n = in->n_tracks;
+  if (n < 0) return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "negative n_tracks");
+  if (n == 0) return ACTS_KF_OK;
+  if (!in->start_params || !in->start_cov || !in->start_plane || !in->seq_offsets ||
+      !in->seq_surfaces || !in->meas_offsets || !in->meas_surfaces || !in->meas_values ||
+      !in->meas_cov)
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "null input array");
+  if (n > kMaxBatch) return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "batch too large");
+  if (!c->staticWs)
+    return fail(ACTS_KF_ERR_DEVICE, "acts_kf_fit_batch_device_noalloc: no static workspace");
+  KF_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  // sections of the static workspace, 256-byte aligned: the sequence-length
+  // slot, then the fit's workspace
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  char* base = static_cast<char*>(c->staticWs);
+  int32_t maxLen = in->max_sequence_length;
+  if (maxLen <= 0) {
+    int32_t* len = reinterpret_cast<int32_t*>(base);
+    hipLaunchKernelGGL(zeroKernel, dim3(1), dim3(64), 0, s, len, 1);
+    const int64_t nb = (n + 255) / 256;
+    hipLaunchKernelGGL(maxLengthKernel, dim3(static_cast<unsigned>(nb)), dim3(256), 0, s,
+                       in->seq_offsets, n, len);
+    KF_HIP(hipGetLastError());
+    KF_HIP(hipMemcpyAsync(&maxLen, len, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    KF_HIP(hipStreamSynchronize(s));
+  }
+  const WorkspaceSizes w = workspaceSizes(c, n, maxLen);
+  const size_t sizes[] = {w.queue, w.track, w.perm, w.hist, w.ref, w.i, w.d};
+  size_t need = 256;
+  for (size_t b : sizes) need += al(b);
+  if (need > kStaticWorkspaceBytes)
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT,
+                "acts_kf_fit_batch_device_noalloc: the batch needs " + std::to_string(need) +
+                    " bytes of workspace (748 per track and state slot), more than the " +
+                    std::to_string(kStaticWorkspaceBytes) +
+                    " of the static workspace; split it or use acts_kf_fit_batch_device");
+  size_t off = 256;
+  auto take = [&](size_t bytes) {
+    void* p = bytes ? base + off : nullptr;
+    off += al(bytes);
+    return p;
+  };
+  Workspace ws;
+  ws.queue = static_cast<int32_t*>(take(w.queue));
+  ws.track = static_cast<int32_t*>(take(w.track));
+  ws.perm = static_cast<int32_t*>(take(w.perm));
+  ws.hist = static_cast<int32_t*>(take(w.hist));
+  ws.ref = static_cast<double*>(take(w.ref));
+  ws.i = static_cast<int32_t*>(take(w.i));
+  ws.d = static_cast<double*>(take(w.d));
+  ws.ckpt = nullptr;  // no transport skipping: its checkpoints are context memory
+  if (const int r = launchFit(c, in, out, s, w, ws, false)) return r;
+  return markQueued(c, s);
 }
 
 int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
@@ -1130,8 +1281,10 @@ int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
   dout.reference_params = static_cast<double*>(optr[20]);
   dout.reference_cov = static_cast<double*>(optr[21]);
   dout.state_measurement = static_cast<int32_t*>(optr[22]);
-  int r = launchFit(c, &din, &dout, s, maxLen);
-  if (r) return r;
+  const WorkspaceSizes w = workspaceSizes(c, n, maxLen);
+  Workspace ws;
+  if (const int r = contextWorkspace(c, w, ws)) return r;
+  if (const int r = launchFit(c, &din, &dout, s, w, ws, c->useGraph)) return r;
   for (int i = 0; i < nOut; ++i)
     if (outs[i].host && outs[i].bytes)
       KF_HIP(hipMemcpyAsync(outs[i].host, optr[i], outs[i].bytes, hipMemcpyDeviceToHost, s));

@@ -347,6 +347,26 @@ int acts_kf_fit_batch(acts_kf_ctx* ctx, const acts_kf_batch_in* in,
 int acts_kf_fit_batch_device(acts_kf_ctx* ctx, const acts_kf_batch_in* in,
                              const acts_kf_batch_out* out, void* stream);
 
+/* acts_kf_fit_batch_device without any device allocation: no counterpart in
+ * the reference.  For callers that cannot allocate device memory during a
+ * fit (code under hipGraph stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same inputs, outputs, preconditions and results
+ * (bit for bit) as acts_kf_fit_batch_device; the fit's workspace (per-state
+ * records, per-track results, queue, sort buffers, reference-surface
+ * states) comes from a 256 MiB block of device memory that the library
+ * holds on every device it runs on until the process ends.  A batch needs
+ * 748 bytes per track and state slot (n_tracks x max_sequence_length) plus
+ * about 16 (+ 432 with a reference surface) bytes per track; a larger batch
+ * is rejected with ACTS_KF_ERR_INVALID_ARGUMENT before anything is queued.
+ * Every call on a device shares that block: calls that overlap on one
+ * device, from any context or thread, race (serialise them, e.g. on one
+ * stream).  The fit is queued eagerly (options.use_graph is not used) and
+ * without transport skipping (options.sched_transport_skip_steps, which
+ * never changes results).  With max_sequence_length 0 the call waits for
+ * the stream once to read the longest sequence back.                      */
+int acts_kf_fit_batch_device_noalloc(acts_kf_ctx* ctx, const acts_kf_batch_in* in,
+                                     const acts_kf_batch_out* out, void* stream);
+
 /* Wait for the work of the context's last call (fit_batch, fit_batch_device
  * or track_states_device), on whichever stream it was queued; work of other
  * contexts on the device is not waited for.  The reference's fit() returns
