@@ -109,7 +109,8 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
       base = __shfl(base, leader);
       if (phase == kPhaseIdle) {
         const int64_t q = static_cast<int64_t>(base) + __popcll(idle & lanesBelow);
-        phase = (q < n) ? laneInit<Mixed, Sympy>(L, order ? order[q] : q, a, geo) : kPhaseRetired;
+        phase = (q < n) ? laneInit<Mixed, Sympy>(L, q, order ? order[q] : q, a, geo)
+                        : kPhaseRetired;
       }
     }
     if (phase == kPhaseFinished) {
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(kBlock) void smoothKernel(DevBatch in, DevOut out,
   const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool valid = g < in.n;
   const int64_t t = !valid ? 0 : (perm ? perm[g] : g);
-  smoothTrack<Mbf>(t, valid, in, out, sc);
+  smoothTrack<Mbf>(g, t, valid, in, out, sc);
 }
 
 // Extrapolation of every successfully smoothed track to the reference
@@ -170,11 +171,13 @@ __global__ __launch_bounds__(kBlock) void referenceKernel(DevOut out, DevScratch
                                                           const DevSurface* __restrict__ geo,
                                                           DevField F, DevOpts o, LineRef L,
                                                           int32_t strategy, int64_t n,
-                                                          const int64_t* __restrict__ seqOff) {
-  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= n || sc.track[t] != 0) return;
+                                                          const int64_t* __restrict__ seqOff,
+                                                          const int32_t* __restrict__ perm) {
+  const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (q >= n || sc.track[q] != 0) return;
+  const int64_t t = perm ? perm[q] : q;
   double p[6], C[36];
-  const int st = extrapolateTrack<Mixed, Sympy>(t, sc, sc.ref, geo, F, o, L, strategy, p, C);
+  const int st = extrapolateTrack<Mixed, Sympy>(q, sc, sc.ref, geo, F, o, L, strategy, p, C);
   if (st) {
     for (int64_t slot = seqOff[t]; slot < seqOff[t + 1]; ++slot) clearSlot(out, slot);
     if (out.status) out.status[t] = st;
@@ -209,7 +212,7 @@ constexpr double kSeamRisk = 0.05;
 struct QueueKeyArgs {
   const double* startParams;
   int32_t byDirection;  // 1: direction bins instead of the sequence length
-  int32_t pad;
+  int32_t byMomentum;   // 1: sequence length, then momentum bins
   const int64_t* seqOff;
   const int64_t* measOff;
   const int32_t* measSurf;
@@ -233,15 +236,34 @@ __device__ inline int32_t directionKey(const QueueKeyArgs& k, int64_t t) {
   const int qb = bp[4] < 0 ? 0 : 1;
   return (eb * 32 + pb) * 2 + qb;
 }
+// With byMomentum the key is the sequence length (capped at kMomentumLenCap)
+// times kMomentumBins plus a bin of log |p| of the start parameters (0.1 to
+// 1000 GeV): tracks of one length are ordered by momentum, so that the
+// lanes of a wavefront bend alike and take similar numbers of steps between
+// their surfaces.
+constexpr int kMomentumBins = 16;
+constexpr int kMomentumLenCap = 62;
+__device__ inline int32_t momentumKey(const QueueKeyArgs& k, int64_t t, int32_t len) {
+  const double p = 1.0 / fabs(k.startParams[6 * t + 4]);
+  const int pb = min(max(static_cast<int>(floor((log10(p) + 1.0) * 4.0)), 0),
+                     kMomentumBins - 1);
+  return min(len, kMomentumLenCap) * kMomentumBins + pb;
+}
 __device__ inline int32_t queueKey(const QueueKeyArgs& k, int64_t t) {
   for (int64_t m = k.measOff[t]; m < k.measOff[t + 1]; ++m) {
     const int32_t s = k.measSurf[m];
     if (s < 0 || s >= k.nSurfaces || k.geo[s].type != ACTS_KF_SURFACE_CYLINDER) continue;
     const double phi = k.measVal[2 * m] / k.geo[s].b0;
-    if (fabs(phi) > kPi - kSeamRisk) return k.byDirection ? kDirectionKeys : k.maxLen + 1;
+    if (fabs(phi) > kPi - kSeamRisk) {
+      if (k.byDirection) return kDirectionKeys;
+      if (k.byMomentum) return (kMomentumLenCap + 1) * kMomentumBins;
+      return k.maxLen + 1;
+    }
   }
   if (k.byDirection) return directionKey(k, t);
-  return min(static_cast<int32_t>(k.seqOff[t + 1] - k.seqOff[t]), k.maxLen);
+  const int32_t len = min(static_cast<int32_t>(k.seqOff[t + 1] - k.seqOff[t]), k.maxLen);
+  if (k.byMomentum) return momentumKey(k, t, len);
+  return len;
 }
 // Counting sort of the tracks by queue key, largest key first, in three
 // passes without global atomics: per-block histograms (LDS atomics) stored
@@ -769,7 +791,7 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
     e.maxTargetSkipping = d.max_target_skipping;
     e.loopProtection = d.loop_protection;
   }
-  if (options->length_sorted < 0 || options->length_sorted > 2)
+  if (options->length_sorted < 0 || options->length_sorted > 3)
     return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: bad length_sorted (queue order)");
   c->queueOrder = options->length_sorted;
   c->timing = options->kernel_timing != 0;
@@ -837,7 +859,8 @@ struct Workspace {
 // Sizes of the workspace of a fit of n tracks of at most `states` states.
 struct WorkspaceSizes {
   int32_t states = 1;
-  bool withRef = false, sorted = false, byDirection = false;
+  int64_t stride = 0;  // slots per record component: n rounded up to a wavefront
+  bool withRef = false, sorted = false, byDirection = false, byMomentum = false;
   int32_t nKeys = 0;
   int64_t sortBlocks = 0;
   size_t d = 0, i = 0, track = 0, ref = 0, queue = 0, perm = 0, hist = 0;
@@ -846,15 +869,19 @@ struct WorkspaceSizes {
 WorkspaceSizes workspaceSizes(const acts_kf_ctx* c, int64_t n, int32_t maxLen) {
   WorkspaceSizes w;
   w.states = std::max(maxLen, 1);
-  w.d = sizeof(double) * static_cast<size_t>(w.states) * kScratchComps * n;
-  w.i = sizeof(int32_t) * static_cast<size_t>(w.states) * kIScratchComps * n;
-  w.track = sizeof(int32_t) * 4 * n;
+  w.stride = (n + kBlock - 1) / kBlock * kBlock;
+  w.d = sizeof(double) * static_cast<size_t>(w.states) * kScratchComps * w.stride;
+  w.i = sizeof(int32_t) * static_cast<size_t>(w.states) * kIScratchComps * w.stride;
+  w.track = sizeof(int32_t) * 4 * w.stride;
   w.withRef = c->refSurface != ACTS_KF_REFERENCE_NONE;
   w.ref = w.withRef ? sizeof(double) * kRefComps * static_cast<size_t>(n) : 0;
   w.queue = sizeof(int32_t);
   w.sorted = c->queueOrder != 0 && n > kBlock;
   w.byDirection = c->queueOrder == 2;
-  w.nKeys = w.byDirection ? kDirectionKeys + 1 : std::min(w.states + 2, kMaxKeys);
+  w.byMomentum = c->queueOrder == 3;
+  w.nKeys = w.byDirection  ? kDirectionKeys + 1
+            : w.byMomentum ? (kMomentumLenCap + 1) * kMomentumBins + 1
+                           : std::min(w.states + 2, kMaxKeys);
   w.sortBlocks = (n + kSortBlock - 1) / kSortBlock;
   w.perm = w.sorted ? sizeof(int32_t) * n : 0;
   w.hist = w.sorted ? sizeof(int32_t) * w.nKeys * w.sortBlocks : 0;
@@ -890,7 +917,7 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   const int64_t n = in->n_tracks;
   if (n == 0) return ACTS_KF_OK;
   if (n > kMaxBatch) return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "batch too large");
-  const int64_t stride = n;
+  const int64_t stride = w.stride;
   const int32_t states = w.states;
   const bool withRef = w.withRef;
   const bool sorted = w.sorted;
@@ -974,7 +1001,7 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
     QueueKeyArgs qk;
     qk.startParams = in->start_params;
     qk.byDirection = byDirection ? 1 : 0;
-    qk.pad = 0;
+    qk.byMomentum = w.byMomentum ? 1 : 0;
     qk.seqOff = in->seq_offsets;
     qk.measOff = in->meas_offsets;
     qk.measSurf = in->meas_surfaces;
@@ -1018,19 +1045,19 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
     if (c->sympy && c->mixed)
       hipLaunchKernelGGL((referenceKernel<true, true>), dim3(static_cast<unsigned>(blocks)),
                          dim3(kBlock), 0, stream, d, sc, c->geo, c->field, c->extOpts,
-                         c->refLine, c->refStrategy, n, in->seq_offsets);
+                         c->refLine, c->refStrategy, n, in->seq_offsets, perm);
     else if (c->sympy)
       hipLaunchKernelGGL((referenceKernel<false, true>), dim3(static_cast<unsigned>(blocks)),
                          dim3(kBlock), 0, stream, d, sc, c->geo, c->field, c->extOpts,
-                         c->refLine, c->refStrategy, n, in->seq_offsets);
+                         c->refLine, c->refStrategy, n, in->seq_offsets, perm);
     else if (c->mixed)
       hipLaunchKernelGGL(referenceKernel<true>, dim3(static_cast<unsigned>(blocks)), dim3(kBlock),
                          0, stream, d, sc, c->geo, c->field, c->extOpts, c->refLine,
-                         c->refStrategy, n, in->seq_offsets);
+                         c->refStrategy, n, in->seq_offsets, perm);
     else
       hipLaunchKernelGGL(referenceKernel<false>, dim3(static_cast<unsigned>(blocks)),
                          dim3(kBlock), 0, stream, d, sc, c->geo, c->field, c->extOpts,
-                         c->refLine, c->refStrategy, n, in->seq_offsets);
+                         c->refLine, c->refStrategy, n, in->seq_offsets, perm);
   }
   const hipError_t launchErr = hipGetLastError();
   if (graph) {

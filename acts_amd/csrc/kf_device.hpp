@@ -532,9 +532,10 @@ __device__ inline double transportD37(const DevOpts& o, const RkStage& S, double
 // (BR row 3 is (0,0,0,1), D_TR row 3 is (0,0,0,D37)).  dF/dG rows i hold the
 // columns 0..3 of row i.
 __device__ inline void blockEntry(int i, int j, const double* dFrow, const double* dGrow,
-                                  const double* br, double tr, double& ntr, double& nbr) {
-  double pt = (dFrow[0] * br[j] + dFrow[1] * br[4 + j]) + dFrow[2] * br[8 + j];
-  double pb = (dGrow[0] * br[j] + dGrow[1] * br[4 + j]) + dGrow[2] * br[8 + j];
+                                  const double bj[3], double tr, double& ntr, double& nbr) {
+  // bj = column j of BR (rows 0..2)
+  double pt = (dFrow[0] * bj[0] + dFrow[1] * bj[1]) + dFrow[2] * bj[2];
+  double pb = (dGrow[0] * bj[0] + dGrow[1] * bj[1]) + dGrow[2] * bj[2];
   if (j == 3) {
     pt = pt + dFrow[3];
     pb = pb + dGrow[3];
@@ -557,19 +558,20 @@ __device__ inline void rkTransport(Track& T, const DevOpts& o, const RkStage& S,
   V3 d1, d2, d3, d4;
   dkdL(S, d1, d2, d3, d4);
   dFdGLambda(S, d1, d2, d3, d4, col[3]);
-  double ntr[12], nbr[12];
+  // column j of both blocks at a time: the new column depends only on the
+  // old column j of BR, so the blocks are updated in place
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    const double dFrow[4] = {col[0][i], col[1][i], col[2][i], col[3][i]};
-    const double dGrow[4] = {col[0][3 + i], col[1][3 + i], col[2][3 + i], col[3][3 + i]};
+  for (int j = 0; j < 4; ++j) {
+    const double bj[3] = {T.br[j], T.br[4 + j], T.br[8 + j]};
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-      blockEntry(i, j, dFrow, dGrow, T.br, T.tr[4 * i + j], ntr[4 * i + j], nbr[4 * i + j]);
-  }
-#pragma unroll
-  for (int i = 0; i < 12; ++i) {
-    T.tr[i] = ntr[i];
-    T.br[i] = nbr[i];
+    for (int i = 0; i < 3; ++i) {
+      const double dFrow[4] = {col[0][i], col[1][i], col[2][i], col[3][i]};
+      const double dGrow[4] = {col[0][3 + i], col[1][3 + i], col[2][3 + i], col[3][3 + i]};
+      double ntr, nbr;
+      blockEntry(i, j, dFrow, dGrow, bj, T.tr[4 * i + j], ntr, nbr);
+      T.tr[4 * i + j] = ntr;
+      T.br[4 * i + j] = nbr;
+    }
   }
   T.tr37 = T.tr37 + transportD37(o, S, p, dtds);
 }

@@ -17,14 +17,15 @@
 
 namespace kf {
 
-// Scratch record of state k of track t: kScratchComps doubles (and
-// kIScratchComps ints) stored contiguously, so that a lane writes and reads
-// whole cache lines of its own track.
-__device__ inline double& sd(const DevScratch& sc, int k, int c, int64_t t) {
-  return sc.d[(t * sc.maxStates + k) * kScratchComps + c];
+// Component c of the scratch record of state k of the track in queue slot
+// q: component-major with the slots innermost, so that the lanes of a
+// wavefront, which hold consecutive slots, store and load one component of
+// the same state index as one contiguous run.
+__device__ inline double& sd(const DevScratch& sc, int k, int c, int64_t q) {
+  return sc.d[(static_cast<int64_t>(k) * kScratchComps + c) * sc.stride + q];
 }
-__device__ inline int32_t& si(const DevScratch& sc, int k, int c, int64_t t) {
-  return sc.i[(t * sc.maxStates + k) * kIScratchComps + c];
+__device__ inline int32_t& si(const DevScratch& sc, int k, int c, int64_t q) {
+  return sc.i[(static_cast<int64_t>(k) * kIScratchComps + c) * sc.stride + q];
 }
 
 // Phases of a lane of the persistent forward kernel.
@@ -44,7 +45,8 @@ struct SeqAt {
 
 // Everything one track needs between two phases of the forward pass.
 struct Lane {
-  int64_t t;
+  int64_t t;  // track index in the batch
+  int64_t q;  // queue slot (index of the track's scratch records)
   int64_t m0;
   int nMeas, nUnique;
   int mNext;  // measurement after the last one matched (all surfaces distinct)
@@ -89,11 +91,12 @@ struct FwdArgs {
 // initialisation and the pre-propagation act of the KalmanFitter actor.
 // Returns the lane's next phase.
 template <bool Mixed, bool Sympy = false>
-__device__ inline int laneInit(Lane& L, const int64_t t, const FwdArgs& a,
+__device__ inline int laneInit(Lane& L, const int64_t q, const int64_t t, const FwdArgs& a,
                                const DevSurface* geo) {
   const DevBatch& in = a.in;
   const DevOpts& o = a.o;
   L.t = t;
+  L.q = q;
   const int64_t s0 = in.seqOff[t];
   const int seqLen = static_cast<int>(in.seqOff[t + 1] - s0);
   L.m0 = in.measOff[t];
@@ -284,7 +287,7 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
   const DevScratch& sc = a.sc;
   Track& T = L.T;
   const int reached = L.reached;
-  const int64_t t = L.t;
+  const int64_t q = L.q;
   L.reached = -1;
   L.lastSurfIter = L.iter;
   L.replaying = false;
@@ -330,12 +333,12 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
         uint32_t flags = ACTS_KF_STATE_HAS_PARAMETERS;
         if (hasMat) flags |= ACTS_KF_STATE_HAS_MATERIAL;
 #pragma unroll
-        for (int i = 0; i < 6; ++i) sd(sc, k, kPred + i, t) = x[i];
+        for (int i = 0; i < 6; ++i) sd(sc, k, kPred + i, q) = x[i];
 #pragma unroll
-        for (int i = 0; i < 21; ++i) sd(sc, k, kPCov + i, t) = T.cov[i];
+        for (int i = 0; i < 21; ++i) sd(sc, k, kPCov + i, q) = T.cov[i];
 #pragma unroll
-        for (int i = 0; i < 36; ++i) sd(sc, k, kJac + i, t) = J[i];
-        sd(sc, k, kPath, t) = T.pathAcc;
+        for (int i = 0; i < 36; ++i) sd(sc, k, kJac + i, q) = J[i];
+        sd(sc, k, kPath, q) = T.pathAcc;
         float chi2f = 0.f;
         bool measured = false;
         if (mi >= 0) {
@@ -353,9 +356,9 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
               measured = true;
               chi2f = static_cast<float>(chi2);
 #pragma unroll
-              for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, t) = f[i];
+              for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, q) = f[i];
 #pragma unroll
-              for (int i = 0; i < 21; ++i) sd(sc, k, kFCov + i, t) = C[i];
+              for (int i = 0; i < 21; ++i) sd(sc, k, kFCov + i, q) = C[i];
               // EigenStepper::update with the filtered parameters
               const double sinTh = sin(f[3]);
               T.dir = mk(cos(f[2]) * sinTh, sin(f[2]) * sinTh, cos(f[3]));
@@ -383,14 +386,14 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
         if (!L.status) {
           if (!measured) {
 #pragma unroll
-            for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, t) = x[i];
+            for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, q) = x[i];
 #pragma unroll
-            for (int i = 0; i < 21; ++i) sd(sc, k, kFCov + i, t) = T.cov[i];
+            for (int i = 0; i < 21; ++i) sd(sc, k, kFCov + i, q) = T.cov[i];
           }
-          sd(sc, k, kChi2, t) = chi2f;
-          si(sc, k, kISurf, t) = reached;
-          si(sc, k, kIFlags, t) = static_cast<int32_t>(flags);
-          si(sc, k, kIMeas, t) = mi;
+          sd(sc, k, kChi2, q) = chi2f;
+          si(sc, k, kISurf, q) = reached;
+          si(sc, k, kIFlags, q) = static_cast<int32_t>(flags);
+          si(sc, k, kIMeas, q) = mi;
           L.nStates = k + 1;
           if (mi >= 0) L.tip = k;
           if (hasMat) L.status = materialInteraction(T, s, slabs[s.mat], a.pc, o);
@@ -413,10 +416,11 @@ __device__ inline void laneFinish(Lane& L, const FwdArgs& a) {
   if (a.out.nSteps) a.out.nSteps[t] = L.T.nSteps;
   if (a.out.nTrials) a.out.nTrials[t] = L.T.nTrials;
   if (!status && L.nMeasStates == 0) status = ACTS_KF_STATUS_NO_MEASUREMENT_FOUND;
-  a.sc.track[t] = status;
-  a.sc.track[a.sc.stride + t] = L.tip;
-  a.sc.track[2 * a.sc.stride + t] = L.firstMeas;
-  a.sc.track[3 * a.sc.stride + t] = L.lastMeas;
+  const int64_t q = L.q;
+  a.sc.track[q] = status;
+  a.sc.track[a.sc.stride + q] = L.tip;
+  a.sc.track[2 * a.sc.stride + q] = L.firstMeas;
+  a.sc.track[3 * a.sc.stride + q] = L.lastMeas;
 }
 
 // An output slot that holds no state: surface and measurement -1, flags 0,
@@ -515,23 +519,23 @@ __device__ inline void mbfTransport(const double F[36], double A[36]) {
 // (MbfSmoother.cpp:29-48, detail/MbfSmootherImpl.hpp:17-57): through the
 // measurement update if state k is a measurement, then through its
 // transport jacobian F.
-__device__ inline void mbfUpdate(int64_t t, int k, uint32_t flags, const DevBatch& in,
+__device__ inline void mbfUpdate(int64_t q, int64_t t, int k, uint32_t flags, const DevBatch& in,
                                  const DevScratch& sc, double L[36], double l[6]) {
   double F[36];
 #pragma unroll
-  for (int i = 0; i < 36; ++i) F[i] = sd(sc, k, kJac + i, t);
+  for (int i = 0; i < 36; ++i) F[i] = sd(sc, k, kJac + i, q);
   const bool isMeasurement =
       (flags & ACTS_KF_STATE_HAS_MEASUREMENT) && !(flags & ACTS_KF_STATE_IS_OUTLIER);
   MeasDev M;
-  if (isMeasurement) loadMeas(in, in.measOff[t] + si(sc, k, kIMeas, t), M);
+  if (isMeasurement) loadMeas(in, in.measOff[t] + si(sc, k, kIMeas, q), M);
   if (isMeasurement && !isLoc01(M)) {
     // a measurement of another subspace: the same recursion with the
     // projector's indices a (, b) in place of loc0 (, loc1)
     double P[21], xp[6];
 #pragma unroll
-    for (int i = 0; i < 21; ++i) P[i] = sd(sc, k, kPCov + i, t);
+    for (int i = 0; i < 21; ++i) P[i] = sd(sc, k, kPCov + i, q);
 #pragma unroll
-    for (int i = 0; i < 6; ++i) xp[i] = sd(sc, k, kPred + i, t);
+    for (int i = 0; i < 6; ++i) xp[i] = sd(sc, k, kPred + i, q);
     const bool two = M.n == 2;
     double Paa = 0, Pab = 0, Pba = 0, Pbb = 0, xa = 0, xb = 0;
 #pragma unroll
@@ -619,8 +623,8 @@ __device__ inline void mbfUpdate(int64_t t, int k, uint32_t flags, const DevBatc
   } else if (isMeasurement) {
     double P[21];
 #pragma unroll
-    for (int i = 0; i < 21; ++i) P[i] = sd(sc, k, kPCov + i, t);
-    const int64_t mg = in.measOff[t] + si(sc, k, kIMeas, t);
+    for (int i = 0; i < 21; ++i) P[i] = sd(sc, k, kPCov + i, q);
+    const int64_t mg = in.measOff[t] + si(sc, k, kIMeas, q);
     const double m0 = in.measVal[2 * mg], m1 = in.measVal[2 * mg + 1];
     const double* V = in.measCov + 4 * mg;
     // S = H P H^T + V and Eigen's closed-form 2x2 inverse
@@ -639,7 +643,7 @@ __device__ inline void mbfUpdate(int64_t t, int k, uint32_t flags, const DevBatc
         CH[6 * i + j] =
             j < 2 ? id - (P[sym(i, 0)] * Si[0][j] + P[sym(i, 1)] * Si[1][j]) : id;
       }
-    const double y0 = m0 - sd(sc, k, kPred + 0, t), y1 = m1 - sd(sc, k, kPred + 1, t);
+    const double y0 = m0 - sd(sc, k, kPred + 0, q), y1 = m1 - sd(sc, k, kPred + 1, q);
     // Lambda~ = (H^T S^-1) H + (CHat^T Lambda) CHat, by the transport helper
     // with CHat in place of F
     mbfTransport(CH, L);
@@ -676,14 +680,15 @@ __device__ inline void mbfUpdate(int64_t t, int k, uint32_t flags, const DevBatc
 
 // Mbf = false: GainMatrixSmoother (RTS); true: MbfSmoother.
 template <bool Mbf>
-__device__ inline void smoothTrack(const int64_t t, const bool valid, const DevBatch& in,
+__device__ inline void smoothTrack(const int64_t q, const int64_t t, const bool valid,
+                                   const DevBatch& in,
                                    const DevOut& out, const DevScratch& sc) {
-  int status = valid ? sc.track[t] : 0;
-  const int tip = (!valid || status) ? -1 : sc.track[sc.stride + t];
+  int status = valid ? sc.track[q] : 0;
+  const int tip = (!valid || status) ? -1 : sc.track[sc.stride + q];
   // with a reference surface: keep the smoothed first and last measurement
   // states for the extrapolation
-  const int firstMeas = (valid && sc.ref) ? sc.track[2 * sc.stride + t] : -2;
-  const int lastMeas = (valid && sc.ref) ? sc.track[3 * sc.stride + t] : -2;
+  const int firstMeas = (valid && sc.ref) ? sc.track[2 * sc.stride + q] : -2;
+  const int lastMeas = (valid && sc.ref) ? sc.track[3 * sc.stride + q] : -2;
   int nHoles = 0, nOut = 0, nMeasT = 0, ndf = 0;
   float chi2sum = 0.f;
   // All lanes walk the same state index k (from the wavefront's largest tip
@@ -705,9 +710,9 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
     if (!active || k > tip) continue;
     double xf[6], Cf[21], xs[6], Cs[21], CsFull[36];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) xf[i] = sd(sc, k, kFilt + i, t);
+    for (int i = 0; i < 6; ++i) xf[i] = sd(sc, k, kFilt + i, q);
 #pragma unroll
-    for (int i = 0; i < 21; ++i) Cf[i] = sd(sc, k, kFCov + i, t);
+    for (int i = 0; i < 21; ++i) Cf[i] = sd(sc, k, kFCov + i, q);
     if constexpr (Mbf) {
       mbfSmoothed(xf, Cf, L, l, xs, CsFull);
 #pragma unroll
@@ -722,11 +727,11 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
     } else {
       double xpN[6], CpN[21], JN[36];
 #pragma unroll
-      for (int i = 0; i < 6; ++i) xpN[i] = sd(sc, k + 1, kPred + i, t);
+      for (int i = 0; i < 6; ++i) xpN[i] = sd(sc, k + 1, kPred + i, q);
 #pragma unroll
-      for (int i = 0; i < 21; ++i) CpN[i] = sd(sc, k + 1, kPCov + i, t);
+      for (int i = 0; i < 21; ++i) CpN[i] = sd(sc, k + 1, kPCov + i, q);
 #pragma unroll
-      for (int i = 0; i < 36; ++i) JN[i] = sd(sc, k + 1, kJac + i, t);
+      for (int i = 0; i < 36; ++i) JN[i] = sd(sc, k + 1, kJac + i, q);
       const int r = smoothStep(xf, Cf, xpN, CpN, xsN, CsN, JN, xs, Cs);
       if (r) {
         status = r;
@@ -735,7 +740,7 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
       }
     }
     if (k == firstMeas || k == lastMeas) {
-      double* r = sc.ref + kRefComps * t + (k == firstMeas ? 0 : 27);
+      double* r = sc.ref + kRefComps * q + (k == firstMeas ? 0 : 27);
 #pragma unroll
       for (int i = 0; i < 6; ++i) r[i] = xs[i];
 #pragma unroll
@@ -745,8 +750,8 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
         for (int i = 0; i < 27; ++i) r[27 + i] = r[i];
       }
     }
-    const uint32_t flags = static_cast<uint32_t>(si(sc, k, kIFlags, t));
-    const float c2 = static_cast<float>(sd(sc, k, kChi2, t));
+    const uint32_t flags = static_cast<uint32_t>(si(sc, k, kIFlags, q));
+    const float c2 = static_cast<float>(sd(sc, k, kChi2, q));
     if (flags & ACTS_KF_STATE_IS_HOLE) {
       ++nHoles;
     } else if (flags & ACTS_KF_STATE_IS_OUTLIER) {
@@ -754,7 +759,7 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
     } else if (flags & ACTS_KF_STATE_HAS_MEASUREMENT) {
       ++nMeasT;
       chi2sum += c2;
-      ndf += in.measDims ? in.measDims[in.measOff[t] + si(sc, k, kIMeas, t)] : 2;
+      ndf += in.measDims ? in.measDims[in.measOff[t] + si(sc, k, kIMeas, q)] : 2;
     }
     const int64_t slot = base + k;
     if (out.smoothed)
@@ -770,24 +775,24 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
       for (int i = 0; i < 6; ++i) out.filtered[6 * slot + i] = xf[i];
     if (out.filteredCov) storeSym(out.filteredCov + 36 * slot, Cf);
     if (out.predicted)
-      for (int i = 0; i < 6; ++i) out.predicted[6 * slot + i] = sd(sc, k, kPred + i, t);
+      for (int i = 0; i < 6; ++i) out.predicted[6 * slot + i] = sd(sc, k, kPred + i, q);
     if (out.predictedCov) {
       double P[21];
-      for (int i = 0; i < 21; ++i) P[i] = sd(sc, k, kPCov + i, t);
+      for (int i = 0; i < 21; ++i) P[i] = sd(sc, k, kPCov + i, q);
       storeSym(out.predictedCov + 36 * slot, P);
     }
     if (out.jacobian)
-      for (int i = 0; i < 36; ++i) out.jacobian[36 * slot + i] = sd(sc, k, kJac + i, t);
-    if (out.stateSurface) out.stateSurface[slot] = si(sc, k, kISurf, t);
+      for (int i = 0; i < 36; ++i) out.jacobian[36 * slot + i] = sd(sc, k, kJac + i, q);
+    if (out.stateSurface) out.stateSurface[slot] = si(sc, k, kISurf, q);
     if (out.stateFlags) out.stateFlags[slot] = flags;
-    if (out.statePath) out.statePath[slot] = sd(sc, k, kPath, t);
+    if (out.statePath) out.statePath[slot] = sd(sc, k, kPath, q);
     if (out.stateChi2) out.stateChi2[slot] = c2;
     if (out.stateMeas) {
-      const int32_t mi = si(sc, k, kIMeas, t);
+      const int32_t mi = si(sc, k, kIMeas, q);
       out.stateMeas[slot] = mi >= 0 ? static_cast<int32_t>(in.measOff[t] + mi) : -1;
     }
     if constexpr (Mbf) {
-      if (k > 0) mbfUpdate(t, k, flags, in, sc, L, l);
+      if (k > 0) mbfUpdate(q, t, k, flags, in, sc, L, l);
     } else {
 #pragma unroll
       for (int i = 0; i < 6; ++i) xsN[i] = xs[i];
@@ -801,7 +806,7 @@ __device__ inline void smoothTrack(const int64_t t, const bool valid, const DevB
   const int seqLen = static_cast<int>(in.seqOff[t + 1] - base);
   for (int k = status ? 0 : tip + 1; k < seqLen; ++k) clearSlot(out, base + k);
   // the status after smoothing, for the extrapolation kernel
-  sc.track[t] = status;
+  sc.track[q] = status;
   if (out.status) out.status[t] = status;
   if (out.nStates) out.nStates[t] = status ? 0 : tip + 1;
   if (out.chi2) out.chi2[t] = status ? 0.f : chi2sum;

@@ -277,17 +277,17 @@ __device__ inline int propagateToLine(Track& T, const LineRef& L, const DevField
 // smoothed first/last measurement states in `ref`); returns a status and
 // writes the parameters and covariance on the line.
 template <bool Mixed, bool Sympy = false>
-__device__ inline int extrapolateTrack(int64_t t, const DevScratch& sc, const double* ref,
+__device__ inline int extrapolateTrack(int64_t q, const DevScratch& sc, const double* ref,
                                        const DevSurface* geo, const DevField& F,
                                        const DevOpts& o, const LineRef& L, int32_t strategy,
                                        double params[6], double cov[36]) {
-  const int first = sc.track[2 * sc.stride + t];
-  const int last = sc.track[3 * sc.stride + t];
+  const int first = sc.track[2 * sc.stride + q];
+  const int last = sc.track[3 * sc.stride + q];
   if (first < 0) return ACTS_KF_STATUS_NO_COMPATIBLE_STATE;
-  const double* rf = ref + kRefComps * t;
+  const double* rf = ref + kRefComps * q;
   // the straight-line distance from a state's smoothed parameters
   auto distanceFrom = [&](int k, const double* p, double& path) -> bool {
-    const DevSurface& s = geo[si(sc, k, kISurf, t)];
+    const DevSurface& s = geo[si(sc, k, kISurf, q)];
     const double sinTheta = sin(p[3]);
     const V3 dir = mk(cos(p[2]) * sinTheta, sin(p[2]) * sinTheta, cos(p[3]));
     const V3 pos = localToGlobal(s, p[0], p[1]);
@@ -321,7 +321,7 @@ __device__ inline int extrapolateTrack(int64_t t, const DevScratch& sc, const do
   // Direction::fromScalarZeroAsPositive
   const double direction = distance >= 0 ? 1.0 : -1.0;
   Track T;
-  initializeFromBound<Sympy>(T, geo[si(sc, k, kISurf, t)], p, p + 6, o);
+  initializeFromBound<Sympy>(T, geo[si(sc, k, kISurf, q)], p, p + 6, o);
   if (const int st = propagateToLine<Mixed, Sympy>(T, L, F, o, direction)) return st;
   return lineBoundState<Sympy>(T, L, o.surfaceTolerance, params, cov);
 }

@@ -50,7 +50,7 @@ class KalmanFitterOptions:
     reference_strategy: int = capi.EXTRAPOLATE_FIRST_OR_LAST
     reference_position: tuple = (0.0, 0.0, 0.0)
     # scheduling only; results do not depend on these
-    length_sorted: int = 1                  # queue order: 0 input, 1 length, 2 direction
+    length_sorted: int = 1                  # queue order: 0 input, 1 length, 2 direction, 3 length+momentum
     sched_inner_steps: int = 0              # 0 = library default
     sched_wait_64ths: int = 0               # 0 = library default
     sched_quarantine_steps: int = 0         # 0 = off

@@ -203,7 +203,9 @@ typedef struct acts_kf_options {
    *                   decreasing sequence length (default), 0 = input
    *                   order, 2 = by start direction and charge (bins of
    *                   eta, phi and the sign of q/p, so that neighbouring
-   *                   lanes follow neighbouring paths); tracks with a
+   *                   lanes follow neighbouring paths), 3 = by sequence
+   *                   length, then by start momentum (bins of log p, so
+   *                   that neighbouring lanes bend alike); tracks with a
    *                   measurement near the +-pi seam of a cylinder first;
    * sched_inner_steps most RK steps the forward kernel takes before the
    *                   lanes waiting at a surface are updated (0 = 64);
