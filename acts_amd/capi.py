@@ -27,6 +27,8 @@ STATUS_OK = 0
 STATUS_UPDATE_FAILED = CAT_KALMAN | 1
 STATUS_SMOOTH_FAILED = CAT_KALMAN | 2
 STATUS_NO_MEASUREMENT_FOUND = CAT_KALMAN | 4
+STATUS_REVERSE_PROPAGATION_FAILED = CAT_KALMAN | 5
+STATUS_INCONSISTENT_TRACK_STATES = CAT_KALMAN | 6
 STATUS_PROPAGATION_FAILURE = CAT_PROPAGATOR | 1
 STATUS_STEP_COUNT_LIMIT = CAT_PROPAGATOR | 2
 STATUS_NEXT_TARGET_LIMIT = CAT_PROPAGATOR | 3
@@ -138,6 +140,10 @@ class Options(C.Structure):
         ("use_graph", C.c_int32),
         ("sched_transport_skip_steps", C.c_int32),
         ("reserved_sched", C.c_int32),
+        ("reverse_filtering", C.c_int32),
+        ("reserved_reverse", C.c_int32),
+        ("reverse_filtering_cov_scaling", C.c_double),
+        ("reverse_filtering_momentum_threshold", C.c_double),
     ]
 
 
@@ -168,7 +174,7 @@ BATCH_OUT_FIELDS = [
     "n_outliers", "n_steps", "n_step_trials", "state_surface", "state_flags",
     "state_path", "state_chi2", "predicted", "predicted_cov", "filtered",
     "filtered_cov", "smoothed", "smoothed_cov", "jacobian", "reference_params",
-    "reference_cov", "state_measurement",
+    "reference_cov", "state_measurement", "reversed",
 ]
 
 

@@ -589,6 +589,9 @@ void acts_kf_default_options(acts_kf_options* o) {
   o->mass = 0.1056583755f;
   o->abs_charge = 1.0f;
   o->length_sorted = 1;
+  o->reverse_filtering = 0;
+  o->reverse_filtering_cov_scaling = 100.0;
+  o->reverse_filtering_momentum_threshold = 0.0;
 }
 
 const char* acts_kf_last_error(void) { return g_lastError.c_str(); }

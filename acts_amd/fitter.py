@@ -58,6 +58,13 @@ class KalmanFitterOptions:
     use_graph: bool = False                 # replay fits as a captured hipGraph
     sched_transport_skip_steps: int = 0     # 0 = off
     reserved_sched: int = 0
+    # reverse filtering instead of smoothing (KalmanFitterOptions::
+    # reverseFiltering, reverseFilteringCovarianceScaling; the Examples'
+    # momentum threshold of SimpleReverseFilteringLogic)
+    reverse_filtering: bool = False
+    reserved_reverse: int = 0
+    reverse_filtering_cov_scaling: float = 100.0
+    reverse_filtering_momentum_threshold: float = 0.0
 
     def struct(self) -> capi.Options:
         o = capi.Options()
@@ -87,7 +94,7 @@ STATE_ARRAYS = {
 TRACK_ARRAYS = {
     "status": np.int32, "n_states": np.int32, "chi2": np.float32, "ndf": np.int32,
     "n_measurements": np.int32, "n_holes": np.int32, "n_outliers": np.int32,
-    "n_steps": np.int32, "n_step_trials": np.int32,
+    "n_steps": np.int32, "n_step_trials": np.int32, "reversed": np.int32,
 }
 # per-track parameter arrays, filled when the options ask for a reference
 # surface: name -> width

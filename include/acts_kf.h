@@ -34,8 +34,10 @@ extern "C" {
 #endif
 
 /* 2: acts_kf_batch_in gained meas_dims and meas_indices (measurement
- * subspaces); callers that value-initialise the struct are unaffected */
-#define ACTS_KF_API_VERSION 2
+ * subspaces); callers that value-initialise the struct are unaffected.
+ * 3: acts_kf_options gained the reverse-filtering options and
+ * acts_kf_batch_out the per-track `reversed` flag (both appended).         */
+#define ACTS_KF_API_VERSION 3
 
 /* ---- API return codes (the call itself) -------------------------------- */
 #define ACTS_KF_OK 0
@@ -57,6 +59,8 @@ extern "C" {
 #define ACTS_KF_STATUS_UPDATE_FAILED (ACTS_KF_CAT_KALMAN | 1)
 #define ACTS_KF_STATUS_SMOOTH_FAILED (ACTS_KF_CAT_KALMAN | 2)
 #define ACTS_KF_STATUS_NO_MEASUREMENT_FOUND (ACTS_KF_CAT_KALMAN | 4)
+#define ACTS_KF_STATUS_REVERSE_PROPAGATION_FAILED (ACTS_KF_CAT_KALMAN | 5)
+#define ACTS_KF_STATUS_INCONSISTENT_TRACK_STATES (ACTS_KF_CAT_KALMAN | 6)
 #define ACTS_KF_STATUS_PROPAGATION_FAILURE (ACTS_KF_CAT_PROPAGATOR | 1)
 #define ACTS_KF_STATUS_STEP_COUNT_LIMIT (ACTS_KF_CAT_PROPAGATOR | 2)
 #define ACTS_KF_STATUS_NEXT_TARGET_LIMIT (ACTS_KF_CAT_PROPAGATOR | 3)
@@ -234,6 +238,25 @@ typedef struct acts_kf_options {
    * loopers save in the field map.  No reference counterpart.               */
   int32_t sched_transport_skip_steps;
   int32_t reserved_sched;
+  /* Reverse filtering instead of smoothing (KalmanFitter.hpp:884-945,
+   * KalmanFitterOptions::reverseFiltering and
+   * reverseFilteringCovarianceScaling, :171-181): after the forward filter
+   * the track is filtered again in the opposite direction, from the last
+   * measurement state's filtered parameters with the covariance times
+   * reverse_filtering_cov_scaling (default 100), through the same surface
+   * sequence, with the reference surface as its target.  The first
+   * measurement state's smoothed parameters are the reverse filter's
+   * filtered parameters on that surface, the last measurement state's are
+   * its own filtered parameters; no other state has smoothed parameters
+   * (their smoothed slots hold NaN).  A track is filtered in reverse when
+   * reverse_filtering is 1, or when |1 / filtered q/p| of its last
+   * measurement state is at most reverse_filtering_momentum_threshold
+   * (the Examples' SimpleReverseFilteringLogic, KalmanFitterFunction.cpp:
+   * 53-57; default 0: never).                                              */
+  int32_t reverse_filtering;
+  int32_t reserved_reverse;
+  double reverse_filtering_cov_scaling;
+  double reverse_filtering_momentum_threshold;
 } acts_kf_options;
 
 /* ---- one batch of tracks ------------------------------------------------- */
@@ -310,6 +333,9 @@ typedef struct acts_kf_batch_out {
    * uncalibrated source link (IndexSourceLink::index); -1 for hole and
    * material-only states                                                   */
   int32_t* state_measurement;
+  /* [n] 1 if the track was smoothed by reverse filtering (the track's
+   * "reversed" column, KalmanFitter.hpp:978-981), else 0                   */
+  int32_t* reversed;
 } acts_kf_batch_out;
 
 typedef struct acts_kf_ctx acts_kf_ctx;

@@ -1417,6 +1417,7 @@ struct DirectNav {
   int surfaceIndex = -1;
   int current = -1;  // -1 none, -2 start surface, >= 0 geometry index
   bool navigationBreak = false;
+  double direction = 1.0;  // propagation direction
 };
 
 struct Target {
@@ -1452,12 +1453,16 @@ Target nextTarget(const Geometry& geo, DirectNav& nav, const V3& pos,
                   const V3& dir) {
   nav.current = -1;
   if (nav.navigationBreak) return Target{};
-  ++nav.surfaceIndex;
+  // nextSurface / endOfSurfaces (DirectNavigator.hpp:107-124): the sequence
+  // in the propagation direction
   const int n = static_cast<int>(nav.seq->size());
-  while (nav.surfaceIndex < n) {
+  const int inc = nav.direction < 0 ? -1 : 1;
+  nav.surfaceIndex += inc;
+  const V3 d = v3(nav.direction * dir[0], nav.direction * dir[1], nav.direction * dir[2]);
+  while (nav.surfaceIndex >= 0 && nav.surfaceIndex < n) {
     const int sid = (*nav.seq)[nav.surfaceIndex];
     const Surface& s = geo.surfaces[sid];
-    MultiIntersection mi = s.intersect(pos, dir, geo.opt.surface_tolerance);
+    MultiIntersection mi = s.intersect(pos, d, geo.opt.surface_tolerance);
     for (int i = 0; i < mi.size; ++i) {
       const Intersection& x = mi.is[i];
       if (x.valid() && x.path > geo.opt.near_limit &&
@@ -1465,7 +1470,7 @@ Target nextTarget(const Geometry& geo, DirectNav& nav, const V3& pos,
         return Target{false, sid, i};
       }
     }
-    ++nav.surfaceIndex;
+    nav.surfaceIndex += inc;
   }
   nav.navigationBreak = true;
   return Target{};
@@ -1528,13 +1533,17 @@ struct MeasLookup {
 // KalmanFitter.hpp:445-671 for a surface of the sequence
 int kfFilter(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
              const Surface& surf, int sid, bool isStartSurface, Stepper& st,
-             KFResult& res) {
+             KFResult& res, double propDir) {
   const bool precedingMeasurementExists = res.measurementStates > 0;
   const bool sensitive = surf.sensitive;
   const bool hasMat = surf.hasMaterial;
-  const int mi = isStartSurface ? -1 : ml.find(sid);
+  // the track's own start plane (sid -2) carries no measurement; a geometry
+  // surface does when a measurement is keyed to it (the reverse filter
+  // starts on the last measurement surface)
+  const int mi = sid >= 0 ? ml.find(sid) : -1;
   const auto& o = geo.opt;
-  // the start surface only takes post-update material (not a fit surface here)
+  // determineMaterialUpdateMode (PointwiseMaterialInteraction.cpp:16-29):
+  // the start surface only takes post-update material
   auto matMode = [&](int requested) {
     if (isStartSurface) return requested & PostUpdate;
     return requested;
@@ -1542,7 +1551,7 @@ int kfFilter(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
   if (mi >= 0) {
     transportCovarianceToBound(surf, st);
     int r = performMaterialInteraction(surf, st, geo.particle, o,
-                                       matMode(PreUpdate), 1.0);
+                                       matMode(PreUpdate), propDir);
     if (r) return r;
     KFState ks;
     ks.isMeasurementBranch = true;
@@ -1585,7 +1594,7 @@ int kfFilter(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
       ++res.measurementStates;
     }
     r = performMaterialInteraction(surf, st, geo.particle, o, matMode(PostUpdate),
-                                   1.0);
+                                   propDir);
     if (r) return r;
     ++res.processedStates;
     res.measurementHoles = res.missedActive;
@@ -1610,7 +1619,7 @@ int kfFilter(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
     if (ks.s.flags & ACTS_KF_STATE_IS_HOLE) ++res.missedActive;
     ++res.processedStates;
     r = performMaterialInteraction(surf, st, geo.particle, o, matMode(FullUpdate),
-                                   1.0);
+                                   propDir);
     if (r) return r;
   }
   return 0;
@@ -1694,48 +1703,108 @@ void mbfSmooth(const TrackIn& in, std::vector<KFState>& S, int tip) {
   }
 }
 
-void fitTrack(const Geometry& geo, const TrackIn& in, TrackOut& out) {
-  const auto& o = geo.opt;
-  out = TrackOut();
-  // make_propagator_options: measurements keyed by surface, first wins
-  // (KalmanFitter.hpp:750-755)
-  MeasLookup ml;
-  for (size_t i = 0; i < in.measSurface.size(); ++i) {
-    if (ml.find(in.measSurface[i]) < 0) {
-      ml.surfaces.push_back(in.measSurface[i]);
-      ml.index.push_back(static_cast<int>(i));
+namespace {
+
+// What one filter pass leaves: the actor's result, the propagation's
+// status and statistics, and, if the actor's target surface was reached,
+// the bound parameters there (KalmanFitterResult::fittedParameters).
+struct PassOut {
+  KFResult res;
+  int status = 0;
+  int nSteps = 0, nTrials = 0;
+  bool targetReached = false;
+  Vec6 targetParams;
+  M6 targetCov;
+};
+
+// SurfaceReached::checkAbort (StandardAborters.hpp:94-155) for the actor's
+// target: reached if the closest intersection is on the surface; the
+// actor step size is set to the first intersection in front of the near
+// limit (updateSingleSurfaceStatus, SteppingHelper.hpp:41-85, which
+// releases the actor constraint first).
+bool targetReachedCheck(const Surface& target, Stepper& st, double propDir, double tol) {
+  const V3 d = st.direction();
+  const V3 dd = v3(propDir * d[0], propDir * d[1], propDir * d[2]);
+  const MultiIntersection mi = target.intersect(st.position(), dd, tol);
+  // closest(): the target is a line surface here, with one intersection
+  const bool reached = mi.size > 0 && mi.is[0].status == IStatus::onSurface;
+  const double nearLimit = -100 * 1e-3;
+  for (int i = 0; i < mi.size; ++i) {
+    const Intersection& x = mi.is[i];
+    if (x.valid() && x.path > nearLimit &&
+        x.path < std::numeric_limits<double>::max() + kOnSurfaceTolerance) {
+      st.stepSize.release(ConstrainedStep::Actor);
+      st.updateStepSize(x.path, ConstrainedStep::Actor);
+      break;
     }
   }
-  const size_t nInputMeas = ml.surfaces.size();
+  return reached;
+}
 
+// filter_impl (KalmanFitter.hpp:797-852): Propagator::propagate
+// (Propagator.ipp:25-210) with the KalmanFitter actor, in direction
+// propDir, from bound parameters on `startSurface` with covariance
+// `startCov`.  startSid is the start surface's geometry index, or -2 for
+// the track's own start plane, which the sequence does not contain.  The
+// DirectNavigator starts next to the start surface's first entry in the
+// sequence, or at the sequence's end in the propagation direction
+// (DirectNavigator.hpp:222-272); the actor acts on the start surface first
+// (the navigator's current surface).  `target` is the actor's targetReached
+// surface (null: none).
+void filterPass(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
+                size_t nInputMeas, const Surface& startSurface, int startSid,
+                const Vec6& startParams, const M6& startCov, double propDir,
+                const Surface* target, PassOut& P) {
+  const auto& o = geo.opt;
   Stepper st;
-  initializeStepper(st, in.startSurface, in.startParams, &in.startCov, o);
+  initializeStepper(st, startSurface, startParams, &startCov, o);
   DirectNav nav;
   nav.seq = &in.sequence;
-  // DirectNavigator::initialize: start surface not in the sequence
-  nav.current = -2;
-  nav.surfaceIndex = -1;
+  nav.direction = propDir;
+  nav.current = startSid;
+  const int n = static_cast<int>(in.sequence.size());
+  int found = -1;
+  for (int i = 0; i < n && startSid >= 0; ++i)
+    if (in.sequence[i] == startSid) {
+      found = i;
+      break;
+    }
+  if (found >= 0)
+    nav.surfaceIndex = found + (propDir < 0 ? 1 : -1);
+  else
+    nav.surfaceIndex = propDir < 0 ? n : -1;
   nav.navigationBreak = false;
   double propPathLimit = o.path_limit;
-  setupLoopProtection(geo, st, propPathLimit, false, 1.0);
+  setupLoopProtection(geo, st, propPathLimit, false, propDir);
 
-  KFResult res;
+  KFResult& res = P.res;
   int status = 0;
 
   // KalmanFitter::Actor::act (KalmanFitter.hpp:339-422)
   auto act = [&]() -> int {
     if (res.finished) return 0;
     if (res.pathLimit == std::numeric_limits<double>::max())
-      setupLoopProtection(geo, st, res.pathLimit, true, 1.0);
+      setupLoopProtection(geo, st, res.pathLimit, true, propDir);
     if (nav.current != -1) {
-      const bool isStart = nav.current == -2;
-      const Surface& s = isStart ? in.startSurface : geo.surfaces[nav.current];
-      int r = kfFilter(geo, in, ml, s, nav.current, isStart, st, res);
+      const bool isStart = nav.current == startSid;
+      const Surface& s = nav.current == -2 ? in.startSurface : geo.surfaces[nav.current];
+      int r = kfFilter(geo, in, ml, s, nav.current, isStart, st, res, propDir);
       if (r) return r;
     }
     const bool complete = res.measurementStates == nInputMeas;
     const bool pathReached = pathLimitCheckAbort(res.pathLimit, st, o.surface_tolerance);
-    if (complete || pathReached) res.finished = true;
+    const bool targetHit =
+        target != nullptr && targetReachedCheck(*target, st, propDir, o.surface_tolerance);
+    if (complete || pathReached || targetHit) {
+      if (targetHit) {
+        // boundState on the target: the fitted parameters there
+        int r = boundState(*target, st, true, P.targetParams);
+        if (r) return r;
+        P.targetCov = st.cov;
+        P.targetReached = true;
+      }
+      res.finished = true;
+    }
     return 0;
   };
   auto checkAbort = [&]() -> bool {
@@ -1752,7 +1821,7 @@ void fitTrack(const Geometry& geo, const TrackIn& in, TrackOut& out) {
         return 0;
       }
       IStatus ps = updateSurfaceStatus(geo.surfaces[nt.surface], st, nt.index,
-                                       1.0, o.surface_tolerance);
+                                       propDir, o.surface_tolerance);
       if (ps == IStatus::onSurface) continue;
       if (ps == IStatus::reachable) {
         t = nt;
@@ -1771,7 +1840,7 @@ void fitTrack(const Geometry& geo, const TrackIn& in, TrackOut& out) {
     status = getNextTarget(target);
     unsigned steps = 0;
     for (; !status && steps < o.max_steps; ++steps) {
-      StepResult sr = step(st, geo.field, geo.particle, o, 1.0);
+      StepResult sr = step(st, geo.field, geo.particle, o, propDir);
       if (sr.status) {
         status = sr.status;
         break;
@@ -1780,7 +1849,7 @@ void fitTrack(const Geometry& geo, const TrackIn& in, TrackOut& out) {
       st.stepSize.release(ConstrainedStep::Actor);
       if (!target.none) {
         IStatus ps = updateSurfaceStatus(geo.surfaces[target.surface], st,
-                                         target.index, 1.0, o.surface_tolerance);
+                                         target.index, propDir, o.surface_tolerance);
         if (ps == IStatus::onSurface && !nav.navigationBreak)
           nav.current = (*nav.seq)[nav.surfaceIndex];
         if (ps != IStatus::reachable) target = Target{};
@@ -1799,8 +1868,48 @@ void fitTrack(const Geometry& geo, const TrackIn& in, TrackOut& out) {
     }
     if (!status && !terminatedNormally) status = ACTS_KF_STATUS_STEP_COUNT_LIMIT;
   }
-  out.nSteps = st.nSteps;
-  out.nTrials = st.nStepTrials;
+  P.status = status;
+  P.nSteps = st.nSteps;
+  P.nTrials = st.nStepTrials;
+}
+
+// findFirstMeasurementState / findLastMeasurementState (TrackHelpers.hpp:
+// 64-113): states with a measurement that is not an outlier, up to the tip
+void measurementStateRange(const std::vector<KFState>& S, int tip, int& first, int& last) {
+  first = last = -1;
+  for (int k = 0; k <= tip; ++k) {
+    const uint32_t f = S[k].s.flags;
+    if ((f & ACTS_KF_STATE_HAS_MEASUREMENT) && !(f & ACTS_KF_STATE_IS_OUTLIER)) {
+      if (first < 0) first = k;
+      last = k;
+    }
+  }
+}
+
+}  // namespace
+
+void fitTrack(const Geometry& geo, const TrackIn& in, TrackOut& out) {
+  const auto& o = geo.opt;
+  out = TrackOut();
+  // make_propagator_options: measurements keyed by surface, first wins
+  // (KalmanFitter.hpp:750-755)
+  MeasLookup ml;
+  for (size_t i = 0; i < in.measSurface.size(); ++i) {
+    if (ml.find(in.measSurface[i]) < 0) {
+      ml.surfaces.push_back(in.measSurface[i]);
+      ml.index.push_back(static_cast<int>(i));
+    }
+  }
+  const size_t nInputMeas = ml.surfaces.size();
+
+  // the forward filter: from the start parameters, no target surface
+  PassOut F;
+  filterPass(geo, in, ml, nInputMeas, in.startSurface, -2, in.startParams, in.startCov, 1.0,
+             nullptr, F);
+  KFResult& res = F.res;
+  const int status = F.status;
+  out.nSteps = F.nSteps;
+  out.nTrials = F.nTrials;
   if (status) {
     out.status = status;
     return;
@@ -1813,8 +1922,57 @@ void fitTrack(const Geometry& geo, const TrackIn& in, TrackOut& out) {
   // states of the track: first .. tip (= lastMeasurementIndex)
   const int tip = res.lastMeasurementIndex;
   std::vector<KFState>& S = res.states;
+  int firstMeas = -1, lastMeas = -1;
+  measurementStateRange(S, tip, firstMeas, lastMeas);
 
-  if (o.smoother == ACTS_KF_SMOOTHER_MBF) {
+  // fit_impl (KalmanFitter.hpp:884-945): reverse filtering when asked for,
+  // or when the reverse-filtering logic accepts the last measurement state
+  // (the Examples' SimpleReverseFilteringLogic, KalmanFitterFunction.cpp:
+  // 53-57: |1 / filtered q/p| <= momentum threshold)
+  const bool doReverse =
+      o.reverse_filtering != 0 ||
+      std::abs(1.0 / S[lastMeas].s.filtered.a[4][0]) <= o.reverse_filtering_momentum_threshold;
+  out.reversed = doReverse;
+  if (doReverse) {
+    // the last measurement state's filtered parameters, covariance scaled
+    const KFState& L = S[lastMeas];
+    M6 C = L.s.filteredCov;
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) C.a[i][j] = C.a[i][j] * o.reverse_filtering_cov_scaling;
+    PassOut R;
+    const Surface* target =
+        o.reference_surface == ACTS_KF_REFERENCE_PERIGEE ? &geo.reference : nullptr;
+    filterPass(geo, in, ml, nInputMeas, geo.surfaces[L.s.surface], L.s.surface, L.s.filtered, C,
+               -1.0, target, R);
+    if (!R.status && R.res.measurementStates == 0) R.status = ACTS_KF_STATUS_NO_MEASUREMENT_FOUND;
+    if (R.status) {
+      out.status = R.status;
+      return;
+    }
+    int rFirst = -1, rLast = -1;
+    measurementStateRange(R.res.states, R.res.lastMeasurementIndex, rFirst, rLast);
+    if (R.res.states[rLast].s.surface != S[firstMeas].s.surface) {
+      out.status = ACTS_KF_STATUS_INCONSISTENT_TRACK_STATES;
+      return;
+    }
+    // only the first and last measurement states carry smoothed parameters
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    for (int k = 0; k <= tip; ++k) {
+      for (int i = 0; i < 6; ++i) {
+        S[k].s.smoothed.a[i][0] = nan;
+        for (int j = 0; j < 6; ++j) S[k].s.smoothedCov.a[i][j] = nan;
+      }
+    }
+    S[lastMeas].s.smoothed = S[lastMeas].s.filtered;
+    S[lastMeas].s.smoothedCov = S[lastMeas].s.filteredCov;
+    S[firstMeas].s.smoothed = R.res.states[rLast].s.filtered;
+    S[firstMeas].s.smoothedCov = R.res.states[rLast].s.filteredCov;
+    if (R.targetReached) {
+      out.hasReference = true;
+      out.referenceParams = R.targetParams;
+      out.referenceCov = R.targetCov;
+    }
+  } else if (o.smoother == ACTS_KF_SMOOTHER_MBF) {
     mbfSmooth(in, S, tip);
   } else {
     // GainMatrixSmoother::operator() (GainMatrixSmoother.hpp:43-99)
@@ -1853,8 +2011,9 @@ void fitTrack(const Geometry& geo, const TrackIn& in, TrackOut& out) {
   out.status = 0;
 
   // KalmanFitter.hpp:959-971: with a reference surface the fit ends with
-  // the extrapolation of the smoothed track, and fails if that fails
-  if (o.reference_surface == ACTS_KF_REFERENCE_PERIGEE) {
+  // the extrapolation of the smoothed track, and fails if that fails (not
+  // when the reverse filter reached the reference surface itself)
+  if (o.reference_surface == ACTS_KF_REFERENCE_PERIGEE && !out.hasReference) {
     const int r = extrapolateToReference(geo, out);
     if (r) {
       const int nSteps = out.nSteps, nTrials = out.nTrials;

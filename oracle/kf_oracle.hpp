@@ -350,6 +350,7 @@ struct StateOut {
 struct TrackOut {
   int status = 0;
   bool hasReference = false;
+  bool reversed = false;  // smoothed by reverse filtering
   Vec6 referenceParams;
   M6 referenceCov;
   std::vector<StateOut> states;  // up to and including the tip

@@ -111,6 +111,7 @@ void storeTrack(const acts_kf_batch_in* in, const acts_kf_batch_out* out,
   if (out->n_outliers) out->n_outliers[t] = tr.nOutliers;
   if (out->n_steps) out->n_steps[t] = tr.nSteps;
   if (out->n_step_trials) out->n_step_trials[t] = tr.nTrials;
+  if (out->reversed) out->reversed[t] = (!tr.status && tr.reversed) ? 1 : 0;
   const int64_t base = in->seq_offsets[t];
   const int64_t cap = in->seq_offsets[t + 1] - base;
   // slots without a state (past the tip, all of a failed track's) hold an
