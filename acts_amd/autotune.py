@@ -6,7 +6,9 @@ a fit, never its results (acts_kf_options):
                      updated;
   sched_wait_64ths   the fraction of busy lanes that must wait at a surface
                      before the surface update runs early;
-  length_sorted      whether tracks enter the queue longest-first;
+  length_sorted      the order in which tracks enter the queue: 1 longest
+                     first, 0 input order, 2 by start direction, 3 by
+                     length then momentum;
   sched_quarantine_steps  a wavefront holding a track older than this many
                      steps takes no new tracks until it ends;
   sched_transport_skip_steps  steps since the last surface after which a
@@ -17,6 +19,13 @@ a fit, never its results (acts_kf_options):
 `autotune` times candidate settings on a device-resident sample of the
 workload by coordinate descent and keeps the fastest; results are cached in
 a JSON file keyed by the workload's shape.
+
+The kernels' shapes are not knobs: the forward kernel's register budget
+(every lane holds its track's state, ~450 registers) allows one 64-lane
+wavefront per SIMD, so a workgroup is one wavefront; forcing two per SIMD
+spills, and running the forward pass as separate stepping and surface
+kernels over the whole batch was measured slower (the lanes' state moves
+through HBM every round).
 """
 from __future__ import annotations
 
@@ -30,11 +39,11 @@ import numpy as np
 SPACE = {
     "sched_inner_steps": [8, 16, 32, 64, 128],
     "sched_wait_64ths": [16, 32, 48, 64],
-    "length_sorted": [True, False],
+    "length_sorted": [1, 0, 2, 3],
     "sched_quarantine_steps": [0, 100, 200, 400],
     "sched_transport_skip_steps": [0, 64, 256],
 }
-DEFAULTS = {"sched_inner_steps": 64, "sched_wait_64ths": 64, "length_sorted": True,
+DEFAULTS = {"sched_inner_steps": 64, "sched_wait_64ths": 64, "length_sorted": 1,
             "sched_quarantine_steps": 0, "sched_transport_skip_steps": 0}
 
 

@@ -14,6 +14,7 @@
 
 #include "kf_field_host.hpp"
 #include "kf_reference.hpp"
+#include "kf_reverse.hpp"
 #include "kf_states.hpp"
 
 using namespace kf;
@@ -63,8 +64,9 @@ constexpr int kMaxDevices = 64;
 // take new tracks; stepping lanes step until no lane steps any more, or at
 // least `waitFrac`/64 of the busy lanes wait at a surface, or `maxInner`
 // steps have been taken; then the waiting lanes run the surface update
-// together, and finished lanes write their summary and go idle.
-template <bool Mixed, bool GeoInLds, bool Sympy = false>
+// together, and finished lanes write their summary and go idle.  With Rev
+// the kernel runs the reverse filter of the tracks that take one.
+template <bool Mixed, bool GeoInLds, bool Sympy = false, bool Rev = false>
 __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKernel(
     FwdArgs a, int32_t* __restrict__ next, int32_t maxInner, int32_t waitFrac,
     int32_t quarantine, const int32_t* __restrict__ order) {
@@ -109,18 +111,22 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
       base = __shfl(base, leader);
       if (phase == kPhaseIdle) {
         const int64_t q = static_cast<int64_t>(base) + __popcll(idle & lanesBelow);
-        phase = (q < n) ? laneInit<Mixed, Sympy>(L, q, order ? order[q] : q, a, geo)
-                        : kPhaseRetired;
+        if (q >= n)
+          phase = kPhaseRetired;
+        else if constexpr (Rev)
+          phase = laneInitReverse<Mixed, Sympy>(L, q, order ? order[q] : q, a, geo);
+        else
+          phase = laneInit<Mixed, Sympy>(L, q, order ? order[q] : q, a, geo);
       }
     }
     if (phase == kPhaseFinished) {
-      laneFinish(L, a);
+      laneFinish<Rev>(L, a);
       phase = kPhaseIdle;
     }
     const uint64_t busy = __ballot(phase != kPhaseRetired);
     if (!busy) break;
     for (int it = 0; it < maxInner; ++it) {
-      if (phase == kPhaseStepping) phase = laneStep<Mixed, Sympy>(L, a, geo);
+      if (phase == kPhaseStepping) phase = laneStep<Mixed, Sympy, Rev>(L, a, geo);
       const int nStep = __popcll(__ballot(phase == kPhaseStepping));
       const int nWait = __popcll(__ballot(phase == kPhaseSurface || phase == kPhaseFinished));
       if (nStep == 0 || 64 * nWait >= waitFrac * (nStep + nWait)) break;
@@ -135,9 +141,9 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
       L.noSkip = true;
       phase = kPhaseStepping;
     }
-    if (phase == kPhaseSurface) phase = laneFilter<Mixed, Sympy>(L, a, geo, slabs);
+    if (phase == kPhaseSurface) phase = laneFilter<Mixed, Sympy, Rev>(L, a, geo, slabs);
     if (phase == kPhaseFinished) {
-      laneFinish(L, a);
+      laneFinish<Rev>(L, a);
       phase = kPhaseIdle;
     }
   }
@@ -155,11 +161,13 @@ __global__ __launch_bounds__(kTsBlock) void trackStatesKernel(TsArgs a) {
 template <bool Mbf>
 __global__ __launch_bounds__(kBlock) void smoothKernel(DevBatch in, DevOut out,
                                                        DevScratch sc,
-                                                       const int32_t* __restrict__ perm) {
+                                                       const int32_t* __restrict__ perm,
+                                                       RevArgs rv, int32_t revEnabled) {
   const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const bool valid = g < in.n;
   const int64_t t = !valid ? 0 : (perm ? perm[g] : g);
-  smoothTrack<Mbf>(g, t, valid, in, out, sc);
+  const bool rev = valid && revEnabled && reverseWanted(rv, sc, g);
+  smoothTrack<Mbf>(g, t, valid, in, out, sc, rev);
 }
 
 // Extrapolation of every successfully smoothed track to the reference
@@ -174,7 +182,8 @@ __global__ __launch_bounds__(kBlock) void referenceKernel(DevOut out, DevScratch
                                                           const int64_t* __restrict__ seqOff,
                                                           const int32_t* __restrict__ perm) {
   const int64_t q = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (q >= n || sc.track[q] != 0) return;
+  // failed fits, and reverse filters that reached the surface themselves
+  if (q >= n || sc.track[q] != 0 || sc.track[2 * sc.stride + q] == kReferenceSet) return;
   const int64_t t = perm ? perm[q] : q;
   double p[6], C[36];
   const int st = extrapolateTrack<Mixed, Sympy>(q, sc, sc.ref, geo, F, o, L, strategy, p, C);
@@ -443,6 +452,13 @@ struct acts_kf_ctx {
   // per resident lane of the forward kernel
   int32_t skipAfter = 0;
   DevBuf ckptBuf;
+  // reverse filtering (KalmanFitterOptions::reverseFiltering, the covariance
+  // scaling, the reverse-filtering logic's momentum threshold)
+  bool revAll = false;
+  bool revEnabled = false;
+  double revScale = 100.0;
+  double revThreshold = 0.0;
+  DevBuf revBuf;
   // reference surface of the fitted parameters, and the default propagator
   // and stepper options its extrapolation runs with
   int32_t refSurface = ACTS_KF_REFERENCE_NONE;
@@ -553,12 +569,24 @@ bool geometryFitsLds(const acts_kf_ctx* c) {
   return c->nSurfaces <= kLdsSurfaces && c->nSlabs <= kLdsSlabs;
 }
 
-ForwardFn forwardKernel(bool mixed, bool geoInLds, bool sympy = false) {
+template <bool Rev>
+ForwardFn forwardKernelOf(bool mixed, bool geoInLds, bool sympy) {
   if (sympy && mixed)
-    return geoInLds ? fitForwardKernel<true, true, true> : fitForwardKernel<true, false, true>;
-  if (sympy) return geoInLds ? fitForwardKernel<false, true, true> : fitForwardKernel<false, false, true>;
-  if (mixed) return geoInLds ? fitForwardKernel<true, true> : fitForwardKernel<true, false>;
-  return geoInLds ? fitForwardKernel<false, true> : fitForwardKernel<false, false>;
+    return geoInLds ? fitForwardKernel<true, true, true, Rev>
+                    : fitForwardKernel<true, false, true, Rev>;
+  if (sympy)
+    return geoInLds ? fitForwardKernel<false, true, true, Rev>
+                    : fitForwardKernel<false, false, true, Rev>;
+  if (mixed)
+    return geoInLds ? fitForwardKernel<true, true, false, Rev>
+                    : fitForwardKernel<true, false, false, Rev>;
+  return geoInLds ? fitForwardKernel<false, true, false, Rev>
+                  : fitForwardKernel<false, false, false, Rev>;
+}
+// the forward filter, or with `rev` the reverse filter
+ForwardFn forwardKernel(bool mixed, bool geoInLds, bool sympy = false, bool rev = false) {
+  return rev ? forwardKernelOf<true>(mixed, geoInLds, sympy)
+             : forwardKernelOf<false>(mixed, geoInLds, sympy);
 }
 
 }  // namespace
@@ -809,6 +837,10 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   if (options->sched_wait_64ths > 0) c->waitFrac = std::min(options->sched_wait_64ths, 64);
   c->quarantine = std::max(options->sched_quarantine_steps, 0);
   c->skipAfter = std::max(options->sched_transport_skip_steps, 0);
+  c->revAll = options->reverse_filtering != 0;
+  c->revScale = options->reverse_filtering_cov_scaling;
+  c->revThreshold = options->reverse_filtering_momentum_threshold;
+  c->revEnabled = c->revAll || c->revThreshold > 0;
   {
     int blocksPerCu = 0, cus = 0;
     const hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -857,6 +889,8 @@ struct Workspace {
   int32_t* perm = nullptr;   // queue order (sorted queues only)
   int32_t* hist = nullptr;   // per-block key histograms of the sort
   Lane* ckpt = nullptr;      // transport-skip checkpoints (null: no skipping)
+  double* rev = nullptr;     // reverse filtering: per slot its record (kRevRecord)
+  int32_t* revReached = nullptr;  // and whether it reached the reference surface
 };
 
 // Sizes of the workspace of a fit of n tracks of at most `states` states.
@@ -867,6 +901,7 @@ struct WorkspaceSizes {
   int32_t nKeys = 0;
   int64_t sortBlocks = 0;
   size_t d = 0, i = 0, track = 0, ref = 0, queue = 0, perm = 0, hist = 0;
+  size_t rev = 0;  // reverse filtering: records and reached flags
 };
 
 WorkspaceSizes workspaceSizes(const acts_kf_ctx* c, int64_t n, int32_t maxLen) {
@@ -888,7 +923,17 @@ WorkspaceSizes workspaceSizes(const acts_kf_ctx* c, int64_t n, int32_t maxLen) {
   w.sortBlocks = (n + kSortBlock - 1) / kSortBlock;
   w.perm = w.sorted ? sizeof(int32_t) * n : 0;
   w.hist = w.sorted ? sizeof(int32_t) * w.nKeys * w.sortBlocks : 0;
+  w.rev = c->revEnabled
+              ? (sizeof(double) * kRevRecord + sizeof(int32_t)) * static_cast<size_t>(n) + 256
+              : 0;
   return w;
+}
+
+// The reverse filter's records and reached flags in a block of w.rev bytes.
+void reverseSections(char* base, const WorkspaceSizes& w, Workspace& ws) {
+  const size_t n = (w.rev - 256) / (sizeof(double) * kRevRecord + sizeof(int32_t));
+  ws.rev = reinterpret_cast<double*>(base);
+  ws.revReached = reinterpret_cast<int32_t*>(base + sizeof(double) * kRevRecord * n);
 }
 
 // The context's own (grow-only) workspace buffers, grown to `w`.
@@ -900,6 +945,9 @@ int contextWorkspace(acts_kf_ctx* c, const WorkspaceSizes& w, Workspace& ws) {
     return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "scratch allocation failed");
   if (w.sorted && (c->permBuf.ensure(w.perm) || c->histBuf.ensure(w.hist)))
     return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "permutation allocation failed");
+  if (w.rev && c->revBuf.ensure(w.rev))
+    return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "reverse-filter scratch allocation failed");
+  if (w.rev) reverseSections(static_cast<char*>(c->revBuf.p), w, ws);
   ws.d = static_cast<double*>(c->scratchD.p);
   ws.i = static_cast<int32_t*>(c->scratchI.p);
   ws.track = static_cast<int32_t*>(c->scratchT.p);
@@ -989,6 +1037,7 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   d.referenceParams = out->reference_params;
   d.referenceCov = out->reference_cov;
   d.stateMeas = out->state_measurement;
+  d.reversed = out->reversed;
   DevScratch sc;
   sc.ref = withRef ? ws.ref : nullptr;
   sc.d = ws.d;
@@ -1034,6 +1083,13 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   fa.ckpt = ws.ckpt;
   fa.skipAfter = ws.ckpt ? c->skipAfter : 0;
   fa.pad = 0;
+  fa.rv.scale = c->revScale;
+  fa.rv.threshold = c->revThreshold;
+  fa.rv.all = c->revAll ? 1 : 0;
+  fa.rv.hasTarget = withRef ? 1 : 0;
+  fa.rv.target = c->refLine;
+  fa.rv.rec = ws.rev;
+  fa.rv.reached = ws.revReached;
   int32_t* queue = ws.queue;
   hipLaunchKernelGGL(zeroKernel, dim3(1), dim3(64), 0, stream, queue, 1);
   const int64_t waves = std::min(blocks, c->residentWaves);
@@ -1042,8 +1098,18 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
                      dim3(static_cast<unsigned>(waves)), dim3(kBlock), 0, stream, fa, queue,
                      c->innerSteps, c->waitFrac, c->quarantine, perm);
   if (timing) KF_HIP(hipEventRecord(c->ev[1], stream));
+  const int32_t revOn = (c->revEnabled && ws.rev) ? 1 : 0;
   hipLaunchKernelGGL(c->mbf ? smoothKernel<true> : smoothKernel<false>,
-                     dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream, b, d, sc, perm);
+                     dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, stream, b, d, sc, perm,
+                     fa.rv, revOn);
+  if (revOn) {
+    // the reverse filter of the tracks that take one, in place of their
+    // smoothing, before the extrapolation
+    hipLaunchKernelGGL(zeroKernel, dim3(1), dim3(64), 0, stream, queue, 1);
+    hipLaunchKernelGGL(forwardKernel(c->mixed, geometryFitsLds(c), c->sympy, true),
+                       dim3(static_cast<unsigned>(waves)), dim3(kBlock), 0, stream, fa, queue,
+                       c->innerSteps, c->waitFrac, 0, perm);
+  }
   if (withRef) {
     if (c->sympy && c->mixed)
       hipLaunchKernelGGL((referenceKernel<true, true>), dim3(static_cast<unsigned>(blocks)),
@@ -1153,7 +1219,7 @@ int acts_kf_fit_batch_device_noalloc(acts_kf_ctx* c, const acts_kf_batch_in* in,
     KF_HIP(hipStreamSynchronize(s));
   }
   const WorkspaceSizes w = workspaceSizes(c, n, maxLen);
-  const size_t sizes[] = {w.queue, w.track, w.perm, w.hist, w.ref, w.i, w.d};
+  const size_t sizes[] = {w.queue, w.track, w.perm, w.hist, w.ref, w.rev, w.i, w.d};
   size_t need = 256;
   for (size_t b : sizes) need += al(b);
   if (need > kStaticWorkspaceBytes)
@@ -1174,6 +1240,7 @@ int acts_kf_fit_batch_device_noalloc(acts_kf_ctx* c, const acts_kf_batch_in* in,
   ws.perm = static_cast<int32_t*>(take(w.perm));
   ws.hist = static_cast<int32_t*>(take(w.hist));
   ws.ref = static_cast<double*>(take(w.ref));
+  if (w.rev) reverseSections(static_cast<char*>(take(w.rev)), w, ws);
   ws.i = static_cast<int32_t*>(take(w.i));
   ws.d = static_cast<double*>(take(w.d));
   ws.ckpt = nullptr;  // no transport skipping: its checkpoints are context memory
@@ -1274,7 +1341,8 @@ int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
       {out->jacobian, sizeof(double) * 36 * nSeq},
       {out->reference_params, sizeof(double) * 6 * n},
       {out->reference_cov, sizeof(double) * 36 * n},
-      {out->state_measurement, sizeof(int32_t) * nSeq}};
+      {out->state_measurement, sizeof(int32_t) * nSeq},
+      {out->reversed, sizeof(int32_t) * n}};
   constexpr int nOut = sizeof(outs) / sizeof(outs[0]);
   size_t totalO = 0;
   for (const OSec& q : outs)
@@ -1311,6 +1379,7 @@ int acts_kf_fit_batch(acts_kf_ctx* c, const acts_kf_batch_in* in,
   dout.reference_params = static_cast<double*>(optr[20]);
   dout.reference_cov = static_cast<double*>(optr[21]);
   dout.state_measurement = static_cast<int32_t*>(optr[22]);
+  dout.reversed = static_cast<int32_t*>(optr[23]);
   const WorkspaceSizes w = workspaceSizes(c, n, maxLen);
   Workspace ws;
   if (const int r = contextWorkspace(c, w, ws)) return r;
@@ -1408,6 +1477,7 @@ void acts_kf_destroy(acts_kf_ctx* c) {
   c->scratchT.release();
   c->refBuf.release();
   c->ckptBuf.release();
+  c->revBuf.release();
   c->lenBuf.release();
   c->permBuf.release();
   c->histBuf.release();

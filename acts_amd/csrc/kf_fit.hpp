@@ -29,6 +29,7 @@ struct DevOut {
   double* referenceParams;
   double* referenceCov;
   int32_t* stateMeas;
+  int32_t* reversed;
 };
 
 // ---------------------------------------------------------------------------
@@ -190,9 +191,12 @@ __device__ inline void transportToBound(Track& T, const DevSurface& s,
 // detail::performMaterialInteraction for a slab scaled by 1
 // (PointwiseMaterialInteraction.hpp:165-226, .cpp:31-87), forward direction
 // ---------------------------------------------------------------------------
+// propDir: the propagation direction; going backwards the momentum grows
+// by the energy loss (PointwiseMaterialInteraction.hpp:188-198).
 __device__ inline int materialInteraction(Track& T, const DevSurface& s,
                                           const DevSlab& slab0,
-                                          const PhysConst& pc, const DevOpts& o) {
+                                          const PhysConst& pc, const DevOpts& o,
+                                          double propDir = 1.0) {
   const double pathCorr = pathCorrection(s, T.pos, T.dir);
   double l0, l1;
   if (!globalToLocal(s, T.pos, l0, l1)) return ACTS_KF_STATUS_NOT_ON_SURFACE;
@@ -219,7 +223,7 @@ __device__ inline int materialInteraction(Track& T, const DevSurface& s,
     varQop = sq * sq;
   }
   const double momentum = copysign(absQ, T.qop) / T.qop;
-  const double nextE = sqrt(mass * mass + momentum * momentum) - eLoss * 1.0;
+  const double nextE = sqrt(mass * mass + momentum * momentum) - eLoss * propDir;
   double nextP = (mass < nextE) ? cathetus(nextE, mass) : 0;
   nextP = fmax(pc.minP, nextP);
   const float signedQ = static_cast<float>(copysign(absQ, static_cast<double>(qopF)));
@@ -693,6 +697,26 @@ __device__ inline int smoothStep(const double xf[6], const double Cf[21],
   return 0;
 }
 
+// The reference surface: a PerigeeSurface, i.e. a LineSurface along the
+// global z axis through the point c (LineSurface.cpp:57-250).
+struct LineRef {
+  double c[3];
+};
+
+// LineSurface::intersect (LineSurface.cpp:136-186) without bounds
+__device__ inline int lineIntersect(const LineRef& L, const V3& pos, const V3& dir,
+                                    double tol, double& path) {
+  const V3 eb = mk(0., 0., 1.);
+  const V3 mab = mk(L.c[0] - pos.x, L.c[1] - pos.y, L.c[2] - pos.z);
+  const double eaTeb = dot(dir, eb);
+  const double denom = 1 - eaTeb * eaTeb;
+  path = INFINITY;
+  if (fabs(denom) < fabs(tol)) return kUnreachable;
+  const double u = (dot(mab, dir) - dot(mab, eb) * eaTeb) / denom;
+  path = u;
+  return fabs(u) > fabs(tol) ? kReachable : kOnSurface;
+}
+
 // ---------------------------------------------------------------------------
 // navigation: DirectNavigator.hpp:280-390, SteppingHelper.hpp:40-78
 // ---------------------------------------------------------------------------
@@ -704,9 +728,17 @@ struct Nav {
   int tIdx;     // intersection index of the target
 };
 
+// The direction the navigation intersects along: the track's direction
+// times the propagation direction (backward for the reverse filter).
+template <bool Rev>
+__device__ inline V3 navDir(const Track& T) {
+  return Rev ? mk(-T.dir.x, -T.dir.y, -T.dir.z) : T.dir;
+}
+
+template <bool Rev = false>
 __device__ inline int updateSurfaceStatus(Track& T, const DevSurface& s, int idx,
                                           double tol) {
-  const Isect x = intersect(s, T.pos, T.dir, tol);
+  const Isect x = intersect(s, T.pos, navDir<Rev>(T), tol);
   const double path = idx ? x.path[1] : x.path[0];
   const int st = idx ? x.status[1] : x.status[0];
   if (st == kOnSurface) {
@@ -722,15 +754,19 @@ __device__ inline int updateSurfaceStatus(Track& T, const DevSurface& s, int idx
   return kUnreachable;
 }
 
-template <typename SeqFn>
+// nextSurface / endOfSurfaces (DirectNavigator.hpp:107-124): the sequence
+// in the propagation direction
+template <bool Rev = false, typename SeqFn>
 __device__ inline void nextTarget(Nav& nv, const Track& T, const DevSurface* geo,
                                   const DevOpts& o, SeqFn seqAt) {
   nv.tSurf = -1;
   if (nv.navBreak) return;
-  ++nv.seqPos;
-  while (nv.seqPos < nv.seqLen) {
+  const int inc = Rev ? -1 : 1;
+  nv.seqPos += inc;
+  const V3 d = navDir<Rev>(T);
+  while (nv.seqPos >= 0 && nv.seqPos < nv.seqLen) {
     const int sid = seqAt(nv.seqPos);
-    const Isect x = intersect(geo[sid], T.pos, T.dir, o.surfaceTolerance);
+    const Isect x = intersect(geo[sid], T.pos, d, o.surfaceTolerance);
     for (int i = 0; i < x.size; ++i) {
       if (x.status[i] != kUnreachable && x.path[i] > o.nearLimit &&
           x.path[i] < DBL_MAX + kOnSurfaceTolerance) {
@@ -739,19 +775,19 @@ __device__ inline void nextTarget(Nav& nv, const Track& T, const DevSurface* geo
         return;
       }
     }
-    ++nv.seqPos;
+    nv.seqPos += inc;
   }
   nv.navBreak = true;
 }
 
 // Propagator.ipp:49-79
-template <typename SeqFn>
+template <bool Rev = false, typename SeqFn>
 __device__ inline int getNextTarget(Nav& nv, Track& T, const DevSurface* geo,
                                     const DevOpts& o, SeqFn seqAt) {
   for (uint32_t i = 0; i < o.maxTargetSkipping; ++i) {
-    nextTarget(nv, T, geo, o, seqAt);
+    nextTarget<Rev>(nv, T, geo, o, seqAt);
     if (nv.tSurf < 0) return 0;
-    const int st = updateSurfaceStatus(T, geo[nv.tSurf], nv.tIdx, o.surfaceTolerance);
+    const int st = updateSurfaceStatus<Rev>(T, geo[nv.tSurf], nv.tIdx, o.surfaceTolerance);
     if (st == kOnSurface) continue;
     if (st == kReachable) return 0;
   }

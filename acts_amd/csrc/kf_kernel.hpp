@@ -65,6 +65,23 @@ struct Lane {
   bool noTransport;       // stepping without transport since a checkpoint
   bool replaying;         // back at the checkpoint: steps with transport
   bool noSkip;            // replayed once: keeps the transport from now on
+  int startSurf;          // reverse filter: its start surface (else -1)
+};
+
+// Reverse filtering (KalmanFitter.hpp:884-945): the second pass of the
+// forward kernel, backwards from the last measurement state.  `rec` holds
+// per queue slot the reverse filter's last measurement state (kRevComps:
+// filtered parameters, packed covariance, surface) and, if it reached its
+// target, the bound parameters and packed covariance there.
+constexpr int kRevComps = 28, kRevTarget = 28, kRevRecord = 70;
+struct RevArgs {
+  double scale;      // reverseFilteringCovarianceScaling
+  double threshold;  // the reverse-filtering logic's momentum threshold
+  int32_t all;       // KalmanFitterOptions::reverseFiltering
+  int32_t hasTarget; // the reference surface is the reverse filter's target
+  LineRef target;
+  double* rec;       // [slot][kRevRecord]
+  int32_t* reached;  // [slot] 1 if the reverse filter reached its target
 };
 
 struct FwdArgs {
@@ -85,7 +102,50 @@ struct FwdArgs {
   Lane* ckpt;
   int32_t skipAfter;
   int32_t pad;
+  RevArgs rv;
 };
+
+// The reverse filter's actor target check (SurfaceReached::checkAbort on the
+// reference surface) and the end of a reverse pass: kf_reverse.hpp.
+template <bool Sympy>
+__device__ inline bool reverseTargetReached(Lane& L, const FwdArgs& a);
+__device__ inline void reverseFinish(Lane& L, const FwdArgs& a);
+
+// The number of distinct measurement surfaces of a track
+// (KalmanFitter.hpp:750-755, first one wins): a set of the surface indices
+// below 256 in four 64-bit words; pairwise comparison only for a track with
+// a measurement on another index.
+__device__ inline int uniqueMeasurementSurfaces(const DevBatch& in, int64_t m0, int nMeas) {
+  int nUnique = 0;
+  bool wide = false;
+  uint64_t seen[4] = {0, 0, 0, 0};
+  for (int i = 0; i < nMeas; ++i) {
+    const int sm = in.measSurf[m0 + i];
+    wide = wide || sm < 0 || sm >= 256;
+    const uint64_t bit = 1ull << (sm & 63);
+    const int w = (sm >> 6) & 3;
+    bool dup = false;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      dup = dup || (w == k && (seen[k] & bit) != 0);
+      seen[k] |= (w == k) ? bit : 0ull;
+    }
+    nUnique += dup ? 0 : 1;
+  }
+  if (wide) {
+    nUnique = 0;
+    for (int i = 0; i < nMeas; ++i) {
+      const int si_ = in.measSurf[m0 + i];
+      bool dup = false;
+      for (int j = 0; j < i; ++j) {
+        const int sj = in.measSurf[m0 + j];
+        dup = dup || sj == si_;
+      }
+      nUnique += dup ? 0 : 1;
+    }
+  }
+  return nUnique;
+}
 
 // EigenStepper::initialize (EigenStepper.ipp:37-70), DirectNavigator
 // initialisation and the pre-propagation act of the KalmanFitter actor.
@@ -116,6 +176,7 @@ __device__ inline int laneInit(Lane& L, const int64_t q, const int64_t t, const 
   L.noTransport = false;
   L.replaying = false;
   L.noSkip = false;
+  L.startSurf = -1;
   L.T.nSteps = 0;
   L.T.nTrials = 0;
   // surface indices come from the caller: reject the track if any is out of
@@ -136,37 +197,7 @@ __device__ inline int laneInit(Lane& L, const int64_t q, const int64_t t, const 
     L.status = ACTS_KF_STATUS_PROPAGATION_FAILURE;
     return kPhaseFinished;
   }
-  // unique measurement surfaces (KalmanFitter.hpp:750-755, first one wins):
-  // a set of the surface indices below 256 in four 64-bit words; pairwise
-  // comparison only for a track with a measurement on another index
-  int nUnique = 0;
-  bool wide = false;
-  uint64_t seen[4] = {0, 0, 0, 0};
-  for (int i = 0; i < L.nMeas; ++i) {
-    const int sm = in.measSurf[L.m0 + i];
-    wide = wide || sm < 0 || sm >= 256;
-    const uint64_t bit = 1ull << (sm & 63);
-    const int w = (sm >> 6) & 3;
-    bool dup = false;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      dup = dup || (w == k && (seen[k] & bit) != 0);
-      seen[k] |= (w == k) ? bit : 0ull;
-    }
-    nUnique += dup ? 0 : 1;
-  }
-  if (wide) {
-    nUnique = 0;
-    for (int i = 0; i < L.nMeas; ++i) {
-      const int si_ = in.measSurf[L.m0 + i];
-      bool dup = false;
-      for (int j = 0; j < i; ++j) {
-        const int sj = in.measSurf[L.m0 + j];
-        dup = dup || sj == si_;
-      }
-      nUnique += dup ? 0 : 1;
-    }
-  }
+  const int nUnique = uniqueMeasurementSurfaces(in, L.m0, L.nMeas);
   L.nUnique = nUnique;
   L.mNext = 0;
 
@@ -213,12 +244,13 @@ __device__ inline int laneInit(Lane& L, const int64_t q, const int64_t t, const 
   return L.status ? kPhaseFinished : kPhaseStepping;
 }
 
-template <bool Mixed>
+template <bool Mixed, bool Sympy = false, bool Rev = false>
 __device__ inline int laneAfterStep(Lane& L, const FwdArgs& a, const DevSurface* geo);
 
 // One iteration of the propagation loop (Propagator.ipp:25-210): step, target
-// status, the actor's finalisation checks, aborters, next target.
-template <bool Mixed, bool Sympy = false>
+// status, the actor's finalisation checks, aborters, next target.  Rev: the
+// reverse filter, propagating backwards.
+template <bool Mixed, bool Sympy = false, bool Rev = false>
 __device__ inline int laneStep(Lane& L, const FwdArgs& a, const DevSurface* geo) {
   const DevOpts& o = a.o;
   Track& T = L.T;
@@ -226,7 +258,8 @@ __device__ inline int laneStep(Lane& L, const FwdArgs& a, const DevSurface* geo)
     L.status = ACTS_KF_STATUS_STEP_COUNT_LIMIT;
     return kPhaseFinished;
   }
-  if (a.skipAfter > 0 && !L.noTransport && !L.replaying && !L.noSkip &&
+  constexpr double propDir = Rev ? -1.0 : 1.0;
+  if (!Rev && a.skipAfter > 0 && !L.noTransport && !L.replaying && !L.noSkip &&
       L.iter - L.lastSurfIter >= static_cast<uint32_t>(a.skipAfter)) {
     a.ckpt[static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x] = L;
     L.noTransport = true;
@@ -235,30 +268,42 @@ __device__ inline int laneStep(Lane& L, const FwdArgs& a, const DevSurface* geo)
   const bool transport = !L.noTransport;
   int sst;
   if constexpr (Mixed && Sympy)
-    sst = rkStepSympyMixed(T, a.F, o, 1.0, transport);
+    sst = rkStepSympyMixed(T, a.F, o, propDir, transport);
   else if constexpr (Mixed)
-    sst = rkStepMixed(T, a.F, o, 1.0, transport);
+    sst = rkStepMixed(T, a.F, o, propDir, transport);
   else if constexpr (Sympy)
-    sst = rkStepSympy(T, a.F, o, 1.0, transport);
+    sst = rkStepSympy(T, a.F, o, propDir, transport);
   else
-    sst = rkStep(T, a.F, o, 1.0, transport);
+    sst = rkStep(T, a.F, o, propDir, transport);
   if (sst) {
     L.status = sst;
     return kPhaseFinished;
   }
-  return laneAfterStep<Mixed>(L, a, geo);
+  return laneAfterStep<Mixed, Sympy, Rev>(L, a, geo);
+}
+
+// The actor's finalisation checks (KalmanFitter.hpp:395-420): all
+// measurements handled, the path limit, and for the reverse filter its
+// target (each check updates the actor step size, in this order).
+template <bool Rev, bool Sympy>
+__device__ inline bool actorFinished(Lane& L, const FwdArgs& a) {
+  bool fin = (L.nMeasStates == L.nUnique) || pathAbort(L.T, L.kfLimit, a.o.surfaceTolerance);
+  if constexpr (Rev) {
+    if (a.rv.hasTarget) fin = reverseTargetReached<Sympy>(L, a) || fin;
+  }
+  return fin;
 }
 
 // The rest of the propagation-loop iteration after an accepted step: target
 // status, the actor's finalisation checks, aborters, next target.
-template <bool Mixed>
+template <bool Mixed, bool Sympy, bool Rev>
 __device__ inline int laneAfterStep(Lane& L, const FwdArgs& a, const DevSurface* geo) {
   const DevOpts& o = a.o;
   Track& T = L.T;
   T.cNav = kNotSet;
   T.cActor = kNotSet;
   if (L.nv.tSurf >= 0) {
-    const int ps = updateSurfaceStatus(T, geo[L.nv.tSurf], L.nv.tIdx, o.surfaceTolerance);
+    const int ps = updateSurfaceStatus<Rev>(T, geo[L.nv.tSurf], L.nv.tIdx, o.surfaceTolerance);
     if (ps == kOnSurface && !L.nv.navBreak) L.reached = L.nv.tSurf;
     if (ps != kReachable) L.nv.tSurf = -1;
   }
@@ -266,20 +311,26 @@ __device__ inline int laneAfterStep(Lane& L, const FwdArgs& a, const DevSurface*
   // act without a current surface: only the finalisation checks
   if (!L.finished) {
     if (L.kfLimit == DBL_MAX) loopProtection<Mixed>(T, a.F, o, L.kfLimit, true);
-    L.finished = (L.nMeasStates == L.nUnique) || pathAbort(T, L.kfLimit, o.surfaceTolerance);
+    L.finished = actorFinished<Rev, Sympy>(L, a);
   }
   if (L.finished || pathAbort(T, L.propLimit, o.surfaceTolerance)) return kPhaseFinished;
   if (L.nv.tSurf < 0) {
     T.cNav = kNotSet;
-    L.status = getNextTarget(L.nv, T, geo, o, L.seq);
+    L.status = getNextTarget<Rev>(L.nv, T, geo, o, L.seq);
     if (L.status) return kPhaseFinished;
   }
   return kPhaseStepping;
 }
 
 // KalmanFitter::Actor::filter on the reached surface (KalmanFitter.hpp:
-// 339-671), then the rest of the propagation-loop iteration.
-template <bool Mixed, bool Sympy = false>
+// 339-671), then the rest of the propagation-loop iteration.  The forward
+// filter writes each state's record; the reverse filter (Rev) writes none
+// and keeps only its last measurement update (RevArgs::rec).  Material is
+// split by direction (ISurfaceMaterial.cpp:18-31, split factor 1): forwards
+// after the update, backwards before it; on the reverse filter's start
+// surface only post-update material counts, which backwards is none
+// (PointwiseMaterialInteraction.cpp:16-29).
+template <bool Mixed, bool Sympy = false, bool Rev = false>
 __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* geo,
                                  const DevSlab* slabs) {
   const DevBatch& in = a.in;
@@ -299,7 +350,8 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
     // come in the order of the surface sequence, it is the one after the
     // last match
     int mi = -1;
-    if (L.nUnique == L.nMeas && L.mNext < L.nMeas && in.measSurf[L.m0 + L.mNext] == reached) {
+    if (!Rev && L.nUnique == L.nMeas && L.mNext < L.nMeas &&
+        in.measSurf[L.m0 + L.mNext] == reached) {
       mi = L.mNext;
     } else {
       for (int i = 0; i < L.nMeas; ++i) {
@@ -309,6 +361,7 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
     }
     if (mi >= 0) L.mNext = mi + 1;
     const bool hasMat = s.mat >= 0;
+    const bool matHere = hasMat && !(Rev && reached == L.startSurf);
     const bool sensitive = (s.flags & ACTS_KF_SURFACE_SENSITIVE) != 0;
     const bool preceding = L.nMeasStates > 0;
     if (mi >= 0 || (preceding && sensitive) || hasMat) {
@@ -318,9 +371,15 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
       transportToBound<Sympy>(T, s, onS, J);
       if (!onS) {
         L.status = ACTS_KF_STATUS_NOT_ON_SURFACE;
-      } else if (L.nStates >= sc.maxStates) {
+      } else if (!Rev && L.nStates >= sc.maxStates) {
         L.status = ACTS_KF_STATUS_PROPAGATION_FAILURE;
       } else {
+        // going backwards, a measurement surface's material comes before
+        // its update (the predicted state includes it)
+        if (Rev && mi >= 0 && matHere)
+          L.status = materialInteraction(T, s, slabs[s.mat], a.pc, o, -1.0);
+      }
+      if (!L.status) {
         // bound parameters (transformFreeToBoundParameters)
         double x[6];
         x[0] = l0;
@@ -332,13 +391,15 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
         const int k = L.nStates;
         uint32_t flags = ACTS_KF_STATE_HAS_PARAMETERS;
         if (hasMat) flags |= ACTS_KF_STATE_HAS_MATERIAL;
+        if constexpr (!Rev) {
 #pragma unroll
-        for (int i = 0; i < 6; ++i) sd(sc, k, kPred + i, q) = x[i];
+          for (int i = 0; i < 6; ++i) sd(sc, k, kPred + i, q) = x[i];
 #pragma unroll
-        for (int i = 0; i < 21; ++i) sd(sc, k, kPCov + i, q) = T.cov[i];
+          for (int i = 0; i < 21; ++i) sd(sc, k, kPCov + i, q) = T.cov[i];
 #pragma unroll
-        for (int i = 0; i < 36; ++i) sd(sc, k, kJac + i, q) = J[i];
-        sd(sc, k, kPath, q) = T.pathAcc;
+          for (int i = 0; i < 36; ++i) sd(sc, k, kJac + i, q) = J[i];
+          sd(sc, k, kPath, q) = T.pathAcc;
+        }
         float chi2f = 0.f;
         bool measured = false;
         if (mi >= 0) {
@@ -355,10 +416,20 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
             } else {
               measured = true;
               chi2f = static_cast<float>(chi2);
+              if constexpr (Rev) {
+                // the reverse filter's latest measurement state
+                double* r = a.rv.rec + kRevRecord * q;
 #pragma unroll
-              for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, q) = f[i];
+                for (int i = 0; i < 6; ++i) r[i] = f[i];
 #pragma unroll
-              for (int i = 0; i < 21; ++i) sd(sc, k, kFCov + i, q) = C[i];
+                for (int i = 0; i < 21; ++i) r[6 + i] = C[i];
+                r[27] = static_cast<double>(reached);
+              } else {
+#pragma unroll
+                for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, q) = f[i];
+#pragma unroll
+                for (int i = 0; i < 21; ++i) sd(sc, k, kFCov + i, q) = C[i];
+              }
               // EigenStepper::update with the filtered parameters
               const double sinTh = sin(f[3]);
               T.dir = mk(cos(f[2]) * sinTh, sin(f[2]) * sinTh, cos(f[3]));
@@ -384,33 +455,44 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
           flags |= ACTS_KF_STATE_IS_HOLE;
         }
         if (!L.status) {
-          if (!measured) {
+          if constexpr (!Rev) {
+            if (!measured) {
 #pragma unroll
-            for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, q) = x[i];
+              for (int i = 0; i < 6; ++i) sd(sc, k, kFilt + i, q) = x[i];
 #pragma unroll
-            for (int i = 0; i < 21; ++i) sd(sc, k, kFCov + i, q) = T.cov[i];
+              for (int i = 0; i < 21; ++i) sd(sc, k, kFCov + i, q) = T.cov[i];
+            }
+            sd(sc, k, kChi2, q) = chi2f;
+            si(sc, k, kISurf, q) = reached;
+            si(sc, k, kIFlags, q) = static_cast<int32_t>(flags);
+            si(sc, k, kIMeas, q) = mi;
           }
-          sd(sc, k, kChi2, q) = chi2f;
-          si(sc, k, kISurf, q) = reached;
-          si(sc, k, kIFlags, q) = static_cast<int32_t>(flags);
-          si(sc, k, kIMeas, q) = mi;
           L.nStates = k + 1;
           if (mi >= 0) L.tip = k;
-          if (hasMat) L.status = materialInteraction(T, s, slabs[s.mat], a.pc, o);
+          // after the update going forwards; a material-only or hole state
+          // takes the full material in either direction
+          if (matHere && (!Rev || mi < 0))
+            L.status = materialInteraction(T, s, slabs[s.mat], a.pc, o, Rev ? -1.0 : 1.0);
         }
       }
     }
     if (L.status) return kPhaseFinished;
-    L.finished = (L.nMeasStates == L.nUnique) || pathAbort(T, L.kfLimit, o.surfaceTolerance);
+    L.finished = actorFinished<Rev, Sympy>(L, a);
   }
   if (L.finished || pathAbort(T, L.propLimit, o.surfaceTolerance)) return kPhaseFinished;
   T.cNav = kNotSet;
-  L.status = getNextTarget(L.nv, T, geo, o, L.seq);
+  L.status = getNextTarget<Rev>(L.nv, T, geo, o, L.seq);
   return L.status ? kPhaseFinished : kPhaseStepping;
 }
 
-// End of the forward pass of a track: counters and the smoother's inputs.
+// End of the forward pass of a track: counters and the smoother's inputs;
+// of the reverse pass: reverseFinish.
+template <bool Rev = false>
 __device__ inline void laneFinish(Lane& L, const FwdArgs& a) {
+  if constexpr (Rev) {
+    reverseFinish(L, a);
+    return;
+  }
   const int64_t t = L.t;
   int status = L.status;
   if (a.out.nSteps) a.out.nSteps[t] = L.T.nSteps;
@@ -678,11 +760,14 @@ __device__ inline void mbfUpdate(int64_t q, int64_t t, int k, uint32_t flags, co
   for (int i = 0; i < 6; ++i) l[i] = ln[i];
 }
 
-// Mbf = false: GainMatrixSmoother (RTS); true: MbfSmoother.
+// Mbf = false: GainMatrixSmoother (RTS); true: MbfSmoother.  `rev`: the
+// track is filtered in reverse instead of smoothed (KalmanFitter.hpp:897-
+// 945); its smoothed slots are NaN here, the reverse pass fills the first
+// and last measurement states.
 template <bool Mbf>
 __device__ inline void smoothTrack(const int64_t q, const int64_t t, const bool valid,
                                    const DevBatch& in,
-                                   const DevOut& out, const DevScratch& sc) {
+                                   const DevOut& out, const DevScratch& sc, const bool rev) {
   int status = valid ? sc.track[q] : 0;
   const int tip = (!valid || status) ? -1 : sc.track[sc.stride + q];
   // with a reference surface: keep the smoothed first and last measurement
@@ -713,7 +798,15 @@ __device__ inline void smoothTrack(const int64_t q, const int64_t t, const bool 
     for (int i = 0; i < 6; ++i) xf[i] = sd(sc, k, kFilt + i, q);
 #pragma unroll
     for (int i = 0; i < 21; ++i) Cf[i] = sd(sc, k, kFCov + i, q);
-    if constexpr (Mbf) {
+    if (rev) {
+      const double nan = __builtin_nan("");
+#pragma unroll
+      for (int i = 0; i < 6; ++i) xs[i] = nan;
+#pragma unroll
+      for (int i = 0; i < 21; ++i) Cs[i] = nan;
+#pragma unroll
+      for (int i = 0; i < 36; ++i) CsFull[i] = nan;
+    } else if constexpr (Mbf) {
       mbfSmoothed(xf, Cf, L, l, xs, CsFull);
 #pragma unroll
       for (int i = 0; i < 6; ++i)
@@ -739,7 +832,7 @@ __device__ inline void smoothTrack(const int64_t q, const int64_t t, const bool 
         continue;
       }
     }
-    if (k == firstMeas || k == lastMeas) {
+    if (!rev && (k == firstMeas || k == lastMeas)) {
       double* r = sc.ref + kRefComps * q + (k == firstMeas ? 0 : 27);
 #pragma unroll
       for (int i = 0; i < 6; ++i) r[i] = xs[i];
@@ -792,7 +885,7 @@ __device__ inline void smoothTrack(const int64_t q, const int64_t t, const bool 
       out.stateMeas[slot] = mi >= 0 ? static_cast<int32_t>(in.measOff[t] + mi) : -1;
     }
     if constexpr (Mbf) {
-      if (k > 0) mbfUpdate(q, t, k, flags, in, sc, L, l);
+      if (k > 0 && !rev) mbfUpdate(q, t, k, flags, in, sc, L, l);
     } else {
 #pragma unroll
       for (int i = 0; i < 6; ++i) xsN[i] = xs[i];
@@ -814,6 +907,7 @@ __device__ inline void smoothTrack(const int64_t q, const int64_t t, const bool 
   if (out.nMeas) out.nMeas[t] = status ? 0 : nMeasT;
   if (out.nHoles) out.nHoles[t] = status ? 0 : nHoles;
   if (out.nOutliers) out.nOutliers[t] = status ? 0 : nOut;
+  if (out.reversed) out.reversed[t] = 0;
 }
 
 }  // namespace kf

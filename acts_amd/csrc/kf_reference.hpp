@@ -18,24 +18,6 @@
 
 namespace kf {
 
-struct LineRef {
-  double c[3];  // a point of the line; its axis is the global z axis
-};
-
-// LineSurface::intersect (LineSurface.cpp:136-186) without bounds
-__device__ inline int lineIntersect(const LineRef& L, const V3& pos, const V3& dir,
-                                    double tol, double& path) {
-  const V3 eb = mk(0., 0., 1.);
-  const V3 mab = mk(L.c[0] - pos.x, L.c[1] - pos.y, L.c[2] - pos.z);
-  const double eaTeb = dot(dir, eb);
-  const double denom = 1 - eaTeb * eaTeb;
-  path = INFINITY;
-  if (fabs(denom) < fabs(tol)) return kUnreachable;
-  const double u = (dot(mab, dir) - dot(mab, eb) * eaTeb) / denom;
-  path = u;
-  return fabs(u) > fabs(tol) ? kReachable : kOnSurface;
-}
-
 // LineSurface::referenceFrame (LineSurface.cpp:101-114): columns
 // (unitD0, unitZ0, unitDistance)
 __device__ inline void lineFrame(const V3& dir, V3& d0, V3& z0, V3& dist) {

@@ -157,6 +157,34 @@ def double_layer_detector(radii=BARREL_RADII, half_length=3000.0, disc_z=None,
     return det
 
 
+RAGGED_RADII = (32.0, 52.0, 72.0, 94.0, 116.0, 144.0, 172.0, 216.0, 260.0, 310.0, 360.0,
+                430.0, 500.0, 660.0, 820.0)
+
+
+def ragged_detector(radii=RAGGED_RADII, half_length=1000.0, gap=2.0,
+                    disc_z=(1200.0, 1800.0, 2400.0), disc_rmin=30.0, disc_rmax=900.0,
+                    thickness=0.3) -> Detector:
+    """The ragged-batching detector: 15 barrel double layers (a second
+    sensitive cylinder `gap` mm outside each, r = 32..822 mm) of half-length
+    1 m, and 2 x 3 endcap discs at |z| = 1.2, 1.8, 2.4 m (r = 30..900 mm).
+    A central track crosses all 30 cylinders; the further forward, the
+    fewer barrel layers and the more discs it crosses, down to a few
+    surfaces at |eta| = 3.5: with 5 % holes and the 6-measurement minimum,
+    tracks carry 6 to 30 measurements (mean ~20)."""
+    det = Detector(name="ragged: 15 double-layer barrel + 2x3 discs")
+    mi = det.add_material(silicon(), thickness)
+    for r in radii:
+        for dr in (0.0, gap):
+            det.surfaces.append(SurfaceDesc(capi.SURFACE_CYLINDER, np.eye(3), np.zeros(3),
+                                            (r + dr, half_length), mi, True))
+    for side in (-1.0, 1.0):
+        for z in disc_z:
+            det.surfaces.append(SurfaceDesc(capi.SURFACE_DISC, np.eye(3),
+                                            np.array([0.0, 0.0, side * z]),
+                                            (disc_rmin, disc_rmax), mi, True))
+    return det
+
+
 # measurement kinds of a telescope plane: (dimension, bound indices, sigmas)
 PIXEL = (2, (0, 1), (25 * 1e-3, 50 * 1e-3))
 STRIP0 = (1, (0, 0), (100 * 1e-3,))

@@ -18,7 +18,7 @@ import numpy as np
 
 from . import capi
 from .detector import (ConstantField, Detector, FieldMap, MUON_MASS, NS, UM,
-                       barrel_detector, barrel_endcap_detector, double_layer_detector,
+                       barrel_detector, barrel_endcap_detector, double_layer_detector, ragged_detector,
                        telescope_detector)
 
 
@@ -207,10 +207,12 @@ WORKLOADS = {
     # and length 6 m (100 coils)
     "solenoid": dict(detector=barrel_endcap_detector, n_tracks=500000, eta=(-2.5, 2.5),
                      pt=(0.5, 100.0), pt_log=True, field="solenoid"),
-    # 2,000,000 muons with 14-27 measurements: barrel layers doubled (up to
-    # 27 crossings with the discs), 5 % hole probability per crossed layer,
-    # 10 % of the tracks with 1-3 outliers at 20 sigma (rejected by a chi2
-    # cut of 25), uniform Bz = 2 T.  A track with a rejected outlier never
+    # 2,000,000 muons with 6-30 measurements (ragged_detector: 15 barrel
+    # double layers of half-length 1 m and 2x3 discs, |eta| < 3.5, so the
+    # number of crossed layers depends on eta), 5 % hole probability per
+    # crossed layer, at least 6 measurements, 10 % of the tracks with 1-3
+    # outliers at 20 sigma (rejected by a chi2 cut of 25), pT log-uniform
+    # 0.5-100 GeV, uniform Bz = 2 T.  A track with a rejected outlier never
     # counts all its measurements, so with the direct navigator it only ends
     # at the loop-protection path limit (half a helix turn); in a finite field
     # map it would end with FIELD_OUT_OF_BOUNDS instead, as in the reference.
@@ -219,7 +221,7 @@ WORKLOADS = {
     # planes), pT 0.5-10 GeV, |eta| < 0.1, |phi| < 0.1, no field
     "telescope": dict(detector=telescope_detector, n_tracks=10000, eta=(-0.1, 0.1),
                       phi=(-0.1, 0.1), pt=(0.5, 10.0), field="zero"),
-    "ragged": dict(detector=double_layer_detector, n_tracks=2_000_000, eta=(-2.5, 2.5),
+    "ragged": dict(detector=ragged_detector, n_tracks=2_000_000, eta=(-3.5, 3.5),
                    pt=(0.5, 100.0), pt_log=True, hole_probability=0.05,
                    outlier_fraction=0.1, min_measurements=6,
                    options=dict(outlier_chi2_cut=25.0)),

@@ -484,7 +484,7 @@ def main() -> int:
         from acts_amd.autotune import autotune
         tuned, _ = autotune(det, field, events, wl_opts, cache_path="")
     elif args.input_order:
-        tuned = {"length_sorted": False}
+        tuned = {"length_sorted": 0}
     b = Bench(args, det, field, events, wl_opts, tuned)
 
     ms64, wall64, kt64, out64 = b.run(capi.PRECISION_FP64)

@@ -6,7 +6,7 @@ from acts_amd.autotune import DEFAULTS, SPACE, coordinate_descent, signature
 
 def test_coordinate_descent_finds_separable_minimum():
     target = {"sched_inner_steps": SPACE["sched_inner_steps"][3],
-              "sched_wait_64ths": SPACE["sched_wait_64ths"][2], "length_sorted": False,
+              "sched_wait_64ths": SPACE["sched_wait_64ths"][2], "length_sorted": 3,
               "sched_quarantine_steps": SPACE["sched_quarantine_steps"][2],
               "sched_transport_skip_steps": SPACE["sched_transport_skip_steps"][1]}
     calls = []
@@ -15,7 +15,7 @@ def test_coordinate_descent_finds_separable_minimum():
         calls.append(cfg)
         c = abs(SPACE["sched_inner_steps"].index(cfg["sched_inner_steps"]) - 3)
         c += abs(SPACE["sched_wait_64ths"].index(cfg["sched_wait_64ths"]) - 2)
-        c += 0 if cfg["length_sorted"] is False else 1
+        c += 0 if cfg["length_sorted"] == 3 else 1
         c += abs(SPACE["sched_quarantine_steps"].index(cfg["sched_quarantine_steps"]) - 2)
         c += abs(SPACE["sched_transport_skip_steps"].index(cfg["sched_transport_skip_steps"]) - 1)
         return 1.0 + c
