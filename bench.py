@@ -81,8 +81,9 @@ def parse():
                     help="queue tracks in input order instead of longest first")
     ap.add_argument("--serving-seconds", type=float, default=2.0,
                     help="length of each loaded event-serving run (0 = skip serving)")
-    ap.add_argument("--autotune", action="store_true",
-                    help="tune the scheduling knobs on the workload before timing")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="time the library's default scheduling knobs instead of tuning "
+                         "them (per precision, on a 100,000-track sample of the workload)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (for the "
                          "HBM-counter calibration of rocprof runs)")
@@ -345,6 +346,8 @@ class Bench:
     """Timing of one workload's fits on the device-resident batch."""
 
     def __init__(self, args, det, field, events, wl_opts, tuned):
+        """`tuned`: scheduling knobs, one dict for every precision or a dict
+        of them keyed by precision."""
         import torch
         from acts_amd.device import DeviceEvents
         self.args = args
@@ -365,8 +368,10 @@ class Bench:
         streams = streams or self.args.streams
         steps = steps or self.args.steps
         n = self.events.n_tracks
+        tuned = self.tuned.get(precision, {}) if self.tuned and all(
+            isinstance(v, dict) for v in self.tuned.values()) else self.tuned
         opts = dataclasses.replace(self.wl_opts, precision=precision, kernel_timing=True,
-                                   **self.tuned, **(extra or {}))
+                                   **tuned, **(extra or {}))
         outs = [DeviceResult(n, self.dev_in.n_slots, outputs, self.dev) for _ in range(streams)]
         fitters = [A.KalmanFitter(self.det, self.field, opts) for _ in range(streams)]
         ms1, wall, kt = timed_fits(fitters[0], self.bin, outs[0].batch_out(), self.stream,
@@ -480,11 +485,20 @@ def main() -> int:
     log(f"{n} tracks generated")
     wl_opts = A.workload_options(args.workload)
     tuned = {}
-    if args.autotune:
-        from acts_amd.autotune import autotune
-        tuned, _ = autotune(det, field, events, wl_opts, cache_path="")
-    elif args.input_order:
+    if args.input_order:
         tuned = {"length_sorted": 0}
+    elif args.autotune:
+        # the autotuner's coordinate descent over the scheduling knobs, per
+        # precision, on the first 100,000 tracks; the timed fits use them
+        from acts_amd.autotune import autotune
+        sample = events.subset(np.arange(min(n, 100_000)))
+        t0 = time.perf_counter()
+        tuned = {}
+        for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
+            tuned[prec], _ = autotune(det, field, sample,
+                                      dataclasses.replace(wl_opts, precision=prec),
+                                      cache_path="")
+        log(f"autotuned in {time.perf_counter() - t0:.1f} s: {tuned}")
     b = Bench(args, det, field, events, wl_opts, tuned)
 
     ms64, wall64, kt64, out64 = b.run(capi.PRECISION_FP64)
@@ -590,6 +604,14 @@ def main() -> int:
             "first_state_rel_err_vs_rts": quality.relative_error(
                 quality.first_states(outb, events), first64)}
         del outb
+        msr, _, ktr, outr = b.run(capi.PRECISION_FP64, {"reverse_filtering": True})
+        line["with_reverse_filtering"] = {
+            "value": n / (msr * 1e-3), "ms_per_step": msr,
+            "note": "every track filtered in reverse instead of smoothed "
+                    "(KalmanFitterOptions::reverseFiltering)",
+            "reversed_tracks": int(np.count_nonzero(outr.host("reversed"))),
+            "fit_failures": int(np.count_nonzero(outr.host("status")))}
+        del outr
         mss, _, kts, outs_ = b.run(capi.PRECISION_FP64, {"stepper": capi.STEPPER_SYMPY})
         line["with_sympy_stepper"] = {
             "value": n / (mss * 1e-3), "ms_per_step": mss,
@@ -620,7 +642,16 @@ def main() -> int:
         log("variants done")
 
     if tuned:
-        line["config"]["scheduling"] = tuned
+        line["config"]["scheduling"] = (
+            {"fp64": tuned[capi.PRECISION_FP64], "mixed": tuned[capi.PRECISION_MIXED],
+             "tuned_on": f"first {min(n, 100_000)} tracks (acts_amd.autotune)"}
+            if capi.PRECISION_FP64 in tuned else tuned)
+        b.tuned = {}
+        msd, _, _, _ = b.run(capi.PRECISION_FP64)
+        b.tuned = tuned
+        line["default_scheduling"] = {
+            "value": n / (msd * 1e-3), "ms_per_step": msd,
+            "note": "FP64 with the library's default scheduling knobs"}
 
     if not args.no_endcap and args.workload != "endcap":
         line["endcap"] = endcap_section(args)
