@@ -23,6 +23,7 @@ ERR_UNSUPPORTED = -4
 
 CAT_KALMAN, CAT_PROPAGATOR, CAT_STEPPER, CAT_SURFACE, CAT_FIELD, CAT_EXTRAPOLATION = (
     0x100, 0x200, 0x300, 0x400, 0x500, 0x600)
+CAT_LIBRARY = 0x700
 STATUS_OK = 0
 STATUS_UPDATE_FAILED = CAT_KALMAN | 1
 STATUS_SMOOTH_FAILED = CAT_KALMAN | 2
@@ -39,6 +40,7 @@ STATUS_NOT_ON_SURFACE = CAT_SURFACE | 1
 STATUS_FIELD_OUT_OF_BOUNDS = CAT_FIELD | 1
 STATUS_NO_COMPATIBLE_STATE = CAT_EXTRAPOLATION | 1
 STATUS_REFERENCE_UNREACHABLE = CAT_EXTRAPOLATION | 2
+STATUS_STATE_CAPACITY = CAT_LIBRARY | 1
 
 STATE_HAS_MEASUREMENT = 1 << 0
 STATE_HAS_PARAMETERS = 1 << 1
@@ -50,6 +52,7 @@ SURFACE_PLANE, SURFACE_CYLINDER, SURFACE_DISC, SURFACE_LINE = 0, 1, 2, 3
 REFERENCE_NONE, REFERENCE_PERIGEE = 0, 1
 EXTRAPOLATE_FIRST, EXTRAPOLATE_LAST, EXTRAPOLATE_FIRST_OR_LAST = 0, 1, 2
 SURFACE_SENSITIVE = 1
+NAVIGATOR_DIRECT, NAVIGATOR_LAYERS = 0, 1
 
 FIELD_CONSTANT, FIELD_GRID_XYZ = 0, 1
 SMOOTHER_GAIN_MATRIX, SMOOTHER_MBF = 0, 1
@@ -139,7 +142,7 @@ class Options(C.Structure):
         ("kernel_timing", C.c_int32),
         ("use_graph", C.c_int32),
         ("sched_transport_skip_steps", C.c_int32),
-        ("reserved_sched", C.c_int32),
+        ("navigator", C.c_int32),
         ("reverse_filtering", C.c_int32),
         ("reserved_reverse", C.c_int32),
         ("reverse_filtering_cov_scaling", C.c_double),

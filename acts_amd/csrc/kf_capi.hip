@@ -648,6 +648,9 @@ const char* acts_kf_status_string(int32_t s) {
       return "Global to local transformation failed: position not on surface.";
     case ACTS_KF_STATUS_FIELD_OUT_OF_BOUNDS:
       return "The lookup position was outside of the validity domain of the field";
+    case ACTS_KF_STATUS_NO_COMPATIBLE_STATE: return "No compatible track state found";
+    case ACTS_KF_STATUS_REFERENCE_UNREACHABLE: return "Reference surface is unreachable";
+    case ACTS_KF_STATUS_STATE_CAPACITY: return "More track states than the track has slots";
     default: return "unknown";
   }
 }
@@ -794,6 +797,12 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   o.absPdg = options->abs_pdg;
   o.mass = options->mass;
   o.absQ = options->abs_charge;
+  if (options->navigator != ACTS_KF_NAVIGATOR_DIRECT &&
+      options->navigator != ACTS_KF_NAVIGATOR_LAYERS) {
+    acts_kf_destroy(c);
+    return fail(ACTS_KF_ERR_INVALID_ARGUMENT, "acts_kf_create: bad navigator");
+  }
+  o.navigator = options->navigator;
   c->pc = makePhysConst();
   c->mbf = options->smoother == ACTS_KF_SMOOTHER_MBF;
   c->sympy = options->stepper == ACTS_KF_STEPPER_SYMPY;
@@ -836,7 +845,9 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   if (options->sched_inner_steps > 0) c->innerSteps = options->sched_inner_steps;
   if (options->sched_wait_64ths > 0) c->waitFrac = std::min(options->sched_wait_64ths, 64);
   c->quarantine = std::max(options->sched_quarantine_steps, 0);
-  c->skipAfter = std::max(options->sched_transport_skip_steps, 0);
+  // the layer navigator keeps its resolved list in device memory, which a
+  // transport-skip checkpoint would not restore: no skipping with it
+  c->skipAfter = o.navigator ? 0 : std::max(options->sched_transport_skip_steps, 0);
   c->revAll = options->reverse_filtering != 0;
   c->revScale = options->reverse_filtering_cov_scaling;
   c->revThreshold = options->reverse_filtering_momentum_threshold;
@@ -904,13 +915,20 @@ struct WorkspaceSizes {
   size_t rev = 0;  // reverse filtering: records and reached flags
 };
 
+// 64-bit words of the layer navigator's reached-layer mask per track.
+static int64_t navMaskWords(const acts_kf_ctx* c) { return (c->nSurfaces + 63) / 64; }
+
 WorkspaceSizes workspaceSizes(const acts_kf_ctx* c, int64_t n, int32_t maxLen) {
   WorkspaceSizes w;
   w.states = std::max(maxLen, 1);
   w.stride = (n + kBlock - 1) / kBlock * kBlock;
   w.d = sizeof(double) * static_cast<size_t>(w.states) * kScratchComps * w.stride;
   w.i = sizeof(int32_t) * static_cast<size_t>(w.states) * kIScratchComps * w.stride;
-  w.track = sizeof(int32_t) * 4 * w.stride;
+  // per track: status, tip, first and last measurement; the layer
+  // navigator's record after them (kNavComps doubles per slot, then one
+  // 64-bit word of its reached-layer mask per 64 surfaces)
+  w.track = sizeof(int32_t) * 4 * w.stride +
+            (c->opts.navigator ? sizeof(double) * (kNavComps + navMaskWords(c)) * w.stride : 0);
   w.withRef = c->refSurface != ACTS_KF_REFERENCE_NONE;
   w.ref = w.withRef ? sizeof(double) * kRefComps * static_cast<size_t>(n) : 0;
   w.queue = sizeof(int32_t);
@@ -1043,6 +1061,8 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   sc.d = ws.d;
   sc.i = ws.i;
   sc.track = ws.track;
+  sc.nav = c->opts.navigator ? reinterpret_cast<double*>(ws.track + 4 * stride) : nullptr;
+  sc.navMask = sc.nav ? reinterpret_cast<uint64_t*>(sc.nav + kNavComps * stride) : nullptr;
   sc.stride = stride;
   sc.maxStates = states;
   const int64_t blocks = (n + kBlock - 1) / kBlock;

@@ -19,6 +19,7 @@ struct DevOpts {
   int32_t loopProtection, multipleScattering, energyLoss, useJoseph;
   int32_t absPdg;
   float mass, absQ;
+  int32_t navigator;  // ACTS_KF_NAVIGATOR_*
 };
 
 // Batch pointers on the device.
@@ -48,6 +49,9 @@ constexpr int kISurf = 0, kIFlags = 1, kIMeas = 2, kIScratchComps = 3;
 // per track with a reference surface: smoothed parameters (6) and packed
 // covariance (21) of the first, then of the last measurement state
 constexpr int kRefComps = 54;
+// layer navigator record: the position and navigation direction its layer
+// list was resolved from, and the path length of the current target
+constexpr int kNavComps = 7;
 
 struct DevScratch {
   double* d;
@@ -56,6 +60,9 @@ struct DevScratch {
                    // first and last measurement (not outlier) state index
   double* ref;     // per track with a reference surface: the smoothed
                    // first and last measurement states (null: none)
+  double* nav;     // layer navigator: per queue slot, component j at
+                   // nav[j * stride + q] (null: direct navigator)
+  uint64_t* navMask;  // and its reached layers, word w at navMask[w * stride + q]
   int64_t stride;  // >= n, multiple of 64
   int32_t maxStates;
 };
@@ -122,6 +129,42 @@ KF_HD Isect intersect(const DevSurface& s, const V3& pos,
       dot(n, mk(s.t[0] - pos.x, s.t[1] - pos.y, s.t[2] - pos.z)) / denom;
   r.path[0] = path;
   r.status[0] = fabs(path) < fabs(tol) ? kOnSurface : kReachable;
+  return r;
+}
+
+// The bounds check of Surface::intersect with BoundaryTolerance::None at
+// the intersection point x: full-azimuth cylinders compare the local z with
+// the half length plus the tolerance (CylinderSurface.cpp:242-257), full
+// discs the squared distance from the centre with rmin^2 and rmax^2
+// (DiscSurface.cpp:286-293), planes the local x and y with the half lengths
+// of their RectangleBounds (inclusive; b0 = b1 = 0: no bounds).
+KF_HD bool insideNavBounds(const DevSurface& s, const V3& x, double tol) {
+  const V3 rel = mk(x.x - s.t[0], x.y - s.t[1], x.z - s.t[2]);
+  if (s.type == ACTS_KF_SURFACE_CYLINDER) {
+    const double cz = dot(rel, col(s.R, 2));
+    return fabs(cz) < fabs(s.b1 + tol);
+  }
+  if (s.type == ACTS_KF_SURFACE_DISC) {
+    const double r2 = dot(rel, rel);
+    return r2 >= s.b0 * s.b0 && r2 <= s.b1 * s.b1;
+  }
+  if (s.type == ACTS_KF_SURFACE_PLANE && s.b0 > 0 && s.b1 > 0) {
+    const double lx = dot(rel, col(s.R, 0)), ly = dot(rel, col(s.R, 1));
+    return -s.b0 <= lx && lx <= s.b0 && -s.b1 <= ly && ly <= s.b1;
+  }
+  return true;
+}
+
+// Surface::intersect with BoundaryTolerance::None: solutions outside the
+// bounds are unreachable.
+KF_HD Isect intersectBounded(const DevSurface& s, const V3& pos, const V3& dir, double tol) {
+  Isect r = intersect(s, pos, dir, tol);
+  for (int i = 0; i < r.size; ++i) {
+    if (r.status[i] == kUnreachable) continue;
+    const V3 x = mk(pos.x + r.path[i] * dir.x, pos.y + r.path[i] * dir.y,
+                    pos.z + r.path[i] * dir.z);
+    if (!insideNavBounds(s, x, tol)) r.status[i] = kUnreachable;
+  }
   return r;
 }
 

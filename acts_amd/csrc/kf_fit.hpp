@@ -721,12 +721,33 @@ __device__ inline int lineIntersect(const LineRef& L, const V3& pos, const V3& d
 // navigation: DirectNavigator.hpp:280-390, SteppingHelper.hpp:40-78
 // ---------------------------------------------------------------------------
 struct Nav {
-  int seqPos;   // DirectNavigator::State::surfaceIndex
-  int seqLen;
+  int seqPos;   // DirectNavigator::State::surfaceIndex; layer navigator: the
+                // last target taken from the resolved list (-1: none yet,
+                // kNavUnresolved: no list)
+  int seqLen;   // the track's sequence length (its state slots)
   bool navBreak;
   int tSurf;    // current target surface (-1 none)
   int tIdx;     // intersection index of the target
 };
+constexpr int kNavUnresolved = -2;
+
+// Where a lane's layer-navigator record lives (DevScratch::nav and navMask
+// of its queue slot) and how many surfaces the geometry has.
+struct LayerCtx {
+  double* rec;
+  uint64_t* mask;
+  int64_t stride;
+  int32_t nSurfaces;
+};
+
+// The layers a track has reached (and its start surface): bit s of word
+// s / 64 of its mask.
+__device__ inline void markReached(const LayerCtx& lc, int s) {
+  lc.mask[(s >> 6) * lc.stride] |= uint64_t{1} << (s & 63);
+}
+__device__ inline void clearReached(const LayerCtx& lc) {
+  for (int w = 0; w < (lc.nSurfaces + 63) / 64; ++w) lc.mask[w * lc.stride] = 0;
+}
 
 // The direction the navigation intersects along: the track's direction
 // times the propagation direction (backward for the reverse filter).
@@ -735,10 +756,14 @@ __device__ inline V3 navDir(const Track& T) {
   return Rev ? mk(-T.dir.x, -T.dir.y, -T.dir.z) : T.dir;
 }
 
+// updateSingleSurfaceStatus (SteppingHelper.hpp:40-78): the direct
+// navigator's targets with BoundaryTolerance::Infinite, the layer
+// navigator's (bounded) with BoundaryTolerance::None.
 template <bool Rev = false>
 __device__ inline int updateSurfaceStatus(Track& T, const DevSurface& s, int idx,
-                                          double tol) {
-  const Isect x = intersect(s, T.pos, navDir<Rev>(T), tol);
+                                          double tol, bool bounded = false) {
+  const V3 d = navDir<Rev>(T);
+  const Isect x = bounded ? intersectBounded(s, T.pos, d, tol) : intersect(s, T.pos, d, tol);
   const double path = idx ? x.path[1] : x.path[0];
   const int st = idx ? x.status[1] : x.status[0];
   if (st == kOnSurface) {
@@ -780,14 +805,118 @@ __device__ inline void nextTarget(Nav& nv, const Track& T, const DevSurface* geo
   nv.navBreak = true;
 }
 
-// Propagator.ipp:49-79
+// A layer of the layer navigator and its approach intersection
+// (Layer::resolve and Layer::surfaceOnApproach without an approach
+// descriptor, Layer.hpp:147-168, Layer.cpp:224-266): a surface that is
+// sensitive or carries material (resolveSensitive, resolveMaterial on,
+// resolvePassive off), its first intersection along the line that lies
+// inside its bounds, beyond the near limit (the on-surface tolerance,
+// NavigatorOptions.hpp:42) and before the far limit (checkPathLength,
+// Intersection.hpp:327-337).
+__device__ inline bool layerApproach(const DevSurface& s, const V3& pos, const V3& dir,
+                                     double& path, int& idx) {
+  if (!(s.flags & ACTS_KF_SURFACE_SENSITIVE) && s.mat < 0) return false;
+  const Isect x = intersectBounded(s, pos, dir, kOnSurfaceTolerance);
+  for (int i = 0; i < x.size; ++i) {
+    if (x.status[i] != kUnreachable && x.path[i] > kOnSurfaceTolerance &&
+        x.path[i] < DBL_MAX + kOnSurfaceTolerance) {
+      path = x.path[i];
+      idx = i;
+      return true;
+    }
+  }
+  return false;
+}
+
+// The layer navigator's next target (Navigator::nextTarget with Gen1
+// targeting, Navigator.cpp:251-316,479-540; resolveLayers :826-849;
+// TrackingVolume::compatibleLayers, TrackingVolume.cpp:443-487).  After the
+// track has reached the layer `on` (and at the start) the layers are
+// resolved again from here, as Gen1 does after a volume switch
+// (resetAfterVolumeSwitch, Navigator.hpp:245-258); otherwise the next layer
+// of the current list is the target.  As in a Gen1 list, which holds each
+// layer of a volume once, a layer already reached (or the start surface) is
+// not resolved again: where a cylinder and a disc cross each other the track
+// would otherwise go back and forth between the two.
+// The list of layers resolved from one position and direction, in
+// increasing path length (ties by surface index), is not stored: the record
+// keeps the position and direction it was resolved from and the path of the
+// last target, and the next entry is the layer with the smallest (path,
+// index) after the last one, recomputed from the same origin.  A list that
+// is used up is resolved again from the current position (renavigation);
+// an empty list then ends the navigation.
+template <bool Rev = false>
+__device__ inline void layerNextTarget(Nav& nv, const Track& T, const DevSurface* geo,
+                                       const LayerCtx& lc, int on) {
+  nv.tSurf = -1;
+  if (nv.navBreak) return;
+  if (on >= 0) {
+    markReached(lc, on);
+    nv.seqPos = kNavUnresolved;
+  }
+  double* rec = lc.rec;
+  const int64_t st = lc.stride;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if (nv.seqPos == kNavUnresolved) {
+      const V3 d = navDir<Rev>(T);
+      rec[0] = T.pos.x;
+      rec[st] = T.pos.y;
+      rec[2 * st] = T.pos.z;
+      rec[3 * st] = d.x;
+      rec[4 * st] = d.y;
+      rec[5 * st] = d.z;
+      rec[6 * st] = -INFINITY;
+      nv.seqPos = -1;
+    }
+    const V3 o = mk(rec[0], rec[st], rec[2 * st]);
+    const V3 d = mk(rec[3 * st], rec[4 * st], rec[5 * st]);
+    const double lastPath = rec[6 * st];
+    const int lastSurf = nv.seqPos;
+    double best = INFINITY;
+    int bestS = -1, bestI = 0;
+    uint64_t word = 0;
+    for (int s = 0; s < lc.nSurfaces; ++s) {
+      if ((s & 63) == 0) word = lc.mask[(s >> 6) * st];
+      if ((word >> (s & 63)) & 1) continue;
+      double p;
+      int i;
+      if (!layerApproach(geo[s], o, d, p, i)) continue;
+      const bool after = p > lastPath || (p == lastPath && s > lastSurf);
+      if (after && p < best) {
+        best = p;
+        bestS = s;
+        bestI = i;
+      }
+    }
+    if (bestS >= 0) {
+      nv.tSurf = bestS;
+      nv.tIdx = bestI;
+      nv.seqPos = bestS;
+      rec[6 * st] = best;
+      return;
+    }
+    if (attempt == 0) nv.seqPos = kNavUnresolved;
+  }
+  nv.navBreak = true;
+}
+
+// Propagator.ipp:49-79.  `on`: the surface the track has just reached
+// (-1 none), which the layer navigator leaves out when it resolves again.
 template <bool Rev = false, typename SeqFn>
 __device__ inline int getNextTarget(Nav& nv, Track& T, const DevSurface* geo,
-                                    const DevOpts& o, SeqFn seqAt) {
+                                    const DevOpts& o, SeqFn seqAt, const LayerCtx& lc,
+                                    int on) {
+  const bool layers = o.navigator != ACTS_KF_NAVIGATOR_DIRECT;
   for (uint32_t i = 0; i < o.maxTargetSkipping; ++i) {
-    nextTarget<Rev>(nv, T, geo, o, seqAt);
+    if (layers)
+      layerNextTarget<Rev>(nv, T, geo, lc, on);
+    else
+      nextTarget<Rev>(nv, T, geo, o, seqAt);
+    // Navigator::nextTarget resets the current surface
+    on = -1;
     if (nv.tSurf < 0) return 0;
-    const int st = updateSurfaceStatus<Rev>(T, geo[nv.tSurf], nv.tIdx, o.surfaceTolerance);
+    const int st =
+        updateSurfaceStatus<Rev>(T, geo[nv.tSurf], nv.tIdx, o.surfaceTolerance, layers);
     if (st == kOnSurface) continue;
     if (st == kReachable) return 0;
   }

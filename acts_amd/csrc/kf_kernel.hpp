@@ -105,6 +105,12 @@ struct FwdArgs {
   RevArgs rv;
 };
 
+// The layer-navigator record of queue slot q.
+__device__ inline LayerCtx layerCtx(const FwdArgs& a, int64_t q) {
+  return LayerCtx{a.sc.nav ? a.sc.nav + q : nullptr, a.sc.nav ? a.sc.navMask + q : nullptr,
+                  a.sc.stride, a.nSurfaces};
+}
+
 // The reverse filter's actor target check (SurfaceReached::checkAbort on the
 // reference surface) and the end of a reverse pass: kf_reverse.hpp.
 template <bool Sympy>
@@ -182,9 +188,11 @@ __device__ inline int laneInit(Lane& L, const int64_t q, const int64_t t, const 
   // surface indices come from the caller: reject the track if any is out of
   // range instead of reading outside the geometry table
   bool badIndex = false;
-  for (int i = 0; i < seqLen; ++i) {
-    const int sid = L.seq(i);
-    badIndex = badIndex || sid < 0 || sid >= a.nSurfaces;
+  if (o.navigator == ACTS_KF_NAVIGATOR_DIRECT) {
+    for (int i = 0; i < seqLen; ++i) {
+      const int sid = L.seq(i);
+      badIndex = badIndex || sid < 0 || sid >= a.nSurfaces;
+    }
   }
   // measurement subspaces the updater does not handle fail the track too
   if (in.measDims || in.measIdx) {
@@ -226,11 +234,12 @@ __device__ inline int laneInit(Lane& L, const int64_t q, const int64_t t, const 
     T.cUser = o.maxStepSize;
     T.cAcc = o.initialStepSize;
   }
-  L.nv.seqPos = -1;
+  L.nv.seqPos = o.navigator == ACTS_KF_NAVIGATOR_DIRECT ? -1 : kNavUnresolved;
   L.nv.seqLen = seqLen;
   L.nv.navBreak = false;
   L.nv.tSurf = -1;
   L.nv.tIdx = 0;
+  if (o.navigator != ACTS_KF_NAVIGATOR_DIRECT) clearReached(layerCtx(a, q));
 
   L.propLimit = o.pathLimit;
   loopProtection<Mixed>(T, a.F, o, L.propLimit, false);
@@ -240,7 +249,7 @@ __device__ inline int laneInit(Lane& L, const int64_t q, const int64_t t, const 
   loopProtection<Mixed>(T, a.F, o, L.kfLimit, true);
   L.finished = (L.nMeasStates == L.nUnique) || pathAbort(T, L.kfLimit, o.surfaceTolerance);
   if (L.finished || pathAbort(T, L.propLimit, o.surfaceTolerance)) return kPhaseFinished;
-  L.status = getNextTarget(L.nv, T, geo, o, L.seq);
+  L.status = getNextTarget(L.nv, T, geo, o, L.seq, layerCtx(a, q), -1);
   return L.status ? kPhaseFinished : kPhaseStepping;
 }
 
@@ -303,7 +312,8 @@ __device__ inline int laneAfterStep(Lane& L, const FwdArgs& a, const DevSurface*
   T.cNav = kNotSet;
   T.cActor = kNotSet;
   if (L.nv.tSurf >= 0) {
-    const int ps = updateSurfaceStatus<Rev>(T, geo[L.nv.tSurf], L.nv.tIdx, o.surfaceTolerance);
+    const int ps = updateSurfaceStatus<Rev>(T, geo[L.nv.tSurf], L.nv.tIdx, o.surfaceTolerance,
+                                            o.navigator != ACTS_KF_NAVIGATOR_DIRECT);
     if (ps == kOnSurface && !L.nv.navBreak) L.reached = L.nv.tSurf;
     if (ps != kReachable) L.nv.tSurf = -1;
   }
@@ -316,7 +326,7 @@ __device__ inline int laneAfterStep(Lane& L, const FwdArgs& a, const DevSurface*
   if (L.finished || pathAbort(T, L.propLimit, o.surfaceTolerance)) return kPhaseFinished;
   if (L.nv.tSurf < 0) {
     T.cNav = kNotSet;
-    L.status = getNextTarget<Rev>(L.nv, T, geo, o, L.seq);
+    L.status = getNextTarget<Rev>(L.nv, T, geo, o, L.seq, layerCtx(a, L.q), -1);
     if (L.status) return kPhaseFinished;
   }
   return kPhaseStepping;
@@ -371,6 +381,9 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
       transportToBound<Sympy>(T, s, onS, J);
       if (!onS) {
         L.status = ACTS_KF_STATUS_NOT_ON_SURFACE;
+      } else if (!Rev && L.nStates >= L.nv.seqLen) {
+        // only the layer navigator can find more states than slots
+        L.status = ACTS_KF_STATUS_STATE_CAPACITY;
       } else if (!Rev && L.nStates >= sc.maxStates) {
         L.status = ACTS_KF_STATUS_PROPAGATION_FAILURE;
       } else {
@@ -481,7 +494,7 @@ __device__ inline int laneFilter(Lane& L, const FwdArgs& a, const DevSurface* ge
   }
   if (L.finished || pathAbort(T, L.propLimit, o.surfaceTolerance)) return kPhaseFinished;
   T.cNav = kNotSet;
-  L.status = getNextTarget<Rev>(L.nv, T, geo, o, L.seq);
+  L.status = getNextTarget<Rev>(L.nv, T, geo, o, L.seq, layerCtx(a, q), reached);
   return L.status ? kPhaseFinished : kPhaseStepping;
 }
 

@@ -84,13 +84,21 @@ __device__ inline int laneInitReverse(Lane& L, const int64_t q, const int64_t t,
   initializeFromBound<Sympy>(T, geo[surf], p, p + 6, o);
   L.startSurf = surf;
   int found = -1;
-  for (int i = 0; i < seqLen; ++i)
-    if (found < 0 && L.seq(i) == surf) found = i;
-  L.nv.seqPos = found >= 0 ? found + 1 : seqLen;
+  if (o.navigator == ACTS_KF_NAVIGATOR_DIRECT)
+    for (int i = 0; i < seqLen; ++i)
+      if (found < 0 && L.seq(i) == surf) found = i;
+  // the layer navigator resolves its lists without the start surface (the
+  // volume's layer at the start position is the start object)
+  L.nv.seqPos = o.navigator == ACTS_KF_NAVIGATOR_DIRECT ? (found >= 0 ? found + 1 : seqLen)
+                                                       : kNavUnresolved;
   L.nv.seqLen = seqLen;
   L.nv.navBreak = false;
   L.nv.tSurf = -1;
   L.nv.tIdx = 0;
+  if (o.navigator != ACTS_KF_NAVIGATOR_DIRECT) {
+    clearReached(layerCtx(a, q));
+    markReached(layerCtx(a, q), surf);
+  }
   L.propLimit = o.pathLimit;
   loopProtection<Mixed>(T, a.F, o, L.propLimit, false);
   L.kfLimit = DBL_MAX;

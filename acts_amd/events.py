@@ -12,6 +12,7 @@ diag(sigma^2).
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 from dataclasses import dataclass
 
 import numpy as np
@@ -95,6 +96,20 @@ class Events:
             else np.ascontiguousarray(self.meas_dims[mid]),
             meas_indices=None if self.meas_indices is None
             else np.ascontiguousarray(self.meas_indices[mid]))
+
+
+def with_state_slots(events: Events, slots) -> Events:
+    """The same tracks for the layer navigator: `slots` state slots per
+    track (an int for every track, or one per track) instead of the surface
+    sequence, whose entries the layer navigator does not read (-1 here)."""
+    n = events.n_tracks
+    lens = np.broadcast_to(np.asarray(slots, dtype=np.int64), (n,))
+    if np.any(lens < 0):
+        raise ValueError("negative slot count")
+    so = np.zeros(n + 1, np.int64)
+    so[1:] = np.cumsum(lens)
+    return dataclasses.replace(events, seq_offsets=so,
+                               seq_surfaces=np.full(int(so[-1]), -1, np.int32))
 
 
 def particle_options(mass: float = MUON_MASS) -> capi.Options:

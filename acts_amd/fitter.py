@@ -57,7 +57,10 @@ class KalmanFitterOptions:
     kernel_timing: bool = False             # events around the kernels
     use_graph: bool = False                 # replay fits as a captured hipGraph
     sched_transport_skip_steps: int = 0     # 0 = off
-    reserved_sched: int = 0
+    # capi.NAVIGATOR_DIRECT: each track's given surface sequence
+    # (DirectNavigator); capi.NAVIGATOR_LAYERS: the surfaces found from the
+    # geometry (Navigator), the sequence offsets giving each track's slots
+    navigator: int = capi.NAVIGATOR_DIRECT
     # reverse filtering instead of smoothing (KalmanFitterOptions::
     # reverseFiltering, reverseFilteringCovarianceScaling; the Examples'
     # momentum threshold of SimpleReverseFilteringLogic)

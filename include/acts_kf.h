@@ -37,7 +37,7 @@ extern "C" {
  * subspaces); callers that value-initialise the struct are unaffected.
  * 3: acts_kf_options gained the reverse-filtering options and
  * acts_kf_batch_out the per-track `reversed` flag (both appended).         */
-#define ACTS_KF_API_VERSION 3
+#define ACTS_KF_API_VERSION 4
 
 /* ---- API return codes (the call itself) -------------------------------- */
 #define ACTS_KF_OK 0
@@ -55,6 +55,7 @@ extern "C" {
 #define ACTS_KF_CAT_SURFACE 0x400    /* SurfaceError.hpp:18-22 */
 #define ACTS_KF_CAT_FIELD 0x500      /* MagneticFieldError.hpp:18-22 */
 #define ACTS_KF_CAT_EXTRAPOLATION 0x600 /* TrackExtrapolationError, TrackHelpers.hpp:48-53 */
+#define ACTS_KF_CAT_LIBRARY 0x700    /* this library's own conditions */
 
 #define ACTS_KF_STATUS_UPDATE_FAILED (ACTS_KF_CAT_KALMAN | 1)
 #define ACTS_KF_STATUS_SMOOTH_FAILED (ACTS_KF_CAT_KALMAN | 2)
@@ -71,6 +72,10 @@ extern "C" {
 #define ACTS_KF_STATUS_FIELD_OUT_OF_BOUNDS (ACTS_KF_CAT_FIELD | 1)
 #define ACTS_KF_STATUS_NO_COMPATIBLE_STATE (ACTS_KF_CAT_EXTRAPOLATION | 1)
 #define ACTS_KF_STATUS_REFERENCE_UNREACHABLE (ACTS_KF_CAT_EXTRAPOLATION | 2)
+/* the layer navigator found more states than the track has slots
+ * (seq_offsets[i+1] - seq_offsets[i]); the reference's track container
+ * grows instead, so it has no such error                                   */
+#define ACTS_KF_STATUS_STATE_CAPACITY (ACTS_KF_CAT_LIBRARY | 1)
 
 /* ---- track-state type flags (TrackStateType.hpp:30-49, bit positions) -- */
 #define ACTS_KF_STATE_HAS_MEASUREMENT (1u << 0)
@@ -101,11 +106,39 @@ extern "C" {
 
 #define ACTS_KF_SURFACE_SENSITIVE 1u /* Surface::isSensitive() */
 
+/* Navigator of the fit (acts_kf_options.navigator).
+ * DIRECT: DirectNavigator through each track's given surface sequence
+ *   (KalmanFitter::fit with sSequence, KalmanFitter.hpp:725,778).
+ * LAYERS: the surfaces are found from the geometry (KalmanFitter::fit
+ *   without a sequence, KalmanFitter.hpp:693-699, with the Navigator as the
+ *   reference's fitter tests configure it: resolveSensitive and
+ *   resolveMaterial on, resolvePassive off, FitterTestsCommon.hpp:96-99).
+ *   Every surface of the geometry table that is sensitive or carries
+ *   material is a layer of its own (Navigator Gen1, Navigator.cpp:251-316,
+ *   479-540,826-849).  At the start and after each reached layer, the
+ *   layers are resolved along the straight line of the current direction,
+ *   without the layer the track is on: the first intersection of each,
+ *   inside its bounds, more than the on-surface tolerance ahead, sorted by
+ *   path length (TrackingVolume::compatibleLayers, Layer::
+ *   surfaceOnApproach); the propagation targets them in that order,
+ *   skipping those it can no longer reach inside their bounds, and once
+ *   they are used up resolves again from where the track is
+ *   (renavigation); when that finds none the navigation ends.  (Gen1
+ *   resolves a volume's layers once, and again after a volume switch; here
+ *   each reached layer counts as one.)  seq_offsets still give every track its state
+ *   slots (its capacity); the entries of seq_surfaces are not read.  The
+ *   reverse filter navigates the same way backwards from the last
+ *   measurement surface.  The transport-skip option is ignored.            */
+#define ACTS_KF_NAVIGATOR_DIRECT 0
+#define ACTS_KF_NAVIGATOR_LAYERS 1
+
 /* One surface.  The placement is the surface's local-to-global transform:
  * rotation columns are the local x, y, z axes expressed in the global frame.
- * Bounds are not used by the fit itself (the direct navigator intersects with
- * BoundaryTolerance::Infinite, DirectNavigator.hpp:305-308) but are used by
- * the geometry helpers and the event generator.                              */
+ * The direct navigator does not use the bounds (it intersects with
+ * BoundaryTolerance::Infinite, DirectNavigator.hpp:305-308); the layer
+ * navigator does (BoundaryTolerance::None), as do the geometry helpers and
+ * the event generator.  Plane bounds: half lengths of a RectangleBounds in
+ * local x and y, or 0 and 0 for an unbounded plane.                          */
 typedef struct acts_kf_surface {
   int32_t type;          /* ACTS_KF_SURFACE_* */
   int32_t material;      /* index into the material table, or -1 */
@@ -237,7 +270,8 @@ typedef struct acts_kf_options {
    * reach a surface repeats the skipped steps, which costs more than the
    * loopers save in the field map.  No reference counterpart.               */
   int32_t sched_transport_skip_steps;
-  int32_t reserved_sched;
+  /* ACTS_KF_NAVIGATOR_* (default DIRECT)                                    */
+  int32_t navigator;
   /* Reverse filtering instead of smoothing (KalmanFitter.hpp:884-945,
    * KalmanFitterOptions::reverseFiltering and
    * reverseFilteringCovarianceScaling, :171-181): after the forward filter
@@ -268,7 +302,9 @@ typedef struct acts_kf_batch_in {
   const double* start_params;   /* [n][6] */
   const double* start_cov;      /* [n][36] */
   const double* start_plane;    /* [n][12]: rotation row-major, then translation */
-  /* surface sequence for the direct navigator (KalmanFitter.hpp:725,778) */
+  /* surface sequence for the direct navigator (KalmanFitter.hpp:725,778);
+   * with the layer navigator only the offsets are used (each track's state
+   * slots), but seq_surfaces must still point to seq_offsets[n] entries   */
   const int64_t* seq_offsets;   /* [n+1] */
   const int32_t* seq_surfaces;  /* [seq_offsets[n]] indices into the geometry */
   /* measurements: 2D (loc0, loc1) source links keyed by their surface */

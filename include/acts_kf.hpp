@@ -3,7 +3,10 @@
 //
 //   Acts::KalmanFitter<Propagator<EigenStepper<>, DirectNavigator>, traj_t>
 //     ::fit(begin, end, startParameters, kfOptions, surfaceSequence, tracks)
-//   (Core/include/Acts/TrackFitting/KalmanFitter.hpp:721-731)
+//   (Core/include/Acts/TrackFitting/KalmanFitter.hpp:721-731), and, with
+//   options.navigator = ACTS_KF_NAVIGATOR_LAYERS, the Navigator overload
+//     ::fit(begin, end, startParameters, kfOptions, tracks)
+//   (KalmanFitter.hpp:693-699), which finds the surfaces from the geometry
 //
 // Header-only; depends on nothing but the C ABI.  Two ways to call it:
 //   - fit(...): one track, synchronous, returns a Result like the reference;
@@ -55,6 +58,8 @@ struct FitResult {
   int32_t status = 0;
   float chi2 = 0;
   int32_t ndf = 0, nMeasurements = 0, nHoles = 0, nOutliers = 0;
+  // filtered in reverse instead of smoothed (reverse filtering options)
+  bool reversed = false;
   std::vector<TrackState> states;
   // with a reference surface in the options: the track parameters and
   // covariance on it (TrackProxy::parameters()/covariance())
@@ -70,7 +75,8 @@ class KalmanFitter {
   KalmanFitter(const std::vector<acts_kf_surface>& surfaces,
                const std::vector<acts_kf_material_slab>& materials,
                const acts_kf_field& field, const acts_kf_options& options, int device = 0)
-      : withReference_(options.reference_surface != ACTS_KF_REFERENCE_NONE) {
+      : withReference_(options.reference_surface != ACTS_KF_REFERENCE_NONE),
+        nSurfaces_(surfaces.size()) {
     const int rc = acts_kf_create(surfaces.data(), static_cast<int32_t>(surfaces.size()),
                                   materials.data(), static_cast<int32_t>(materials.size()),
                                   &field, &options, device, &ctx_);
@@ -112,6 +118,15 @@ class KalmanFitter {
     return idx;
   }
 
+  // The same for the layer navigator: no surface sequence, `stateSlots`
+  // states at most (the track fails with ACTS_KF_STATUS_STATE_CAPACITY if
+  // the navigator finds more).
+  template <typename MeasIt>
+  size_t enqueue(MeasIt begin, MeasIt end, const BoundTrackParameters& start,
+                 size_t stateSlots) {
+    return enqueue(begin, end, start, std::vector<int32_t>(stateSlots, -1));
+  }
+
   size_t queued() const { return seqOffsets_.size() - 1; }
 
   // Fit every queued track (host buffers, one batch) and clear the queue.
@@ -120,7 +135,8 @@ class KalmanFitter {
     const int64_t slots = seqOffsets_.back();
     std::vector<FitResult> results(static_cast<size_t>(n));
     if (n == 0) return results;
-    std::vector<int32_t> status(n), nStates(n), ndf(n), nMeas(n), nHoles(n), nOut(n);
+    std::vector<int32_t> status(n), nStates(n), ndf(n), nMeas(n), nHoles(n), nOut(n),
+        reversed(n);
     std::vector<float> chi2(n), stateChi2(slots);
     std::vector<int32_t> stateSurface(slots);
     std::vector<uint32_t> stateFlags(slots);
@@ -147,6 +163,7 @@ class KalmanFitter {
     out.n_measurements = nMeas.data();
     out.n_holes = nHoles.data();
     out.n_outliers = nOut.data();
+    out.reversed = reversed.data();
     out.state_surface = stateSurface.data();
     out.state_flags = stateFlags.data();
     out.state_path = statePath.data();
@@ -167,6 +184,7 @@ class KalmanFitter {
       r.nMeasurements = nMeas[t];
       r.nHoles = nHoles[t];
       r.nOutliers = nOut[t];
+      r.reversed = reversed[t] != 0;
       if (!r.ok()) continue;
       if (withReference_) {
         r.hasReference = true;
@@ -198,6 +216,16 @@ class KalmanFitter {
     return std::move(fitBatch().front());
   }
 
+  // The Navigator overload (layer navigator): as many state slots as the
+  // geometry has surfaces, unless given.
+  template <typename MeasIt>
+  FitResult fit(MeasIt begin, MeasIt end, const BoundTrackParameters& start,
+                size_t stateSlots = 0) {
+    if (queued() != 0) throw std::logic_error("fit() with tracks queued: call fitBatch()");
+    enqueue(begin, end, start, stateSlots ? stateSlots : nSurfaces_);
+    return std::move(fitBatch().front());
+  }
+
   acts_kf_ctx* context() const { return ctx_; }
 
  private:
@@ -220,6 +248,7 @@ class KalmanFitter {
 
   acts_kf_ctx* ctx_ = nullptr;
   bool withReference_ = false;
+  size_t nSurfaces_ = 0;
   std::vector<double> params_, cov_, plane_;
   std::vector<int64_t> seqOffsets_{0};
   std::vector<int32_t> seqSurfaces_;

@@ -1147,7 +1147,11 @@ enum MatMode { NoUpdate = 0, PreUpdate = 1, PostUpdate = 2, FullUpdate = 3 };
 //   1: material applied before the update on measurement surfaces (forward
 //      PreUpdate factor 1, PostUpdate factor 0),
 //   2: phi variance theta0^2 without the 1/sin^2(theta) factor,
-//   3: no 10 MeV momentum floor after the energy loss.
+//   3: no 10 MeV momentum floor after the energy loss;
+// and in the layer navigator:
+//   4: the layer list resolved once at the start (not after each reached
+//      layer),
+//   5: layers intersected without their bounds.
 // The oracle proper is built with 0.
 #ifndef ORACLE_PLANT
 #define ORACLE_PLANT 0
@@ -1426,12 +1430,51 @@ struct Target {
   int index = 0;
 };
 
-// SteppingHelper.hpp:40-78 (boundary tolerance infinite)
+// The bounds check of Surface::intersect with BoundaryTolerance::None at
+// the point x: CylinderSurface.cpp:242-257 (full azimuth: |local z| below
+// the half length plus the tolerance), DiscSurface.cpp:286-293 (full disc:
+// squared distance from the centre within rmin^2 .. rmax^2), and for a plane
+// its RectangleBounds (half lengths b0, b1; none when both are 0).
+bool insideBoundsNone(const Surface& s, const V3& x, double tol) {
+  const V3 rel = v3(x[0] - s.pl.t[0], x[1] - s.pl.t[1], x[2] - s.pl.t[2]);
+  if (s.type == ACTS_KF_SURFACE_CYLINDER) {
+    const double cZ = dot(rel, s.pl.col(2));
+    return std::abs(cZ) < std::abs(s.b1 + tol);
+  }
+  if (s.type == ACTS_KF_SURFACE_DISC) {
+    const double r2 = dot(rel, rel);
+    return r2 >= s.b0 * s.b0 && r2 <= s.b1 * s.b1;
+  }
+  if (s.type == ACTS_KF_SURFACE_PLANE && s.b0 > 0 && s.b1 > 0) {
+    const double lx = dot(rel, s.pl.col(0)), ly = dot(rel, s.pl.col(1));
+    return -s.b0 <= lx && lx <= s.b0 && -s.b1 <= ly && ly <= s.b1;
+  }
+  return true;
+}
+
+// Surface::intersect with BoundaryTolerance::None: a solution outside the
+// bounds is unreachable.
+MultiIntersection intersectBounded(const Surface& s, const V3& pos, const V3& dir,
+                                   double tol) {
+  MultiIntersection mi = s.intersect(pos, dir, tol);
+  if (ORACLE_PLANT == 5) return mi;
+  for (int i = 0; i < mi.size; ++i) {
+    Intersection& x = mi.is[i];
+    if (!x.valid()) continue;
+    const V3 at = v3(pos[0] + x.path * dir[0], pos[1] + x.path * dir[1], pos[2] + x.path * dir[2]);
+    if (!insideBoundsNone(s, at, tol)) x.status = IStatus::unreachable;
+  }
+  return mi;
+}
+
+// SteppingHelper.hpp:40-78: BoundaryTolerance::Infinite for the direct
+// navigator's targets, None for the layer navigator's (bounded)
 IStatus updateSurfaceStatus(const Surface& s, Stepper& st, int index,
-                            double propDir, double tol) {
+                            double propDir, double tol, bool bounded = false) {
   V3 d = st.direction();
   V3 dd = v3(propDir * d[0], propDir * d[1], propDir * d[2]);
-  MultiIntersection mi = s.intersect(st.position(), dd, tol);
+  MultiIntersection mi =
+      bounded ? intersectBounded(s, st.position(), dd, tol) : s.intersect(st.position(), dd, tol);
   const Intersection& x = mi.is[index];
   if (x.status == IStatus::onSurface) {
     st.stepSize.release(ConstrainedStep::Navigator);
@@ -1471,6 +1514,99 @@ Target nextTarget(const Geometry& geo, DirectNav& nav, const V3& pos,
       }
     }
     nav.surfaceIndex += inc;
+  }
+  nav.navigationBreak = true;
+  return Target{};
+}
+
+// The layer navigator: Navigator with Gen1 targeting (Navigator.cpp:
+// 251-316 nextTarget and renavigation, :479-540 getNextTargetGen1, :826-849
+// resolveLayers) over the single surfaces of the geometry table as layers,
+// resolved as the reference's fitter tests configure it (resolveSensitive
+// and resolveMaterial on, resolvePassive off, FitterTestsCommon.hpp:96-99).
+// Such a layer has no surface array, so reaching it yields no further
+// surfaces (Layer::compatibleSurfaces, Layer.cpp:116-126).  Each reached
+// layer is treated like a volume switch (resetAfterVolumeSwitch,
+// Navigator.hpp:245-258): the remaining layers are resolved again from it,
+// with it as the start object.  (Gen1 resolves a volume's list once; with
+// cylinders and discs crossing each other in one volume, as in the endcap
+// detectors here, a list resolved along the start direction misorders
+// nearly coincident crossings.)  As in a Gen1 list, which holds each layer
+// of a volume once, a layer that has been reached (or the start surface) is
+// not resolved again: where a cylinder and a disc cross each other the
+// track would otherwise go back and forth between the two.  A used-up list
+// means the boundaries, of which there are none, so the navigator
+// renavigates.
+struct LayerNav {
+  struct Cand {
+    double path;
+    int surface;
+    int index;
+  };
+  std::vector<Cand> list;  // resolved layers, by path length (ties: index)
+  size_t next = 0;         // the next entry to target
+  bool resolved = false;
+  std::vector<char> reached;  // layers reached (and the start surface)
+  int current = -1;        // -1 none, -2 start surface, >= 0 geometry index
+  bool navigationBreak = false;
+  double direction = 1.0;
+};
+
+// TrackingVolume::compatibleLayers (TrackingVolume.cpp:443-487) with
+// Layer::resolve and Layer::surfaceOnApproach (Layer.hpp:147-168,
+// Layer.cpp:224-266: no approach descriptor, so the layer surface's first
+// intersection inside its bounds within the path window), sorted as
+// resolveLayers sorts them.  The near limit is NavigatorPlainOptions'
+// (the on-surface tolerance, NavigatorOptions.hpp:42), the far limit
+// unbounded; the layers already reached are left out (the start object).
+// Here the layers are not walked through a binned layer array (nextLayer),
+// so the list is sorted by path length from the start and TrackingVolume's
+// rotation to the closest entry changes nothing.
+void resolveLayers(const Geometry& geo, LayerNav& nav, const V3& pos, const V3& dir) {
+  nav.list.clear();
+  nav.next = 0;
+  const int n = static_cast<int>(geo.surfaces.size());
+  for (int sid = 0; sid < n; ++sid) {
+    if (nav.reached[sid]) continue;
+    const Surface& s = geo.surfaces[sid];
+    if (!s.sensitive && !s.hasMaterial) continue;
+    const MultiIntersection mi = intersectBounded(s, pos, dir, kOnSurfaceTolerance);
+    for (int i = 0; i < mi.size; ++i) {
+      const Intersection& x = mi.is[i];
+      if (x.valid() && x.path > kOnSurfaceTolerance &&
+          x.path < std::numeric_limits<double>::max() + kOnSurfaceTolerance) {
+        nav.list.push_back({x.path, sid, i});
+        break;
+      }
+    }
+  }
+  std::sort(nav.list.begin(), nav.list.end(), [](const LayerNav::Cand& a, const LayerNav::Cand& b) {
+    return a.path < b.path || (a.path == b.path && a.surface < b.surface);
+  });
+}
+
+// Navigator::nextTarget for the layer navigator: after a reached layer (or
+// at the start) the list is resolved from here; else the next layer of the
+// list; a used-up list is resolved again from the current position
+// (renavigation); if that finds none, the navigation ends.
+Target nextTargetLayers(const Geometry& geo, LayerNav& nav, const V3& pos, const V3& dir) {
+  const int on = nav.current;
+  nav.current = -1;
+  if (nav.navigationBreak) return Target{};
+  if (on >= 0) nav.reached[on] = 1;
+  const V3 d = v3(nav.direction * dir[0], nav.direction * dir[1], nav.direction * dir[2]);
+  if (!nav.resolved || (on >= 0 && ORACLE_PLANT != 4)) {
+    resolveLayers(geo, nav, pos, d);
+    nav.resolved = true;
+  }
+  if (nav.next < nav.list.size()) {
+    const LayerNav::Cand& c = nav.list[nav.next++];
+    return Target{false, c.surface, c.index};
+  }
+  resolveLayers(geo, nav, pos, d);
+  if (nav.next < nav.list.size()) {
+    const LayerNav::Cand& c = nav.list[nav.next++];
+    return Target{false, c.surface, c.index};
   }
   nav.navigationBreak = true;
   return Target{};
@@ -1559,6 +1695,9 @@ int kfFilter(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
     Vec6 bp;
     r = boundState(surf, st, false, bp);
     if (r) return r;
+    // the track's state slots (only the layer navigator can find more)
+    if (propDir > 0 && res.states.size() >= in.sequence.size())
+      return ACTS_KF_STATUS_STATE_CAPACITY;
     ks.s.surface = sid;
     ks.s.predicted = bp;
     ks.s.predictedCov = st.cov;
@@ -1604,6 +1743,8 @@ int kfFilter(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
     Vec6 bp;
     int r = boundState(surf, st, true, bp);
     if (r) return r;
+    if (propDir > 0 && res.states.size() >= in.sequence.size())
+      return ACTS_KF_STATUS_STATE_CAPACITY;
     ks.s.surface = sid;
     ks.s.predicted = bp;
     ks.s.predictedCov = st.cov;
@@ -1748,7 +1889,8 @@ bool targetReachedCheck(const Surface& target, Stepper& st, double propDir, doub
 // the track's own start plane, which the sequence does not contain.  The
 // DirectNavigator starts next to the start surface's first entry in the
 // sequence, or at the sequence's end in the propagation direction
-// (DirectNavigator.hpp:222-272); the actor acts on the start surface first
+// (DirectNavigator.hpp:222-272); the layer navigator resolves its first
+// list without the start surface; the actor acts on the start surface first
 // (the navigator's current surface).  `target` is the actor's targetReached
 // surface (null: none).
 void filterPass(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
@@ -1758,10 +1900,18 @@ void filterPass(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
   const auto& o = geo.opt;
   Stepper st;
   initializeStepper(st, startSurface, startParams, &startCov, o);
+  const bool layers = o.navigator == ACTS_KF_NAVIGATOR_LAYERS;
+  LayerNav lnav;
+  lnav.direction = propDir;
+  lnav.current = startSid;
+  lnav.reached.assign(geo.surfaces.size(), 0);
+  if (startSid >= 0) lnav.reached[startSid] = 1;
   DirectNav nav;
   nav.seq = &in.sequence;
   nav.direction = propDir;
   nav.current = startSid;
+  int& current = layers ? lnav.current : nav.current;
+  const bool& navigationBreak = layers ? lnav.navigationBreak : nav.navigationBreak;
   const int n = static_cast<int>(in.sequence.size());
   int found = -1;
   for (int i = 0; i < n && startSid >= 0; ++i)
@@ -1785,10 +1935,10 @@ void filterPass(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
     if (res.finished) return 0;
     if (res.pathLimit == std::numeric_limits<double>::max())
       setupLoopProtection(geo, st, res.pathLimit, true, propDir);
-    if (nav.current != -1) {
-      const bool isStart = nav.current == startSid;
-      const Surface& s = nav.current == -2 ? in.startSurface : geo.surfaces[nav.current];
-      int r = kfFilter(geo, in, ml, s, nav.current, isStart, st, res, propDir);
+    if (current != -1) {
+      const bool isStart = current == startSid;
+      const Surface& s = current == -2 ? in.startSurface : geo.surfaces[current];
+      int r = kfFilter(geo, in, ml, s, current, isStart, st, res, propDir);
       if (r) return r;
     }
     const bool complete = res.measurementStates == nInputMeas;
@@ -1815,13 +1965,14 @@ void filterPass(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
   // Propagator::propagate (Propagator.ipp:25-210)
   auto getNextTarget = [&](Target& t) -> int {
     for (unsigned i = 0; i < o.max_target_skipping; ++i) {
-      Target nt = nextTarget(geo, nav, st.position(), st.direction());
+      Target nt = layers ? nextTargetLayers(geo, lnav, st.position(), st.direction())
+                         : nextTarget(geo, nav, st.position(), st.direction());
       if (nt.none) {
         t = nt;
         return 0;
       }
       IStatus ps = updateSurfaceStatus(geo.surfaces[nt.surface], st, nt.index,
-                                       propDir, o.surface_tolerance);
+                                       propDir, o.surface_tolerance, layers);
       if (ps == IStatus::onSurface) continue;
       if (ps == IStatus::reachable) {
         t = nt;
@@ -1849,9 +2000,8 @@ void filterPass(const Geometry& geo, const TrackIn& in, const MeasLookup& ml,
       st.stepSize.release(ConstrainedStep::Actor);
       if (!target.none) {
         IStatus ps = updateSurfaceStatus(geo.surfaces[target.surface], st,
-                                         target.index, propDir, o.surface_tolerance);
-        if (ps == IStatus::onSurface && !nav.navigationBreak)
-          nav.current = (*nav.seq)[nav.surfaceIndex];
+                                         target.index, propDir, o.surface_tolerance, layers);
+        if (ps == IStatus::onSurface && !navigationBreak) current = target.surface;
         if (ps != IStatus::reachable) target = Target{};
       }
       status = act();

@@ -48,8 +48,8 @@ _plants = {}
 
 def load_plant(n: int):
     """A test-only build of the oracle with planted mistake number `n` in the
-    material application (kf_oracle.cpp, ORACLE_PLANT): tests show that their
-    checks fail on it."""
+    material application or the layer navigator (kf_oracle.cpp,
+    ORACLE_PLANT): tests show that their checks fail on it."""
     if n in _plants:
         return _plants[n]
     name = f"liboracle_kf_plant{int(n)}.so"

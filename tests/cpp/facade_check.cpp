@@ -2,7 +2,8 @@
 // reference's KalmanFitterFunctionImpl calls KalmanFitter::fit
 // (Examples/Algorithms/TrackFitting/src/KalmanFitterFunction.cpp:150-157):
 // one call per track, and the batched form.  Checks that both give the
-// same results bit for bit and that the fits succeed.
+// same results bit for bit and that the fits succeed, and that the
+// Navigator overload (no surface sequence) gives the same fits.
 //
 // Usage: facade_check [n_tracks]   (exit status 0 = pass)
 #include <cmath>
@@ -141,10 +142,31 @@ int main(int argc, char** argv) {
       }
       if (!same) ++mismatches;
     }
-    std::printf("facade_check: tracks=%lld ok=%lld single_vs_batch_mismatches=%lld\n",
+    // the Navigator overload (no surface sequence): on this barrel the
+    // layer navigator finds the generator's surfaces, so the fits are the
+    // same bit for bit
+    acts_kf_options layerOpts = fitOpts;
+    layerOpts.navigator = ACTS_KF_NAVIGATOR_LAYERS;
+    acts_amd::KalmanFitter layers(surfaces, mats, field, layerOpts, 0);
+    int64_t layerMismatches = 0;
+    for (int64_t t = 0; t < nSingle; ++t) {
+      const acts_amd::FitResult one = layers.fit(meas[t].begin(), meas[t].end(), starts[t]);
+      const acts_amd::FitResult& b = batch[t];
+      bool same = one.status == b.status && one.states.size() == b.states.size() &&
+                  one.chi2 == b.chi2 &&
+                  std::memcmp(one.referenceParameters.data(), b.referenceParameters.data(),
+                              6 * sizeof(double)) == 0;
+      for (size_t k = 0; same && k < one.states.size(); ++k)
+        same = one.states[k].surface == b.states[k].surface &&
+               std::memcmp(one.states[k].smoothed.data(), b.states[k].smoothed.data(),
+                           6 * sizeof(double)) == 0;
+      if (!same) ++layerMismatches;
+    }
+    std::printf("facade_check: tracks=%lld ok=%lld single_vs_batch_mismatches=%lld "
+                "layer_vs_direct_mismatches=%lld\n",
                 static_cast<long long>(nt), static_cast<long long>(nOk),
-                static_cast<long long>(mismatches));
-    if (mismatches != 0 || nOk < nt * 95 / 100) return 1;
+                static_cast<long long>(mismatches), static_cast<long long>(layerMismatches));
+    if (mismatches != 0 || layerMismatches != 0 || nOk < nt * 95 / 100) return 1;
   } catch (const std::exception& e) {
     std::fprintf(stderr, "facade_check: %s\n", e.what());
     return 2;

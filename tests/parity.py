@@ -47,8 +47,14 @@ def cov_rel_err(a: np.ndarray, b: np.ndarray, scale_cov: np.ndarray | None = Non
 
 
 def compare(got, ref, events, rel: float = 1e-10, rel_smoothed_cov: float = 5e-8,
-            rel_jacobian: float | None = None, rel_cov: float | None = None) -> dict:
-    """Return a report; `ok` is True if everything agrees within tolerance."""
+            rel_jacobian: float | None = None, rel_cov: float | None = None,
+            max_chi2_ndf: float | None = None) -> dict:
+    """Return a report; `ok` is True if everything agrees within tolerance.
+    With `max_chi2_ndf`, tracks whose reference fit has a larger chi2/ndf
+    (fits that did not converge, e.g. after a catastrophic energy loss on
+    the last surfaces, whose smoother amplifies last-bit differences far
+    beyond the bar) must still agree in status, states and counters, but
+    are left out of the parameter and covariance bars (`n_not_converged`)."""
     n = events.n_tracks
     rep = {"n_tracks": n}
     same_status = got.status == ref.status
@@ -59,6 +65,7 @@ def compare(got, ref, events, rel: float = 1e-10, rel_smoothed_cov: float = 5e-8
     same_ns = got.n_states == ref.n_states
     rep["n_states_mismatch"] = int(np.count_nonzero(both_ok & ~same_ns))
     good = both_ok & same_ns
+    rep["n_not_converged"] = 0
     for name in ("chi2", "ndf", "n_measurements", "n_holes", "n_outliers"):
         g, r = getattr(got, name), getattr(ref, name)
         if name == "chi2":
@@ -76,6 +83,12 @@ def compare(got, ref, events, rel: float = 1e-10, rel_smoothed_cov: float = 5e-8
         if ref.state_flags is not None and got.state_flags is not None:
             seq_bad_slots |= got.state_flags[idx] != ref.state_flags[idx]
     rep["state_sequence_mismatch_slots"] = int(np.count_nonzero(seq_bad_slots))
+    if max_chi2_ndf is not None:
+        converged = ref.chi2 / np.maximum(ref.ndf, 1) <= max_chi2_ndf
+        rep["n_not_converged"] = int(np.count_nonzero(good & ~converged))
+        good = good & converged
+        idx = np.concatenate([np.arange(so[t], so[t] + ref.n_states[t])
+                              for t in np.nonzero(good)[0]]) if good.any() else np.zeros(0, int)
     worst = {}
     for name in ("predicted", "filtered", "smoothed"):
         g, r = getattr(got, name), getattr(ref, name)
