@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libacts_kf.so")
 
 SOURCES = ["kf_capi.hip", "generator.hip"]
 HEADERS = ["kf_common.hpp", "kf_device.hpp", "kf_mixed.hpp", "kf_fit.hpp", "kf_kernel.hpp",
-           "kf_reference.hpp", "kf_field_host.hpp"]
+           "kf_reference.hpp", "kf_reverse.hpp", "kf_states.hpp", "kf_field_host.hpp"]
 
 # -ffp-contract=off: the FP64 fit keeps the reference's separate multiply and
 # add roundings (x86-64 without -march has no FMA), so every decision of the

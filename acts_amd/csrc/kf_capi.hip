@@ -65,11 +65,17 @@ constexpr int kMaxDevices = 64;
 // least `waitFrac`/64 of the busy lanes wait at a surface, or `maxInner`
 // steps have been taken; then the waiting lanes run the surface update
 // together, and finished lanes write their summary and go idle.  With Rev
-// the kernel runs the reverse filter of the tracks that take one.
-template <bool Mixed, bool GeoInLds, bool Sympy = false, bool Rev = false>
+// the kernel runs the reverse filter of the tracks that take one.  The
+// navigator is a template parameter (Layers: the layer navigator): its
+// option becomes a constant, so the direct navigator's instantiations carry
+// none of the layer navigator's code (a run-time choice cost the direct
+// fit's forward kernel 4 %).
+template <bool Mixed, bool GeoInLds, bool Sympy = false, bool Rev = false, bool Layers = false>
 __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKernel(
-    FwdArgs a, int32_t* __restrict__ next, int32_t maxInner, int32_t waitFrac,
+    FwdArgs args, int32_t* __restrict__ next, int32_t maxInner, int32_t waitFrac,
     int32_t quarantine, const int32_t* __restrict__ order) {
+  FwdArgs a = args;
+  a.o.navigator = Layers ? ACTS_KF_NAVIGATOR_LAYERS : ACTS_KF_NAVIGATOR_DIRECT;
   const int lane = static_cast<int>(threadIdx.x) & 63;
   const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t n = a.in.n;
@@ -570,7 +576,16 @@ bool geometryFitsLds(const acts_kf_ctx* c) {
 }
 
 template <bool Rev>
-ForwardFn forwardKernelOf(bool mixed, bool geoInLds, bool sympy) {
+ForwardFn forwardKernelOf(bool mixed, bool geoInLds, bool sympy, bool layers) {
+  // the layer navigator reads the geometry from global memory (it loops
+  // over every surface a few times per surface reached, from the cache)
+  if (layers) {
+    if (sympy)
+      return mixed ? fitForwardKernel<true, false, true, Rev, true>
+                   : fitForwardKernel<false, false, true, Rev, true>;
+    return mixed ? fitForwardKernel<true, false, false, Rev, true>
+                 : fitForwardKernel<false, false, false, Rev, true>;
+  }
   if (sympy && mixed)
     return geoInLds ? fitForwardKernel<true, true, true, Rev>
                     : fitForwardKernel<true, false, true, Rev>;
@@ -583,10 +598,12 @@ ForwardFn forwardKernelOf(bool mixed, bool geoInLds, bool sympy) {
   return geoInLds ? fitForwardKernel<false, true, false, Rev>
                   : fitForwardKernel<false, false, false, Rev>;
 }
-// the forward filter, or with `rev` the reverse filter
-ForwardFn forwardKernel(bool mixed, bool geoInLds, bool sympy = false, bool rev = false) {
-  return rev ? forwardKernelOf<true>(mixed, geoInLds, sympy)
-             : forwardKernelOf<false>(mixed, geoInLds, sympy);
+// the forward filter, or with `rev` the reverse filter, of a context
+ForwardFn forwardKernel(const acts_kf_ctx* c, bool rev = false) {
+  const bool lds = geometryFitsLds(c);
+  const bool layers = c->opts.navigator != ACTS_KF_NAVIGATOR_DIRECT;
+  return rev ? forwardKernelOf<true>(c->mixed, lds, c->sympy, layers)
+             : forwardKernelOf<false>(c->mixed, lds, c->sympy, layers);
 }
 
 }  // namespace
@@ -855,7 +872,7 @@ int acts_kf_create(const acts_kf_surface* surfaces, int32_t nSurfaces,
   {
     int blocksPerCu = 0, cus = 0;
     const hipError_t oe = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &blocksPerCu, forwardKernel(c->mixed, geometryFitsLds(c), c->sympy), kBlock, 0);
+        &blocksPerCu, forwardKernel(c), kBlock, 0);
     if (oe != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) !=
             hipSuccess ||
@@ -918,7 +935,7 @@ struct WorkspaceSizes {
 // 64-bit words of the layer navigator's reached-layer mask per track.
 static int64_t navMaskWords(const acts_kf_ctx* c) { return (c->nSurfaces + 63) / 64; }
 
-WorkspaceSizes workspaceSizes(const acts_kf_ctx* c, int64_t n, int32_t maxLen) {
+static WorkspaceSizes workspaceSizes(const acts_kf_ctx* c, int64_t n, int32_t maxLen) {
   WorkspaceSizes w;
   w.states = std::max(maxLen, 1);
   w.stride = (n + kBlock - 1) / kBlock * kBlock;
@@ -948,14 +965,14 @@ WorkspaceSizes workspaceSizes(const acts_kf_ctx* c, int64_t n, int32_t maxLen) {
 }
 
 // The reverse filter's records and reached flags in a block of w.rev bytes.
-void reverseSections(char* base, const WorkspaceSizes& w, Workspace& ws) {
+static void reverseSections(char* base, const WorkspaceSizes& w, Workspace& ws) {
   const size_t n = (w.rev - 256) / (sizeof(double) * kRevRecord + sizeof(int32_t));
   ws.rev = reinterpret_cast<double*>(base);
   ws.revReached = reinterpret_cast<int32_t*>(base + sizeof(double) * kRevRecord * n);
 }
 
 // The context's own (grow-only) workspace buffers, grown to `w`.
-int contextWorkspace(acts_kf_ctx* c, const WorkspaceSizes& w, Workspace& ws) {
+static int contextWorkspace(acts_kf_ctx* c, const WorkspaceSizes& w, Workspace& ws) {
   if (w.withRef && c->refBuf.ensure(w.ref))
     return fail(ACTS_KF_ERR_OUT_OF_MEMORY, "reference scratch allocation failed");
   if (c->scratchD.ensure(w.d) || c->scratchI.ensure(w.i) || c->scratchT.ensure(w.track) ||
@@ -1114,7 +1131,7 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
   hipLaunchKernelGGL(zeroKernel, dim3(1), dim3(64), 0, stream, queue, 1);
   const int64_t waves = std::min(blocks, c->residentWaves);
   if (timing) KF_HIP(hipEventRecord(c->ev[0], stream));
-  hipLaunchKernelGGL(forwardKernel(c->mixed, geometryFitsLds(c), c->sympy),
+  hipLaunchKernelGGL(forwardKernel(c),
                      dim3(static_cast<unsigned>(waves)), dim3(kBlock), 0, stream, fa, queue,
                      c->innerSteps, c->waitFrac, c->quarantine, perm);
   if (timing) KF_HIP(hipEventRecord(c->ev[1], stream));
@@ -1126,7 +1143,7 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
     // the reverse filter of the tracks that take one, in place of their
     // smoothing, before the extrapolation
     hipLaunchKernelGGL(zeroKernel, dim3(1), dim3(64), 0, stream, queue, 1);
-    hipLaunchKernelGGL(forwardKernel(c->mixed, geometryFitsLds(c), c->sympy, true),
+    hipLaunchKernelGGL(forwardKernel(c, true),
                        dim3(static_cast<unsigned>(waves)), dim3(kBlock), 0, stream, fa, queue,
                        c->innerSteps, c->waitFrac, 0, perm);
   }
