@@ -84,6 +84,10 @@ def parse():
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="time the library's default scheduling knobs instead of tuning "
                          "them (per precision, on a 100,000-track sample of the workload)")
+    ap.add_argument("--scheduling", default="",
+                    help="scheduling knobs as JSON instead of tuning them: one set for both "
+                         "precisions, or {\"fp64\": {...}, \"mixed\": {...}} as the "
+                         "output's config.scheduling prints them")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run a plain device copy of this size (for the "
                          "HBM-counter calibration of rocprof runs)")
@@ -386,6 +390,17 @@ class Bench:
         return ms, wall, kt, outs[0]
 
 
+def same_surface_tracks(sa, oa, na, sb, ob, nb) -> np.ndarray:
+    """Per track, whether two fits' state-surface sequences are equal
+    (surfaces sa/sb in slot layouts with offsets oa/ob, state counts na/nb)."""
+    eq = na == nb
+    n = np.where(eq, na, 0).astype(np.int64)
+    t = np.repeat(np.arange(len(n)), n)
+    k = np.arange(len(t)) - np.repeat(np.cumsum(n) - n, n)
+    bad = np.bincount(t, weights=(sa[oa[t] + k] != sb[ob[t] + k]), minlength=len(n))
+    return eq & (bad == 0)
+
+
 def summaries(res):
     class R:
         pass
@@ -485,7 +500,13 @@ def main() -> int:
     log(f"{n} tracks generated")
     wl_opts = A.workload_options(args.workload)
     tuned = {}
-    if args.input_order:
+    autotuned = False
+    if args.scheduling:
+        tuned = json.loads(args.scheduling)
+        if "fp64" in tuned:
+            tuned = {capi.PRECISION_FP64: tuned["fp64"],
+                     capi.PRECISION_MIXED: tuned.get("mixed", tuned["fp64"])}
+    elif args.input_order:
         tuned = {"length_sorted": 0}
     elif args.autotune:
         # the autotuner's coordinate descent over the scheduling knobs, per
@@ -498,6 +519,7 @@ def main() -> int:
             tuned[prec], _ = autotune(det, field, sample,
                                       dataclasses.replace(wl_opts, precision=prec),
                                       cache_path="")
+        autotuned = True
         log(f"autotuned in {time.perf_counter() - t0:.1f} s: {tuned}")
     b = Bench(args, det, field, events, wl_opts, tuned)
 
@@ -619,6 +641,26 @@ def main() -> int:
             "first_state_rel_err_vs_eigen": quality.relative_error(
                 quality.first_states(outs_, events), first64)}
         del outs_
+        # the layer navigator: surfaces found from the geometry instead of
+        # the given sequences, each track with 6 spare state slots
+        ev_l = A.with_state_slots(events, np.diff(events.seq_offsets) + 6)
+        bl = Bench(args, det, field, ev_l, wl_opts, tuned)
+        msl, _, ktl, outl = bl.run(capi.PRECISION_FP64, {"navigator": capi.NAVIGATOR_LAYERS},
+                                   ("state_surface",))
+        st_l = outl.host("status")
+        same = same_surface_tracks(out64.host("state_surface"), events.seq_offsets, r.n_states,
+                                   outl.host("state_surface"), ev_l.seq_offsets,
+                                   outl.host("n_states"))
+        both = (r.status == 0) & (st_l == 0)
+        line["with_layer_navigator"] = {
+            "value": n / (msl * 1e-3), "ms_per_step": msl,
+            "forward_kernel_ms": float(np.mean([k[0] for k in ktl])),
+            "note": "surfaces found from the geometry (acts_kf_options.navigator = LAYERS) "
+                    "instead of the given sequences",
+            "fit_failures": int(np.count_nonzero(st_l)),
+            "same_surfaces_as_direct": float(np.count_nonzero(same & both) /
+                                             max(np.count_nonzero(both), 1))}
+        del outl, bl
         if not args.no_mixed:
             msx, _, ktx, outx = b.run(capi.PRECISION_MIXED, {"stepper": capi.STEPPER_SYMPY})
             line["with_sympy_stepper"]["mixed"] = {
@@ -641,11 +683,14 @@ def main() -> int:
         del oute
         log("variants done")
 
-    if tuned:
-        line["config"]["scheduling"] = (
-            {"fp64": tuned[capi.PRECISION_FP64], "mixed": tuned[capi.PRECISION_MIXED],
-             "tuned_on": f"first {min(n, 100_000)} tracks (acts_amd.autotune)"}
-            if capi.PRECISION_FP64 in tuned else tuned)
+    if tuned and capi.PRECISION_FP64 not in tuned:
+        line["config"]["scheduling"] = tuned
+    elif tuned:
+        line["config"]["scheduling"] = {
+            "fp64": tuned[capi.PRECISION_FP64], "mixed": tuned[capi.PRECISION_MIXED],
+            "tuned_on": (f"first {min(n, 100_000)} tracks (acts_amd.autotune)" if autotuned
+                         else "given (--scheduling)")}
+    if autotuned:
         b.tuned = {}
         msd, _, _, _ = b.run(capi.PRECISION_FP64)
         b.tuned = tuned
