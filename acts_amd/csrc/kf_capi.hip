@@ -577,14 +577,18 @@ bool geometryFitsLds(const acts_kf_ctx* c) {
 
 template <bool Rev>
 ForwardFn forwardKernelOf(bool mixed, bool geoInLds, bool sympy, bool layers) {
-  // the layer navigator reads the geometry from global memory (it loops
-  // over every surface a few times per surface reached, from the cache)
   if (layers) {
+    if (sympy && mixed)
+      return geoInLds ? fitForwardKernel<true, true, true, Rev, true>
+                      : fitForwardKernel<true, false, true, Rev, true>;
     if (sympy)
-      return mixed ? fitForwardKernel<true, false, true, Rev, true>
-                   : fitForwardKernel<false, false, true, Rev, true>;
-    return mixed ? fitForwardKernel<true, false, false, Rev, true>
-                 : fitForwardKernel<false, false, false, Rev, true>;
+      return geoInLds ? fitForwardKernel<false, true, true, Rev, true>
+                      : fitForwardKernel<false, false, true, Rev, true>;
+    if (mixed)
+      return geoInLds ? fitForwardKernel<true, true, false, Rev, true>
+                      : fitForwardKernel<true, false, false, Rev, true>;
+    return geoInLds ? fitForwardKernel<false, true, false, Rev, true>
+                    : fitForwardKernel<false, false, false, Rev, true>;
   }
   if (sympy && mixed)
     return geoInLds ? fitForwardKernel<true, true, true, Rev>
