@@ -38,7 +38,7 @@ int fail(int code, const std::string& msg) {
 
 constexpr int kBlock = 64;
 // geometry kept in LDS by the forward kernel (224 B per surface)
-constexpr int kLdsSurfaces = 128;
+constexpr int kLdsSurfaces = 96;
 constexpr int kLdsSlabs = 32;
 // the track queue counts in int32 and may overshoot n by one per lane
 constexpr int64_t kMaxBatch = (int64_t{1} << 31) - (int64_t{1} << 24);
@@ -77,6 +77,23 @@ __global__ __launch_bounds__(kBlock, KF_FORWARD_WAVES_PER_EU) void fitForwardKer
   FwdArgs a = args;
   a.o.navigator = Layers ? ACTS_KF_NAVIGATOR_LAYERS : ACTS_KF_NAVIGATOR_DIRECT;
   const int lane = static_cast<int>(threadIdx.x) & 63;
+  // each lane's last field-map cell (cell-major maps): the first lookup of
+  // an RK step is nearly always in the cell of the previous step's last one
+  __shared__ double sCell[Mixed ? 1 : kCellValues * kCellCacheStride];
+  __shared__ float sCellF[Mixed ? kCellValues * kCellCacheStride : 1];
+  __shared__ int64_t sCellIdx[kCellCacheStride];
+  static_assert(kBlock == kCellCacheStride, "one cache column per lane of the workgroup");
+  a.F.cache = nullptr;
+  a.F.cacheF = nullptr;
+  a.F.cacheIdx = nullptr;
+  if (Mixed ? a.F.cellsF != nullptr : a.F.cells != nullptr) {
+    sCellIdx[lane] = -1;
+    if constexpr (Mixed)
+      a.F.cacheF = sCellF + lane;
+    else
+      a.F.cache = sCell + lane;
+    a.F.cacheIdx = sCellIdx + lane;
+  }
   const uint64_t lanesBelow = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int64_t n = a.in.n;
   // Surfaces and material slabs are read at every step: with GeoInLds (the

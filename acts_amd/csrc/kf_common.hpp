@@ -248,9 +248,17 @@ struct DevField {
   // padding) instead of four point pairs in different cache lines.
   const double* __restrict__ cells;
   const float* __restrict__ cellsF;
+  // The lane's copy of the last cell record it read (the reference's cached
+  // FieldCell, InterpolatedBFieldMap.hpp:354-366), in LDS, value j at
+  // cache[j * kCellCacheStride] (cacheF for the FP32 map), and that cell's
+  // index at cacheIdx[0]; null (host code, kernels without it): no cache.
+  double* cache;
+  float* cacheF;
+  int64_t* cacheIdx;
 };
 constexpr int kCellValues = 24;
 constexpr int kCellDoubles = 24;  // 192 B: two 128-byte cache lines
 constexpr int kCellFloats = 32;   // 128 B: one cache line
+constexpr int kCellCacheStride = 64;  // lanes of the wavefront sharing the LDS cache
 
 }  // namespace kf

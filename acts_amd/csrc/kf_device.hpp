@@ -363,12 +363,26 @@ KF_HD int gridField(const DevField& F, const V3& p, V3& out) {
   double c[4][3];
   if (F.cells) {
     const int64_t cell = (static_cast<int64_t>(lo[0]) * F.nb[1] + lo[1]) * F.nb[2] + lo[2];
-    const double* q = static_cast<const double*>(
-        __builtin_assume_aligned(F.cells + kCellDoubles * cell, 64));
+    double v[kCellValues];
+    if (F.cache && F.cacheIdx[0] == cell) {
+      // the lane's last cell: the same values without the gather
+#pragma unroll
+      for (int j = 0; j < kCellValues; ++j) v[j] = F.cache[j * kCellCacheStride];
+    } else {
+      const double* q = static_cast<const double*>(
+          __builtin_assume_aligned(F.cells + kCellDoubles * cell, 64));
+#pragma unroll
+      for (int j = 0; j < kCellValues; ++j) v[j] = q[j];
+      if (F.cache) {
+#pragma unroll
+        for (int j = 0; j < kCellValues; ++j) F.cache[j * kCellCacheStride] = v[j];
+        F.cacheIdx[0] = cell;
+      }
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) c[i][k] = (1 - f[2]) * q[6 * i + k] + f[2] * q[6 * i + 3 + k];
+      for (int k = 0; k < 3; ++k) c[i][k] = (1 - f[2]) * v[6 * i + k] + f[2] * v[6 * i + 3 + k];
   } else {
     const int64_t sy = F.nb[2] + 1;
     const int64_t sx = static_cast<int64_t>(F.nb[1] + 1) * (F.nb[2] + 1);

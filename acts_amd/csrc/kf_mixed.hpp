@@ -50,13 +50,26 @@ __device__ inline int gridFieldF(const DevField& F, const F3& p, F3& out) {
   if (F.cellsF) {
     // the cell's record: one 128-byte line
     const int64_t cell = (static_cast<int64_t>(lo[0]) * F.nb[1] + lo[1]) * F.nb[2] + lo[2];
-    const float* q = static_cast<const float*>(
-        __builtin_assume_aligned(F.cellsF + kCellFloats * cell, 128));
+    float v[kCellValues];
+    if (F.cacheF && F.cacheIdx[0] == cell) {
+#pragma unroll
+      for (int j = 0; j < kCellValues; ++j) v[j] = F.cacheF[j * kCellCacheStride];
+    } else {
+      const float* q = static_cast<const float*>(
+          __builtin_assume_aligned(F.cellsF + kCellFloats * cell, 128));
+#pragma unroll
+      for (int j = 0; j < kCellValues; ++j) v[j] = q[j];
+      if (F.cacheF) {
+#pragma unroll
+        for (int j = 0; j < kCellValues; ++j) F.cacheF[j * kCellCacheStride] = v[j];
+        F.cacheIdx[0] = cell;
+      }
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int k = 0; k < 3; ++k)
-        c[i][k] = (1.f - f[2]) * q[6 * i + k] + f[2] * q[6 * i + 3 + k];
+        c[i][k] = (1.f - f[2]) * v[6 * i + k] + f[2] * v[6 * i + 3 + k];
   } else {
     const int64_t sy = F.nb[2] + 1;
     const int64_t sx = static_cast<int64_t>(F.nb[1] + 1) * (F.nb[2] + 1);

@@ -83,7 +83,7 @@ def parse():
                     help="length of each loaded event-serving run (0 = skip serving)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="time the library's default scheduling knobs instead of tuning "
-                         "them (per precision, on a 100,000-track sample of the workload)")
+                         "them (per precision, on the batch itself)")
     ap.add_argument("--scheduling", default="",
                     help="scheduling knobs as JSON instead of tuning them: one set for both "
                          "precisions, or {\"fp64\": {...}, \"mixed\": {...}} as the "
@@ -448,8 +448,8 @@ def endcap_section(args):
 
 def ragged_section(args):
     """The ragged batch (6-30 measurements, holes, outliers) replayed from a
-    captured hipGraph with scheduling knobs chosen by the autotuner (on a
-    200,000-track sample of the same workload), FP64 and mixed."""
+    captured hipGraph with scheduling knobs chosen by the autotuner on the
+    batch itself, FP64 and mixed."""
     import acts_amd as A
     from acts_amd import capi
     from acts_amd.autotune import autotune
@@ -461,13 +461,12 @@ def ragged_section(args):
                         f"{int(lens.min())}-{int(lens.max())} measurements "
                         f"(mean {lens.mean():.1f}), 5 % holes, 10 % of tracks with 1-3 "
                         "outliers at 20 sigma (chi2 cut 25), uniform Bz = 2 T"}
-    sample = events.subset(np.arange(min(events.n_tracks, 200_000)))
     t0 = time.perf_counter()
     for name, prec in (("fp64", capi.PRECISION_FP64), ("mixed", capi.PRECISION_MIXED)):
         if name == "mixed" and args.no_mixed:
             continue
         o = dataclasses.replace(opts, precision=prec)
-        tuned, _ = autotune(det, field, sample, o, cache_path="")
+        tuned, _ = autotune(det, field, events, o, cache_path="")
         log(f"ragged {name}: tuned {tuned}")
         b = Bench(args, det, field, events, o, dict(tuned, use_graph=True))
         ms, _, _, _ = b.run(prec, streams=2, steps=max(3, args.steps // 4))
@@ -510,13 +509,15 @@ def main() -> int:
         tuned = {"length_sorted": 0}
     elif args.autotune:
         # the autotuner's coordinate descent over the scheduling knobs, per
-        # precision, on the first 100,000 tracks; the timed fits use them
+        # precision, on the batch itself (a 100,000-track sample of the
+        # field-map batch once chose knobs 30 % slower on the whole batch
+        # than the defaults: the tail of a small batch behaves differently);
+        # the timed fits use them
         from acts_amd.autotune import autotune
-        sample = events.subset(np.arange(min(n, 100_000)))
         t0 = time.perf_counter()
         tuned = {}
         for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
-            tuned[prec], _ = autotune(det, field, sample,
+            tuned[prec], _ = autotune(det, field, events,
                                       dataclasses.replace(wl_opts, precision=prec),
                                       cache_path="")
         autotuned = True
@@ -604,7 +605,9 @@ def main() -> int:
             "first_state_smoothed_rel_err_vs_fp64": quality.relative_error(firstm, first64)}
         del outm
         log(f"mixed: {msm:.3f} ms per fit in flight")
-    line["with_host_copies"] = host_copies(det, field, wl_opts, events, 5)
+    knobs64 = tuned.get(capi.PRECISION_FP64, {}) if capi.PRECISION_FP64 in tuned else tuned
+    line["with_host_copies"] = host_copies(det, field, dataclasses.replace(wl_opts, **knobs64),
+                                           events, 5)
     log("host copies done")
 
     if not args.no_variants:
@@ -688,8 +691,8 @@ def main() -> int:
     elif tuned:
         line["config"]["scheduling"] = {
             "fp64": tuned[capi.PRECISION_FP64], "mixed": tuned[capi.PRECISION_MIXED],
-            "tuned_on": (f"first {min(n, 100_000)} tracks (acts_amd.autotune)" if autotuned
-                         else "given (--scheduling)")}
+            "tuned_on": (f"the batch ({n} tracks, acts_amd.autotune, one fit at a time)"
+                         if autotuned else "given (--scheduling)")}
     if autotuned:
         b.tuned = {}
         msd, _, _, _ = b.run(capi.PRECISION_FP64)

@@ -176,7 +176,7 @@ def test_duplicate_measurement_first_wins(oracle_mod):
 @pytest.mark.parametrize("stepper", [capi.STEPPER_EIGEN, capi.STEPPER_SYMPY])
 def test_large_geometry(oracle_mod, precision, stepper):
     """300 more surfaces: the geometry no longer fits the forward kernel's
-    on-chip copy (128 surfaces) and is read from global memory, and the
+    on-chip copy (96 surfaces) and is read from global memory, and the
     measurement surfaces lie above 255.  The fit is bit for bit the one of
     the small geometry, with every state's surface index moved; in FP64 it
     also matches the oracle."""
