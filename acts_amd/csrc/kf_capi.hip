@@ -281,22 +281,10 @@ __device__ inline int32_t momentumKey(const QueueKeyArgs& k, int64_t t, int32_t 
                      kMomentumBins - 1);
   return min(len, kMomentumLenCap) * kMomentumBins + pb;
 }
-__device__ inline int32_t queueKey(const QueueKeyArgs& k, int64_t t) {
-  for (int64_t m = k.measOff[t]; m < k.measOff[t + 1]; ++m) {
-    const int32_t s = k.measSurf[m];
-    if (s < 0 || s >= k.nSurfaces || k.geo[s].type != ACTS_KF_SURFACE_CYLINDER) continue;
-    const double phi = k.measVal[2 * m] / k.geo[s].b0;
-    if (fabs(phi) > kPi - kSeamRisk) {
-      if (k.byDirection) return kDirectionKeys;
-      if (k.byMomentum) return (kMomentumLenCap + 1) * kMomentumBins;
-      return k.maxLen + 1;
-    }
-  }
-  if (k.byDirection) return directionKey(k, t);
-  const int32_t len = min(static_cast<int32_t>(k.seqOff[t + 1] - k.seqOff[t]), k.maxLen);
-  if (k.byMomentum) return momentumKey(k, t, len);
-  return len;
-}
+// A track's queue key: the seam key if one of its measurements on a
+// cylinder lies within kSeamRisk of phi = +-pi (such tracks may become
+// loopers and go first), else its direction key, or its sequence length
+// (capped at maxLen), or length and momentum (queueKeyKernel).
 // Counting sort of the tracks by queue key, largest key first, in three
 // passes without global atomics: per-block histograms (LDS atomics) stored
 // key-major with the largest key first, so that one exclusive scan over the
@@ -307,21 +295,68 @@ constexpr int kSortBlock = 256;
 constexpr int kScanBlock = 1024;
 constexpr int kMaxKeys = 1026;  // lengths above 1023 share a bucket
 
-__device__ inline int32_t sortKey(const QueueKeyArgs& k, int64_t t) {
-  return min(queueKey(k, t), kMaxKeys - 1);
-}
 __device__ inline int64_t histIndex(int32_t key, int32_t nKeys, int64_t block, int64_t nBlocks) {
   return static_cast<int64_t>(nKeys - 1 - key) * nBlocks + block;
 }
 
-__global__ __launch_bounds__(kSortBlock) void keyHistogramKernel(QueueKeyArgs k, int64_t n,
+// The sort key of every track (its queue key, capped at kMaxKeys - 1),
+// once: a workgroup takes
+// kSortBlock consecutive tracks, reads their measurements with consecutive
+// threads on consecutive measurements (a thread per track walking its own
+// measurements touched a different cache line per lane and load and fetched
+// ~7x the measurement arrays from HBM), marks in LDS the tracks with a
+// cylinder measurement near the seam (the owner found by binary search
+// over the workgroup's measurement offsets), then gives each track its key.
+__global__ __launch_bounds__(kSortBlock) void queueKeyKernel(QueueKeyArgs k, int64_t n,
+                                                             int32_t* keys) {
+  __shared__ int64_t off[kSortBlock + 1];
+  __shared__ int32_t seam[kSortBlock];
+  const int64_t t0 = static_cast<int64_t>(blockIdx.x) * kSortBlock;
+  const int nt = static_cast<int>(min<int64_t>(kSortBlock, n - t0));
+  for (int i = threadIdx.x; i <= nt; i += kSortBlock) off[i] = k.measOff[t0 + i];
+  seam[threadIdx.x] = 0;
+  __syncthreads();
+  for (int64_t m = off[0] + threadIdx.x; m < off[nt]; m += kSortBlock) {
+    const int32_t s = k.measSurf[m];
+    if (s < 0 || s >= k.nSurfaces || k.geo[s].type != ACTS_KF_SURFACE_CYLINDER) continue;
+    const double phi = k.measVal[2 * m] / k.geo[s].b0;
+    if (fabs(phi) > kPi - kSeamRisk) {
+      int lo = 0, hi = nt;  // off[lo] <= m < off[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (off[mid] <= m)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      seam[lo] = 1;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x >= nt) return;
+  const int64_t t = t0 + threadIdx.x;
+  int32_t key;
+  if (seam[threadIdx.x]) {
+    key = k.byDirection ? kDirectionKeys
+          : k.byMomentum ? (kMomentumLenCap + 1) * kMomentumBins
+                         : k.maxLen + 1;
+  } else if (k.byDirection) {
+    key = directionKey(k, t);
+  } else {
+    const int32_t len = min(static_cast<int32_t>(k.seqOff[t + 1] - k.seqOff[t]), k.maxLen);
+    key = k.byMomentum ? momentumKey(k, t, len) : len;
+  }
+  keys[t] = min(key, kMaxKeys - 1);
+}
+
+__global__ __launch_bounds__(kSortBlock) void keyHistogramKernel(const int32_t* keys, int64_t n,
                                                                  int32_t nKeys,
                                                                  int32_t* blockHist) {
   __shared__ int32_t h[kMaxKeys];
   for (int i = threadIdx.x; i < nKeys; i += kSortBlock) h[i] = 0;
   __syncthreads();
   const int64_t t = static_cast<int64_t>(blockIdx.x) * kSortBlock + threadIdx.x;
-  if (t < n) atomicAdd(&h[sortKey(k, t)], 1);
+  if (t < n) atomicAdd(&h[keys[t]], 1);
   __syncthreads();
   for (int i = threadIdx.x; i < nKeys; i += kSortBlock)
     blockHist[histIndex(i, nKeys, blockIdx.x, gridDim.x)] = h[i];
@@ -356,7 +391,7 @@ __global__ __launch_bounds__(kScanBlock) void exclusiveScanKernel(int32_t* a, in
   }
 }
 
-__global__ __launch_bounds__(kSortBlock) void keyScatterKernel(QueueKeyArgs k, int64_t n,
+__global__ __launch_bounds__(kSortBlock) void keyScatterKernel(const int32_t* keys, int64_t n,
                                                                int32_t nKeys,
                                                                const int32_t* blockOffset,
                                                                int32_t* perm) {
@@ -365,7 +400,7 @@ __global__ __launch_bounds__(kSortBlock) void keyScatterKernel(QueueKeyArgs k, i
   __syncthreads();
   const int64_t t = static_cast<int64_t>(blockIdx.x) * kSortBlock + threadIdx.x;
   if (t < n) {
-    const int32_t key = sortKey(k, t);
+    const int32_t key = keys[t];
     const int32_t rank = atomicAdd(&h[key], 1);
     perm[blockOffset[histIndex(key, nKeys, blockIdx.x, gridDim.x)] + rank] =
         static_cast<int32_t>(t);
@@ -977,7 +1012,8 @@ static WorkspaceSizes workspaceSizes(const acts_kf_ctx* c, int64_t n, int32_t ma
             : w.byMomentum ? (kMomentumLenCap + 1) * kMomentumBins + 1
                            : std::min(w.states + 2, kMaxKeys);
   w.sortBlocks = (n + kSortBlock - 1) / kSortBlock;
-  w.perm = w.sorted ? sizeof(int32_t) * n : 0;
+  // the queue permutation, then the tracks' sort keys
+  w.perm = w.sorted ? sizeof(int32_t) * 2 * n : 0;
   w.hist = w.sorted ? sizeof(int32_t) * w.nKeys * w.sortBlocks : 0;
   w.rev = c->revEnabled
               ? (sizeof(double) * kRevRecord + sizeof(int32_t)) * static_cast<size_t>(n) + 256
@@ -1120,12 +1156,15 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
     qk.nSurfaces = c->nSurfaces;
     qk.maxLen = states;
     const unsigned nb = static_cast<unsigned>(sortBlocks);
-    hipLaunchKernelGGL(keyHistogramKernel, dim3(nb), dim3(kSortBlock), 0, stream, qk, n, nKeys,
-                       hist);
+    // the keys after the permutation in the same buffer (workspaceSizes)
+    int32_t* keys = perm + n;
+    hipLaunchKernelGGL(queueKeyKernel, dim3(nb), dim3(kSortBlock), 0, stream, qk, n, keys);
+    hipLaunchKernelGGL(keyHistogramKernel, dim3(nb), dim3(kSortBlock), 0, stream, keys, n,
+                       nKeys, hist);
     hipLaunchKernelGGL(exclusiveScanKernel, dim3(1), dim3(kScanBlock), 0, stream, hist,
                        static_cast<int64_t>(nKeys) * sortBlocks);
-    hipLaunchKernelGGL(keyScatterKernel, dim3(nb), dim3(kSortBlock), 0, stream, qk, n, nKeys,
-                       hist, perm);
+    hipLaunchKernelGGL(keyScatterKernel, dim3(nb), dim3(kSortBlock), 0, stream, keys, n,
+                       nKeys, hist, perm);
   }
   FwdArgs fa;
   fa.in = b;
