@@ -270,6 +270,16 @@ def load_library(path: str = LIB_PATH) -> C.CDLL:
         raise RuntimeError(
             f"{path} not found: build it with `python -m acts_amd.build` "
             "(there is no CPU fallback)")
+    # PyTorch for ROCm bundles its own HIP runtime and links it as
+    # libamdhip64.so, a name the library's dependency (libamdhip64.so.7)
+    # does not match: imported after the library, torch maps a second
+    # runtime into the process, and whichever initialises second finds no
+    # device.  Imported first, its runtime (soname libamdhip64.so.7) is the
+    # one the library binds to.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -3,16 +3,19 @@
 The reference processes events with the Examples' Sequencer, one
 TrackFittingAlgorithm::execute per event, fitting the event's tracks one by
 one (Examples/Algorithms/TrackFitting/src/TrackFittingAlgorithm.cpp:117-173).
-Here events arrive as a Poisson stream; a dispatcher groups the events that
-have arrived into batches, admitted by an HBM budget per batch, and hands
-each batch to a free slot.  A slot is one fitter context (own stream,
-staging buffers and scratch, geometry and field resident) with pinned host
-buffers; its worker thread copies the batch's tracks into them, fits the
-batch (host-to-device copies, kernels, device-to-host copies on the slot's
-stream), leaving the results (surface, flags and smoothed parameters of
-every state, track summaries) in pinned memory, and stamps every event
-of the batch with its completion time.  Latency of an event = completion -
-arrival.
+Here events arrive as a Poisson stream into a ring of pinned host memory
+(the reader's buffers: each event's tracks follow the previous event's, and
+an event that would run past the end of the ring starts again at its
+beginning).  A dispatcher groups the events that have arrived into batches
+of consecutive events, admitted by an HBM budget, and hands each batch to a
+free slot.  A slot is one fitter context (own stream, staging buffers and
+scratch, geometry and field resident) with pinned output buffers; its
+worker thread fits the batch straight from the ring (only the two offset
+arrays are rebased to the batch's first track, into pinned arrays of the
+slot; the library's host-to-device copies read the ring by DMA), leaving
+the results (surface, flags and smoothed parameters of every state, track
+summaries) in pinned memory, and stamps every event of the batch with its
+completion time.  Latency of an event = completion - arrival.
 
 Admission: a batch is admitted when a slot is free and its device memory
 (inputs, per-state scratch, outputs: `batch_bytes`) fits, together with the
@@ -56,7 +59,11 @@ class ServingConfig:
 
 
 def event_stream(cfg: ServingConfig, pool_tracks: int):
-    """(arrival times [s], multiplicities, offsets into the track pool)."""
+    """(arrival times [s], multiplicities, offsets into the track pool).
+
+    The pool is the ring the events are read into: event e's tracks follow
+    event e-1's, and an event that would run past the end of the pool
+    starts at its beginning."""
     rng = np.random.default_rng(cfg.seed)
     n_max = int(cfg.rate_hz * cfg.duration_s * 2 + 16)
     gaps = rng.exponential(1.0 / cfg.rate_hz, size=n_max)
@@ -65,7 +72,13 @@ def event_stream(cfg: ServingConfig, pool_tracks: int):
     m = np.exp(rng.normal(np.log(cfg.median_tracks), cfg.sigma, size=len(t)))
     m = np.clip(np.round(m), cfg.min_tracks, cfg.max_tracks).astype(np.int64)
     m = np.minimum(m, pool_tracks)
-    off = rng.integers(0, pool_tracks - m + 1)
+    off = np.zeros(len(t), np.int64)
+    cursor = int(rng.integers(0, pool_tracks))
+    for e in range(len(t)):
+        if cursor + m[e] > pool_tracks:
+            cursor = 0
+        off[e] = cursor
+        cursor += int(m[e])
     return t, m, off
 
 
@@ -104,36 +117,53 @@ def batch_bytes(n_tracks: int, max_len: int) -> float:
     return n_tracks * (per_track + max_len * per_slot)
 
 
+def _pinned(n: int, w: int, dt) -> np.ndarray:
+    """An uninitialised numpy array in pinned host memory."""
+    import torch
+    tdt = {np.dtype(np.float64): torch.float64, np.dtype(np.int32): torch.int32,
+           np.dtype(np.int64): torch.int64, np.dtype(np.float32): torch.float32,
+           np.dtype(np.uint8): torch.uint8}[np.dtype(dt)]
+    shape = (n,) if w == 1 else (n, w)
+    return torch.empty(shape, dtype=tdt, pin_memory=True).numpy()
+
+
+RING_ARRAYS = ("start_params", "start_cov", "start_plane", "seq_offsets", "seq_surfaces",
+               "meas_offsets", "meas_surfaces", "meas_values", "meas_cov", "meas_dims",
+               "meas_indices")
+
+
+def pinned_ring(pool: Events) -> Events:
+    """The pool's fit inputs copied into pinned host memory: the ring the
+    events are read into."""
+    arrs = {}
+    for name in RING_ARRAYS:
+        a = getattr(pool, name)
+        if a is None:
+            arrs[name] = None
+            continue
+        a = np.ascontiguousarray(a)
+        w = 1 if a.ndim == 1 else a.shape[1]
+        arrs[name] = _pinned(a.shape[0], w, a.dtype)
+        arrs[name][...] = a
+    return Events(n_tracks=pool.n_tracks, truth_start=None, truth_meas=None,
+                  meas_is_outlier=None, **arrs)
+
+
 class _Slot:
     """One in-flight batch: a fitter context plus pinned host buffers for the
-    batch's inputs and outputs, reused from batch to batch."""
+    batch's rebased offsets and its outputs, reused from batch to batch."""
 
-    IN = (("start_params", 6, np.float64, "t"), ("start_cov", 36, np.float64, "t"),
-          ("start_plane", 12, np.float64, "t"), ("seq_surfaces", 1, np.int32, "s"),
-          ("meas_surfaces", 1, np.int32, "m"), ("meas_values", 2, np.float64, "m"),
-          ("meas_cov", 4, np.float64, "m"))
-
-    def __init__(self, fitter: KalmanFitter, cap_tracks: int, cap_seq: int, cap_meas: int):
-        import torch
-
-        def pinned(n, w, dt):
-            tdt = {np.float64: torch.float64, np.int32: torch.int32,
-                   np.int64: torch.int64, np.float32: torch.float32}[dt]
-            shape = (n,) if w == 1 else (n, w)
-            return torch.empty(shape, dtype=tdt, pin_memory=True).numpy()
-
+    def __init__(self, fitter: KalmanFitter, cap_tracks: int, cap_seq: int):
         self.fitter = fitter
-        caps = {"t": cap_tracks, "s": cap_seq, "m": cap_meas}
-        self.arr = {name: pinned(caps[k], w, dt) for name, w, dt, k in self.IN}
-        self.arr["seq_offsets"] = pinned(cap_tracks + 1, 1, np.int64)
-        self.arr["meas_offsets"] = pinned(cap_tracks + 1, 1, np.int64)
-        self.out = {name: pinned(cap_tracks, 1, np.int32)
+        self.seq_offsets = _pinned(cap_tracks + 1, 1, np.int64)
+        self.meas_offsets = _pinned(cap_tracks + 1, 1, np.int64)
+        self.out = {name: _pinned(cap_tracks, 1, np.int32)
                     for name in ("status", "n_states", "ndf", "n_measurements", "n_holes",
                                  "n_outliers", "n_steps", "n_step_trials")}
-        self.out["chi2"] = pinned(cap_tracks, 1, np.float32)
-        self.out["state_surface"] = pinned(cap_seq, 1, np.int32)
-        self.out["state_flags"] = pinned(cap_seq, 1, np.int32)
-        self.out["smoothed"] = pinned(cap_seq, 6, np.float64)
+        self.out["chi2"] = _pinned(cap_tracks, 1, np.float32)
+        self.out["state_surface"] = _pinned(cap_seq, 1, np.int32)
+        self.out["state_flags"] = _pinned(cap_seq, 1, np.int32)
+        self.out["smoothed"] = _pinned(cap_seq, 6, np.float64)
 
     def results(self, t0: int, cnt: int, s0: int, ns: int) -> dict:
         """Copies of the results of the batch's tracks t0..t0+cnt-1, whose
@@ -146,33 +176,27 @@ class _Slot:
         out["state_flags"] = out["state_flags"].view(np.uint32)
         return out
 
-    def fit(self, pool: Events, ranges) -> int:
-        """Copy the tracks pool[off:off+m] of `ranges` into the pinned inputs,
-        fit them, and leave the results in the pinned outputs."""
-        so, mo = pool.seq_offsets, pool.meas_offsets
-        a = self.arr
-        t = s = m = 0
-        a["seq_offsets"][0] = 0
-        a["meas_offsets"][0] = 0
-        for off, cnt in ranges:
-            lo, hi = off, off + cnt
-            ns, nm = int(so[hi] - so[lo]), int(mo[hi] - mo[lo])
-            a["start_params"][t:t + cnt] = pool.start_params[lo:hi]
-            a["start_cov"][t:t + cnt] = pool.start_cov[lo:hi]
-            a["start_plane"][t:t + cnt] = pool.start_plane[lo:hi]
-            a["seq_surfaces"][s:s + ns] = pool.seq_surfaces[so[lo]:so[hi]]
-            a["meas_surfaces"][m:m + nm] = pool.meas_surfaces[mo[lo]:mo[hi]]
-            a["meas_values"][m:m + nm] = pool.meas_values[mo[lo]:mo[hi]]
-            a["meas_cov"][m:m + nm] = pool.meas_cov[mo[lo]:mo[hi]]
-            a["seq_offsets"][t + 1:t + cnt + 1] = so[lo + 1:hi + 1] - so[lo] + s
-            a["meas_offsets"][t + 1:t + cnt + 1] = mo[lo + 1:hi + 1] - mo[lo] + m
-            t, s, m = t + cnt, s + ns, m + nm
+    def fit(self, ring: Events, lo: int, n: int) -> int:
+        """Fit the tracks ring[lo:lo+n] in place (inputs read from the pinned
+        ring, offsets rebased to the first track) and leave the results in
+        the pinned outputs."""
+        so, mo = ring.seq_offsets, ring.meas_offsets
+        hi = lo + n
+        s0, m0 = int(so[lo]), int(mo[lo])
+        np.subtract(so[lo:hi + 1], s0, out=self.seq_offsets[:n + 1])
+        np.subtract(mo[lo:hi + 1], m0, out=self.meas_offsets[:n + 1])
         bin_ = capi.BatchIn()
-        bin_.n_tracks = t
-        for name in ("start_params", "start_cov", "start_plane", "seq_offsets", "seq_surfaces",
-                     "meas_offsets", "meas_surfaces", "meas_values", "meas_cov"):
-            setattr(bin_, name, a[name].ctypes.data)
-        bin_.max_sequence_length = int(np.max(np.diff(a["seq_offsets"][:t + 1]))) if t else 0
+        bin_.n_tracks = n
+        row = {"start_params": lo, "start_cov": lo, "start_plane": lo, "seq_surfaces": s0,
+               "meas_surfaces": m0, "meas_values": m0, "meas_cov": m0, "meas_dims": m0,
+               "meas_indices": m0}
+        for name, r in row.items():
+            a = getattr(ring, name)
+            if a is not None:
+                setattr(bin_, name, a.ctypes.data + r * a.strides[0])
+        bin_.seq_offsets = self.seq_offsets.ctypes.data
+        bin_.meas_offsets = self.meas_offsets.ctypes.data
+        bin_.max_sequence_length = int(np.max(np.diff(self.seq_offsets[:n + 1]))) if n else 0
         bout = capi.BatchOut()
         for name in capi.BATCH_OUT_FIELDS:
             arr = self.out.get(name)
@@ -181,28 +205,31 @@ class _Slot:
         capi.check(lib.acts_kf_fit_batch(ctx, C.byref(bin_), C.byref(bout), None),
                    "acts_kf_fit_batch")
         capi.check(lib.acts_kf_sync(ctx), "acts_kf_sync")
-        return t
+        return n
 
 
 def serve(detector, field, pool: Events, cfg: ServingConfig,
-          options: KalmanFitterOptions | None = None, device: int = 0) -> dict:
+          options: KalmanFitterOptions | None = None, device: int = 0,
+          ring: Events | None = None) -> dict:
     """Run the serving loop; returns throughput and latency figures (and,
     with cfg.keep_results, each event's results under "results" and its
-    tracks as pool ranges under "event_ranges")."""
+    tracks as pool ranges under "event_ranges").  `ring`: the pool already
+    in pinned memory (pinned_ring(pool)), to reuse it across runs."""
     options = options or KalmanFitterOptions()
     arrivals, mult, offs = event_stream(cfg, pool.n_tracks)
     n_ev = len(arrivals)
     done_at = np.full(n_ev, np.nan)
     max_len = max(pool.max_sequence_length, 1)
-    max_meas = int(np.max(np.diff(pool.meas_offsets))) if pool.n_tracks else 1
     cap_tracks = max(cfg.max_tracks,
                      min(cfg.max_batch_tracks, int(cfg.hbm_budget_bytes // batch_bytes(1, max_len))))
 
+    if ring is None:
+        ring = pinned_ring(pool)
     slots = [_Slot(KalmanFitter(detector, field, options, device=device), cap_tracks,
-                   cap_tracks * max_len, cap_tracks * max_meas) for _ in range(cfg.slots)]
+                   cap_tracks * max_len) for _ in range(cfg.slots)]
     # warm each context up to the largest batch it may get (scratch sizes)
     for sl in slots:
-        sl.fit(pool, [(0, min(cap_tracks, pool.n_tracks))])
+        sl.fit(ring, 0, min(cap_tracks, pool.n_tracks))
 
     free = queue.Queue()
     for i in range(cfg.slots):
@@ -222,7 +249,7 @@ def serve(detector, field, pool: Events, cfg: ServingConfig,
             if item is None:
                 return
             ids, nbytes = item
-            n = sl.fit(pool, [(int(offs[e]), int(mult[e])) for e in ids])
+            n = sl.fit(ring, int(offs[ids[0]]), int(mult[ids].sum()))
             done_at[ids] = time.perf_counter() - t0[0]
             if cfg.keep_results:
                 t = s = 0
@@ -259,6 +286,9 @@ def serve(detector, field, pool: Events, cfg: ServingConfig,
                     e = nxt + len(ids)
                     b = batch_bytes(tracks + int(mult[e]), max_len)
                     if ids and tracks + mult[e] > cap_tracks:
+                        break
+                    # a batch is a contiguous range of the ring
+                    if ids and offs[e] != offs[e - 1] + mult[e - 1]:
                         break
                     # admitted if it fits the budget left, or if nothing
                     # else is in flight (an event above the whole budget)

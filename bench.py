@@ -146,7 +146,8 @@ def serving(args, det, field, pool):
     and 0.95 of the FP64 capacity, to both precisions."""
     import acts_amd as A
     from acts_amd import capi
-    from acts_amd.serving import ServingConfig, serve
+    from acts_amd.serving import ServingConfig, pinned_ring, serve
+    ring = pinned_ring(pool)  # the reader's pinned buffers, shared by the runs
     mean_tracks = 5000 * np.exp(0.7 ** 2 / 2)
     precs = (("fp64", capi.PRECISION_FP64), ("mixed", capi.PRECISION_MIXED))
     base = ServingConfig(seed=args.seed)
@@ -161,7 +162,7 @@ def serving(args, det, field, pool):
     for name, prec in precs:
         # enough events for >= 1 s at ~10 M tracks/s, all arrived at once
         sat_cfg = dataclasses.replace(base, rate_hz=4e4, duration_s=0.05)
-        sat = serve(det, field, pool, sat_cfg, A.KalmanFitterOptions(precision=prec))
+        sat = serve(det, field, pool, sat_cfg, A.KalmanFitterOptions(precision=prec), ring=ring)
         cap[name] = sat["throughput_tracks_per_s"]
         log(f"serving {name}: capacity {cap[name]:.3g} tracks/s")
         out[f"saturated_{name}"] = sat
@@ -172,7 +173,7 @@ def serving(args, det, field, pool):
         cfg = dataclasses.replace(base, rate_hz=rate, duration_s=args.serving_seconds)
         entry = {"rate_hz": rate}
         for name, prec in precs:
-            r = serve(det, field, pool, cfg, A.KalmanFitterOptions(precision=prec))
+            r = serve(det, field, pool, cfg, A.KalmanFitterOptions(precision=prec), ring=ring)
             entry[name] = {k: r[k] for k in ("throughput_tracks_per_s", "latency_ms_p50",
                                              "latency_ms_p99", "latency_ms_max", "events",
                                              "batches", "mean_batch_tracks",

@@ -26,3 +26,19 @@ def test_batch_of_matches_subset():
               "meas_offsets", "meas_surfaces", "meas_values", "meas_cov"):
         assert np.array_equal(getattr(got, k), getattr(ref, k)), k
     assert got.n_tracks == ref.n_tracks == len(idx)
+
+
+def test_event_stream_fills_the_ring_in_arrival_order():
+    """Each event's tracks follow the previous event's in the pool, except
+    where the event would run past the end of the pool: then it starts at
+    the beginning."""
+    pool = 30000
+    cfg = ServingConfig(rate_hz=2000.0, duration_s=1.0, median_tracks=5000, seed=9)
+    t, m, off = event_stream(cfg, pool_tracks=pool)
+    nxt = off[:-1] + m[:-1]
+    cont = off[1:] == nxt
+    wrap = off[1:] == 0
+    assert np.all(cont | wrap)
+    # a wrap only where the event would not have fitted after the previous one
+    assert np.all(nxt[wrap & ~cont] + m[1:][wrap & ~cont] > pool)
+    assert np.count_nonzero(wrap) >= 10 and np.all(off + m <= pool)
