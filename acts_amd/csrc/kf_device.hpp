@@ -344,12 +344,19 @@ KF_HD int gridLocate(const DevField& F, const V3& p, int lo[3], double f[3]) {
   for (int a = 0; a < 3; ++a) {
     if (!(x[a] >= F.min[a]) || x[a] >= F.upper[a]) return ACTS_KF_STATUS_FIELD_OUT_OF_BOUNDS;
     // 1 <= bin <= nb: x >= min, and x < upper up to one rounding
-    int bin = static_cast<int>(floor((x[a] - F.min[a]) / F.width[a]) + 1);
+    const double u = F.recip ? divRecip(x[a] - F.min[a], F.width[a], F.rw[a])
+                             : (x[a] - F.min[a]) / F.width[a];
+    int bin = static_cast<int>(floor(u) + 1);
     bin = bin < 1 ? 1 : (bin > F.nb[a] ? F.nb[a] : bin);
     lo[a] = bin - 1;
     const double ll = F.min[a] + static_cast<double>(bin - 1) * F.width[a];
-    const double ur = F.min[a] + static_cast<double>(bin) * F.width[a];
-    f[a] = (x[a] - ll) / (ur - ll);
+    if (F.recip) {
+      // ur - ll is den[a] in every bin
+      f[a] = divRecip(x[a] - ll, F.den[a], F.rden[a]);
+    } else {
+      const double ur = F.min[a] + static_cast<double>(bin) * F.width[a];
+      f[a] = (x[a] - ll) / (ur - ll);
+    }
   }
   return 0;
 }
