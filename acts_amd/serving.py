@@ -52,8 +52,13 @@ class ServingConfig:
     min_tracks: int = 500
     max_tracks: int = 20000
     slots: int = 3                   # fitter contexts: most batches in flight
-    hbm_budget_bytes: float = 16e9   # device memory of all batches in flight
-    max_batch_tracks: int = 60000    # latency bound: tracks per batch
+    hbm_budget_bytes: float = 32e9   # device memory of all batches in flight
+    # tracks per batch: a batch's forward kernel lasts as long as its slowest
+    # track (a looper that ends at max_steps: ~9 ms in the field map whether
+    # the batch holds 48 000 tracks or 500 000), so large batches amortise
+    # it; measured saturated capacity in the field map with 3 slots: 6.9 M
+    # tracks/s at 60 000, 8.1 M at 150 000, 13.8 M at 300 000
+    max_batch_tracks: int = 300000
     seed: int = 1
     keep_results: bool = False       # keep every event's results (tests)
 
@@ -202,9 +207,12 @@ class _Slot:
             arr = self.out.get(name)
             setattr(bout, name, arr.ctypes.data if arr is not None else None)
         lib, ctx = self.fitter.lib, self.fitter.ctx
+        t0 = time.perf_counter()
         capi.check(lib.acts_kf_fit_batch(ctx, C.byref(bin_), C.byref(bout), None),
                    "acts_kf_fit_batch")
+        t1 = time.perf_counter()
         capi.check(lib.acts_kf_sync(ctx), "acts_kf_sync")
+        self.last_ms = ((t1 - t0) * 1e3, (time.perf_counter() - t0) * 1e3)
         return n
 
 
@@ -236,7 +244,7 @@ def serve(detector, field, pool: Events, cfg: ServingConfig,
         free.put(i)
     work = [queue.Queue() for _ in range(cfg.slots)]
     stats = {"batches": 0, "batch_tracks": [], "busy": 0, "max_busy": 0,
-             "inflight": 0.0, "max_inflight": 0.0}
+             "inflight": 0.0, "max_inflight": 0.0, "submit_ms": [], "fit_ms": []}
     results = {}
     cond = threading.Condition()
     t0 = [0.0]
@@ -261,6 +269,8 @@ def serve(detector, field, pool: Events, cfg: ServingConfig,
             with cond:
                 stats["batches"] += 1
                 stats["batch_tracks"].append(n)
+                stats["submit_ms"].append(sl.last_ms[0])
+                stats["fit_ms"].append(sl.last_ms[1])
                 stats["busy"] -= 1
                 stats["inflight"] -= nbytes
                 cond.notify_all()
@@ -322,6 +332,10 @@ def serve(detector, field, pool: Events, cfg: ServingConfig,
            "latency_ms_max": float(np.max(lat)),
            "batches": stats["batches"],
            "mean_batch_tracks": float(np.mean(stats["batch_tracks"])),
+           # per batch: host time of the fit call (copies and launches
+           # queued), and from the call to results in host memory
+           "mean_batch_submit_ms": float(np.mean(stats["submit_ms"])),
+           "mean_batch_fit_ms": float(np.mean(stats["fit_ms"])),
            "max_batches_in_flight": stats["max_busy"],
            "max_inflight_bytes": stats["max_inflight"],
            "hbm_budget_bytes": cfg.hbm_budget_bytes,

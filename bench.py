@@ -160,8 +160,8 @@ def serving(args, det, field, pool):
                       "duration_s": args.serving_seconds}}
     cap = {}
     for name, prec in precs:
-        # enough events for >= 1 s at ~10 M tracks/s, all arrived at once
-        sat_cfg = dataclasses.replace(base, rate_hz=4e4, duration_s=0.05)
+        # enough events for >= 1 s at ~15 M tracks/s, all arrived at once
+        sat_cfg = dataclasses.replace(base, rate_hz=4e4, duration_s=0.1)
         sat = serve(det, field, pool, sat_cfg, A.KalmanFitterOptions(precision=prec), ring=ring)
         cap[name] = sat["throughput_tracks_per_s"]
         log(f"serving {name}: capacity {cap[name]:.3g} tracks/s")
@@ -711,10 +711,10 @@ def main() -> int:
 
     if args.serving_seconds > 0:
         if args.workload == "solenoid" and args.n_tracks is None:
-            pool = events.subset(np.arange(200_000))
+            pool = events
             sdet, sfield = det, field
         else:
-            sdet, sfield, pool = A.workload("solenoid", n_tracks=200_000, seed=args.seed + 1)
+            sdet, sfield, pool = A.workload("solenoid", n_tracks=500_000, seed=args.seed + 1)
         line["serving"] = serving(args, sdet, sfield, pool)
         log("serving done")
 
