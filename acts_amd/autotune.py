@@ -32,6 +32,7 @@ from __future__ import annotations
 import dataclasses
 import json
 import os
+import time
 from typing import Callable
 
 import numpy as np
@@ -90,10 +91,17 @@ def default_cache_path() -> str:
 
 
 def autotune(detector, field, events, options=None, steps: int = 3, warmup: int = 1,
-             cache_path: str | None = None, device: int = 0) -> tuple[dict, dict]:
+             cache_path: str | None = None, device: int = 0,
+             in_flight: int = 1) -> tuple[dict, dict]:
     """Tune the scheduling knobs on `events` (fitted on `device`); returns
     (best settings, measured milliseconds per setting).  Uses and updates the
-    cache unless cache_path is the empty string."""
+    cache unless cache_path is the empty string.
+
+    in_flight = 1 times one fit at a time; in_flight = k times the fits
+    spread round-robin over k contexts, each on its own stream with its own
+    outputs (the way an event loop keeps fits in flight), per fit.  The two
+    can prefer different knobs: an order that starts the slowest tracks last
+    leaves a long tail to one fit but fills it with the next fit's bulk."""
     import torch
     from .device import DeviceEvents, DeviceResult
     from .fitter import KalmanFitter, KalmanFitterOptions
@@ -101,6 +109,8 @@ def autotune(detector, field, events, options=None, steps: int = 3, warmup: int 
     options = options or KalmanFitterOptions()
     dev = torch.device(f"cuda:{device}")
     sig = signature(events, field, options, torch.cuda.get_device_name(dev))
+    if in_flight > 1:
+        sig = json.dumps({"sig": sig, "in_flight": in_flight}, sort_keys=True)
     path = default_cache_path() if cache_path is None else cache_path
     cache = {}
     if path and os.path.exists(path):
@@ -110,24 +120,41 @@ def autotune(detector, field, events, options=None, steps: int = 3, warmup: int 
             return cache[sig]["best"], cache[sig]["ms"]
 
     din = DeviceEvents(events, dev)
-    dout = DeviceResult(events.n_tracks, din.n_slots, ("smoothed",), dev)
+    douts = [DeviceResult(events.n_tracks, din.n_slots, ("smoothed",), dev)
+             for _ in range(max(1, in_flight))]
     stream = torch.cuda.Stream(device=dev)
 
     def evaluate(cfg):
-        fitter = KalmanFitter(detector, field, dataclasses.replace(options, **cfg), device=device)
+        opts = dataclasses.replace(options, **cfg)
+        fitters = [KalmanFitter(detector, field, opts, device=device)
+                   for _ in range(max(1, in_flight))]
         try:
-            for _ in range(warmup):
-                fitter.fit_device(din.batch_in(), dout.batch_out(), stream.cuda_stream)
+            if in_flight <= 1:
+                fitter, dout = fitters[0], douts[0]
+                for _ in range(warmup):
+                    fitter.fit_device(din.batch_in(), dout.batch_out(), stream.cuda_stream)
+                torch.cuda.synchronize()
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                for _ in range(steps):
+                    fitter.fit_device(din.batch_in(), dout.batch_out(), stream.cuda_stream)
+                b.record(stream)
+                torch.cuda.synchronize()
+                return a.elapsed_time(b) / steps
+            # the contexts' own streams (None), each fit with its own outputs
+            for f, do in zip(fitters, douts):
+                for _ in range(warmup):
+                    f.fit_device(din.batch_in(), do.batch_out(), None)
             torch.cuda.synchronize()
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record(stream)
-            for _ in range(steps):
-                fitter.fit_device(din.batch_in(), dout.batch_out(), stream.cuda_stream)
-            b.record(stream)
+            t0 = time.perf_counter()
+            for i in range(steps * in_flight):
+                k = i % in_flight
+                fitters[k].fit_device(din.batch_in(), douts[k].batch_out(), None)
             torch.cuda.synchronize()
-            return a.elapsed_time(b) / steps
+            return (time.perf_counter() - t0) * 1e3 / (steps * in_flight)
         finally:
-            fitter.close()
+            for f in fitters:
+                f.close()
 
     best, seen = coordinate_descent(evaluate)
     ms = {k: v for k, v in seen.items()}

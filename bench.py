@@ -467,7 +467,7 @@ def ragged_section(args):
         if name == "mixed" and args.no_mixed:
             continue
         o = dataclasses.replace(opts, precision=prec)
-        tuned, _ = autotune(det, field, events, o, cache_path="")
+        tuned, _ = autotune(det, field, events, o, cache_path="", in_flight=2)
         log(f"ragged {name}: tuned {tuned}")
         b = Bench(args, det, field, events, o, dict(tuned, use_graph=True))
         ms, _, _, _ = b.run(prec, streams=2, steps=max(3, args.steps // 4))
@@ -512,15 +512,16 @@ def main() -> int:
         # the autotuner's coordinate descent over the scheduling knobs, per
         # precision, on the batch itself (a 100,000-track sample of the
         # field-map batch once chose knobs 30 % slower on the whole batch
-        # than the defaults: the tail of a small batch behaves differently);
-        # the timed fits use them
+        # than the defaults: the tail of a small batch behaves differently),
+        # timed with as many fits in flight as the timed steps keep; the
+        # timed fits use them
         from acts_amd.autotune import autotune
         t0 = time.perf_counter()
         tuned = {}
         for prec in (capi.PRECISION_FP64, capi.PRECISION_MIXED):
             tuned[prec], _ = autotune(det, field, events,
                                       dataclasses.replace(wl_opts, precision=prec),
-                                      cache_path="")
+                                      cache_path="", in_flight=args.streams)
         autotuned = True
         log(f"autotuned in {time.perf_counter() - t0:.1f} s: {tuned}")
     b = Bench(args, det, field, events, wl_opts, tuned)
@@ -692,7 +693,8 @@ def main() -> int:
     elif tuned:
         line["config"]["scheduling"] = {
             "fp64": tuned[capi.PRECISION_FP64], "mixed": tuned[capi.PRECISION_MIXED],
-            "tuned_on": (f"the batch ({n} tracks, acts_amd.autotune, one fit at a time)"
+            "tuned_on": (f"the batch ({n} tracks, acts_amd.autotune, {args.streams} fits "
+                         "in flight as in the timed steps)"
                          if autotuned else "given (--scheduling)")}
     if autotuned:
         b.tuned = {}
