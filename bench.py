@@ -467,7 +467,7 @@ def ragged_section(args):
         if name == "mixed" and args.no_mixed:
             continue
         o = dataclasses.replace(opts, precision=prec)
-        tuned, _ = autotune(det, field, events, o, cache_path="", in_flight=2)
+        tuned, _ = autotune(det, field, events, o, cache_path="", in_flight=2, steps=2)
         log(f"ragged {name}: tuned {tuned}")
         b = Bench(args, det, field, events, o, dict(tuned, use_graph=True))
         ms, _, _, _ = b.run(prec, streams=2, steps=max(3, args.steps // 4))
