@@ -397,6 +397,65 @@ __global__ __launch_bounds__(kScanBlock) void exclusiveScanKernel(int32_t* a, in
   }
 }
 
+// The scan of the whole histogram array in three passes over chunks of
+// kScanChunk entries, so that many workgroups read it coalesced: the sum of
+// every chunk (chunkSumKernel), the exclusive scan of those sums
+// (exclusiveScanKernel, one workgroup over a few hundred entries), then each
+// chunk scanned in LDS from its sum's offset (chunkScanKernel).
+constexpr int kScanChunk = 4096;
+constexpr int kScanThreads = 256;
+__global__ __launch_bounds__(kScanThreads) void chunkSumKernel(const int32_t* a, int64_t count,
+                                                               int32_t* sums) {
+  __shared__ int32_t red[kScanThreads];
+  const int64_t lo = static_cast<int64_t>(blockIdx.x) * kScanChunk;
+  const int64_t hi = min<int64_t>(count, lo + kScanChunk);
+  int32_t sum = 0;
+  for (int64_t i = lo + threadIdx.x; i < hi; i += kScanThreads) sum += a[i];
+  red[threadIdx.x] = sum;
+  __syncthreads();
+  for (int w = kScanThreads / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(kScanThreads) void chunkScanKernel(int32_t* a, int64_t count,
+                                                                const int32_t* offsets) {
+  constexpr int kPer = kScanChunk / kScanThreads;
+  __shared__ int32_t v[kScanChunk];
+  __shared__ int32_t part[kScanThreads];
+  const int64_t lo = static_cast<int64_t>(blockIdx.x) * kScanChunk;
+  const int n = static_cast<int>(min<int64_t>(kScanChunk, count - lo));
+  for (int i = threadIdx.x; i < n; i += kScanThreads) v[i] = a[lo + i];
+  __syncthreads();
+  const int first = static_cast<int>(threadIdx.x) * kPer;
+  int32_t sum = 0;
+  for (int j = 0; j < kPer; ++j)
+    if (first + j < n) sum += v[first + j];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t acc = offsets[blockIdx.x];
+    for (int t = 0; t < kScanThreads; ++t) {
+      const int32_t c = part[t];
+      part[t] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+  int32_t acc = part[threadIdx.x];
+  for (int j = 0; j < kPer; ++j) {
+    if (first + j < n) {
+      const int32_t c = v[first + j];
+      v[first + j] = acc;
+      acc += c;
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += kScanThreads) a[lo + i] = v[i];
+}
+
 __global__ __launch_bounds__(kSortBlock) void keyScatterKernel(const int32_t* keys, int64_t n,
                                                                int32_t nKeys,
                                                                const int32_t* blockOffset,
@@ -1021,7 +1080,10 @@ static WorkspaceSizes workspaceSizes(const acts_kf_ctx* c, int64_t n, int32_t ma
   w.sortBlocks = (n + kSortTile - 1) / kSortTile;
   // the queue permutation, then the tracks' sort keys
   w.perm = w.sorted ? sizeof(int32_t) * 2 * n : 0;
-  w.hist = w.sorted ? sizeof(int32_t) * w.nKeys * w.sortBlocks : 0;
+  // the histogram, then one sum per scan chunk
+  w.hist = w.sorted ? sizeof(int32_t) * (w.nKeys * w.sortBlocks +
+                                         (w.nKeys * w.sortBlocks + kScanChunk - 1) / kScanChunk)
+                    : 0;
   w.rev = c->revEnabled
               ? (sizeof(double) * kRevRecord + sizeof(int32_t)) * static_cast<size_t>(n) + 256
               : 0;
@@ -1169,8 +1231,14 @@ static int launchFit(acts_kf_ctx* c, const acts_kf_batch_in* in,
     hipLaunchKernelGGL(queueKeyKernel, dim3(nbKey), dim3(kSortBlock), 0, stream, qk, n, keys);
     hipLaunchKernelGGL(keyHistogramKernel, dim3(nb), dim3(kSortBlock), 0, stream, keys, n,
                        nKeys, hist);
-    hipLaunchKernelGGL(exclusiveScanKernel, dim3(1), dim3(kScanBlock), 0, stream, hist,
-                       static_cast<int64_t>(nKeys) * sortBlocks);
+    const int64_t count = static_cast<int64_t>(nKeys) * sortBlocks;
+    const int64_t chunks = (count + kScanChunk - 1) / kScanChunk;
+    int32_t* sums = hist + count;
+    hipLaunchKernelGGL(chunkSumKernel, dim3(static_cast<unsigned>(chunks)), dim3(kScanThreads), 0,
+                       stream, hist, count, sums);
+    hipLaunchKernelGGL(exclusiveScanKernel, dim3(1), dim3(kScanBlock), 0, stream, sums, chunks);
+    hipLaunchKernelGGL(chunkScanKernel, dim3(static_cast<unsigned>(chunks)), dim3(kScanThreads), 0,
+                       stream, hist, count, sums);
     hipLaunchKernelGGL(keyScatterKernel, dim3(nb), dim3(kSortBlock), 0, stream, keys, n,
                        nKeys, hist, perm);
   }
